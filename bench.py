@@ -1,0 +1,194 @@
+"""Benchmark of the ray-acceleration and radiance path on one MI355X.
+
+Workload (one step): the 65,536-triangle soup + emissive quad (65,538 shapes, seed 1), rebuilt as an
+LBVH from device-resident shapes (bounds + Morton, onesweep sort, Karras topology, refit) and
+rendered at 512x512, 8 spp, MAX_DEPTH 5 (6 path segments) with BVH closest-hit and any-hit shadow
+queries.  `value` is Mrays/s = (closest-hit + any-hit queries) / wall time of the timed steps,
+build included.  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+W = H = 512
+SPP = 8
+DEPTH = 5
+N_TRI = 65536
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
+    """The reference's own host path (geomfunc.h RadiancePathTracing, BVH mode, brute-force shadow
+    rays), built from the reference sources into oracle/_ref/, timed on this host's cores on a
+    pixel subset; its query count comes from the oracle in host-twin semantics, which follows the
+    same paths bit for bit.  Falls back to the oracle itself when oracle/_ref was not built."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    from ptgpu_pc_amd.layout import default_seeds
+    threads = min(os.cpu_count() or 1, 16)
+    ref = O.ref_fn("ref_render", 1, DEPTH)
+    kind = "reference" if ref is not None else "port"
+    rng = np.random.default_rng(7)
+    npix = 64
+    elapsed = 0.0
+    total_pix = 0
+    rays = 0
+    t_all = time.perf_counter()
+    while elapsed < budget_s and time.perf_counter() - t_all < 3 * budget_s:
+        pix = np.sort(rng.choice(W * H, npix, replace=False)).astype(np.int32)
+        colors = np.zeros(W * H * 3, np.float32)
+        seeds = default_seeds(W, H)
+        t0 = time.perf_counter()
+        if ref is not None:
+            ref(colors.ctypes.data, seeds.ctypes.data, cam.ctypes.data, shapes.ctypes.data, len(shapes), lc,
+                btn.ctypes.data, btl.ctypes.data, 0, 0, W, H, 0, 1, pix.ctypes.data, len(pix), threads)
+        else:
+            O.radiance_frame(colors, seeds, cam, shapes, lc, O.accel_bvh(btn, btl, quirks=1), W, H, 0, 1, None,
+                             DEPTH, sem=1, pix_list=pix, threads=threads)
+        elapsed += time.perf_counter() - t0
+        c2 = np.zeros(W * H * 3, np.float32)
+        s2 = default_seeds(W, H)
+        q = O.radiance_frame(c2, s2, cam, shapes, lc, O.accel_bvh(btn, btl, quirks=1), W, H, 0, 1, None, DEPTH,
+                             sem=1, pix_list=pix, threads=threads)
+        rays += q[0] + q[1]
+        total_pix += npix
+        npix = min(npix * 2, 4096)
+    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
+            "sample": f"{total_pix} pixels x 1 spp of the same 512x512 frame (BVH closest hit, brute-force "
+                      f"shadow rays over 65,538 shapes), {elapsed:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by scene or "
+              f"pixel tile, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import default_seeds
+
+    shapes, lc, cam = scenes.soup(N_TRI, seed=1, width=W, height=H)
+    pt.update_camera(cam)
+    n = len(shapes)
+    ctx = pt.Context(0)
+    d_shapes = ctx.to_device(shapes)
+    d_cam = ctx.to_device(cam)
+    d_col = ctx.alloc(W * H * 12, np.float32).zero()
+    seeds0 = ctx.to_device(default_seeds(W, H))
+    d_seeds = ctx.alloc(W * H * 8, np.uint32)
+    d_pix = ctx.alloc(W * H * 4, np.uint32)
+    d_rays = ctx.alloc(16, np.uint64).zero()
+    d_stats = ctx.alloc(16, np.uint64).zero()
+    bvh = pt.CLBVH(ctx, d_shapes, n)
+
+    def step(ray_counter=None, trav_stats=None):
+        ctx.record(0)
+        bvh.reset()
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        d_seeds.copy_from(seeds0)
+        ctx.record(1)
+        pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_shapes, shapeCnt=n,
+                  lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=d_pix,
+                  ray_counter=ray_counter, trav_stats=trav_stats)
+        ctx.record(2)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    # one counting frame (untimed) for the roofline's algorithmic bytes
+    step(trav_stats=d_stats)
+    ctx.synchronize()
+    node_visits, prim_tests = (int(x) for x in d_stats.download(np.uint64)[:2])
+
+    d_rays.zero()
+    ctx.synchronize()
+    render_ms, build_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(ray_counter=d_rays)
+        build_ms.append(ctx.elapsed_ms(0, 1))
+        render_ms.append(ctx.elapsed_ms(1, 2))
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    closest, anyhit = (int(x) for x in d_rays.download(np.uint64)[:2])
+    rays = closest + anyhit
+    stage = bvh.stage_times()
+    mrays = rays / wall / 1e6
+    render_avg = float(np.mean(render_ms))
+    build_avg = float(np.mean(build_ms))
+
+    # algorithmic bytes of one render launch: two 24-B child boxes per node visit, 36 B of vertices
+    # per triangle test, and the frame buffers (colors 12 B read+write, seeds 8 B read+write,
+    # pixels 4 B write per pixel)
+    alg_bytes = 48 * node_visits + 36 * prim_tests + W * H * (24 + 16 + 4)
+    achieved = alg_bytes / (render_avg * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_render (path-tracing megakernel)",
+                "per_launch": {"node_visits": node_visits, "prim_tests": prim_tests, "alg_bytes": alg_bytes,
+                               "avg_ms": round(render_avg, 4)}}
+
+    cpu = None
+    if not args.no_cpu:
+        btn, btl = bvh.getTrees()
+        s_host = d_shapes.download(dtype=shapes.dtype, shape=(n,))
+        cpu = cpu_baseline(s_host, lc, cam, btn, btl)
+
+    out = {
+        "metric": "Mrays/s + LBVH build Mtri/s; served p50/p99 frame latency vs plain GPU path",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "LBVH build + BVH traversal: 65,536-triangle soup (seed 1) + emissive quad, "
+                               "512x512, 8 spp, depth 5, 30-bit Morton, one leaf per primitive",
+                   "shapes": n, "width": W, "height": H, "spp": SPP, "max_depth": DEPTH},
+        "rays_per_step": rays // args.steps,
+        "closest_per_step": closest // args.steps,
+        "anyhit_per_step": anyhit // args.steps,
+        "render_ms": round(render_avg, 3),
+        "build_ms": round(build_avg, 4),
+        "build_mtri_s": round(n / (build_avg * 1e-3) / 1e6, 1),
+        "build_stage_ms": {k: round(v, 4) for k, v in stage.items()},
+        "msamples_s": round(W * H * SPP * args.steps / wall / 1e6, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    bvh.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

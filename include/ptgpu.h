@@ -1,0 +1,255 @@
+/*
+ * ptgpu.h -- C ABI of the MI355X (gfx950) ray-acceleration and radiance library.
+ *
+ * Every structure below keeps the byte layout of the PTGPU_PC reference so that buffers produced by
+ * the reference's host code (Shape arrays, Camera, BVHNodeGPU/KDNodeGPU trees, seed and colour
+ * buffers) can be handed to this library unchanged.  Every entry point returns an int status
+ * (PTGPU_OK == 0); nothing throws across the boundary.  ptgpu_last_error() returns a readable
+ * message for the last failure on the calling thread.
+ *
+ * Pointer conventions: arguments named d_* are device (HBM) pointers, h_* are host pointers.
+ * Work is ordered on the context's HIP stream; calls that return host data synchronise it.
+ * A context is single-threaded: one host thread drives one context at a time.
+ */
+#ifndef PTGPU_H
+#define PTGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------ */
+/* Layout types (bit-identical to the reference).                                              */
+/* ------------------------------------------------------------------------------------------ */
+
+/* vec.h:27-29 */
+typedef struct { float x, y, z; } Vec;
+/* geom.h:48-50 */
+typedef struct { Vec o, d; } Ray;
+/* geom.h:55-57 */
+enum Refl { DIFF = 0, SPEC = 1, REFR = 2 };
+/* geom.h:59-62 */
+typedef struct { float rad; Vec p; } Sphere;
+/* geom.h:68-70 */
+typedef struct { Vec p1, p2, p3; } Triangle;
+/* geom.h:72 */
+typedef enum tagType { SPHERE = 0, TRIANGLE = 1 } Type;
+/* geom.h:73 -- min and max are interleaved per axis */
+typedef struct { float min_x, max_x, min_y, max_y, min_z, max_z; } Bound;
+/* geom.h:75-91 -- 104 bytes */
+typedef struct {
+    Type type;
+    union { Triangle t; Sphere s; };
+    Vec e, c;            /* emission, colour */
+    enum Refl refl;
+    int index;
+    unsigned int morton_code;
+    Bound b;
+    float area;
+} Shape;
+/* camera.h:29-38 -- 76 bytes */
+typedef struct {
+    Vec orig, target;
+    float pitch, yaw;
+    float width, height;
+    Vec dir, x, y;
+} Camera;
+/* BVHNodeGPU.h:4-16 -- 60 bytes */
+typedef struct BVHNodeGPU {
+    int min, max;
+    int nShape;
+    int leaf;
+    int nLeft, nRight, nParent;
+    Bound bound;
+    float cost;
+    float area;
+} BVHNodeGPU;
+/* KDNodeGPU.h:4-14 -- 36 bytes (int16 indices) */
+typedef struct KDNodeGPU {
+    short leaf;
+    short nLeft, nRight, nParent;
+    Bound bound;
+    short min;
+    short max;
+} KDNodeGPU;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Status codes                                                                                */
+/* ------------------------------------------------------------------------------------------ */
+enum {
+    PTGPU_OK = 0,
+    PTGPU_ERR_INVALID = 1,      /* bad argument (null pointer, size out of range, wrong state) */
+    PTGPU_ERR_HIP = 2,          /* a HIP runtime call failed */
+    PTGPU_ERR_OOM = 3,          /* device allocation failed or admission budget exceeded */
+    PTGPU_ERR_UNSUPPORTED = 4,  /* the request cannot be represented (e.g. int16 KD export overflow) */
+    PTGPU_ERR_NO_DEVICE = 5     /* no HIP device is visible */
+};
+
+/* Acceleration modes: the reference's compile-time ACCELSTR (geom.h:35) as a runtime value. */
+enum { PTGPU_ACCEL_NONE = 0, PTGPU_ACCEL_BVH = 1, PTGPU_ACCEL_KD = 2 };
+
+typedef struct ptgpu_ctx ptgpu_ctx;
+typedef struct ptgpu_clbvh ptgpu_clbvh;
+typedef struct ptgpu_kdtree ptgpu_kdtree;
+
+/* ------------------------------------------------------------------------------------------ */
+/* Library, context, memory                                                                    */
+/* ------------------------------------------------------------------------------------------ */
+
+/* Size of each layout type as compiled into the library (layout self-check for FFI callers). */
+int ptgpu_layout_sizes(int *shape, int *camera, int *bvhnode, int *kdnode, int *ray, int *bound);
+const char *ptgpu_last_error(void);
+int ptgpu_device_count(int *count);
+
+/* Replaces the OpenCL context/queue pair made in SetUpOpenCL (smallptGPU.cpp:286-441).
+ * `stream` may be NULL (the library creates a default-priority stream) or a hipStream_t. */
+int ptgpu_ctx_create(int device, void *stream, ptgpu_ctx **out);
+int ptgpu_ctx_destroy(ptgpu_ctx *ctx);
+int ptgpu_synchronize(ptgpu_ctx *ctx);
+/* MAX_DEPTH (geom.h:32) used by ptgpu_radiance_gpu: the reference fixes it at compile time (6);
+ * here it is a per-context value, default 6. */
+int ptgpu_ctx_set_max_depth(ptgpu_ctx *ctx, int max_depth);
+int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);
+int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr);
+int ptgpu_memcpy_h2d(ptgpu_ctx *ctx, void *d_dst, const void *h_src, size_t bytes); /* synchronous */
+int ptgpu_memcpy_d2h(ptgpu_ctx *ctx, void *h_dst, const void *d_src, size_t bytes); /* synchronous */
+int ptgpu_memset_d(ptgpu_ctx *ctx, void *d_dst, int value, size_t bytes);          /* stream-ordered */
+int ptgpu_memcpy_d2d(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
+/* Host buffers returned by ptgpu_kdtree_get_trees are released with this. */
+void ptgpu_host_free(void *h_ptr);
+
+/* Stream-ordered timing: record a named event slot (0..15) and read the elapsed ms between two. */
+int ptgpu_event_record(ptgpu_ctx *ctx, int slot);
+int ptgpu_event_elapsed_ms(ptgpu_ctx *ctx, int slot_begin, int slot_end, float *ms);
+
+/* ------------------------------------------------------------------------------------------ */
+/* LBVH builder -- replaces class CLBVH (CLBVH.h:39-47, CLBVH.cpp:80-366)                      */
+/* ------------------------------------------------------------------------------------------ */
+
+/* CLBVH::CLBVH (CLBVH.cpp:80-181).  Uploads h_shapes, computes per-shape AABBs, scene centroid
+ * bounds and 30-bit Morton codes on the GPU, and -- as the reference does -- writes `index`,
+ * `morton_code` and `b` back into the caller's h_shapes array.  n >= 1. */
+int ptgpu_clbvh_create(ptgpu_ctx *ctx, Shape *h_shapes, int n, ptgpu_clbvh **out);
+/* Same, with a device-resident Shape array (no PCIe traffic).  d_shapes is updated in place
+ * (index, morton_code, b) and must stay valid for the lifetime of the handle. */
+int ptgpu_clbvh_create_device(ptgpu_ctx *ctx, Shape *d_shapes, int n, ptgpu_clbvh **out);
+/* Re-run the constructor's work (bounds + Morton) on the same device Shape array, e.g. after the
+ * caller moved vertices; keeps allocations.  n must not exceed the handle's capacity. */
+int ptgpu_clbvh_reset(ptgpu_clbvh *h, int n);
+/* CLBVH::buildRadixTree (CLBVH.cpp:207-238; kernel BVH.cl:141-208). */
+int ptgpu_clbvh_build_radix_tree(ptgpu_clbvh *h);
+/* CLBVH::buildBVHTree (CLBVH.cpp:245-311; kernel BVH.cl:214-258): stable descending sort by Morton
+ * key, leaf fill and bottom-up bound refit. */
+int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h);
+/* CLBVH::optimize (CLBVH.cpp:318-355; kernel BVH.cl:514-543): treelet restructuring with up to
+ * `treelet_size` (2..7) leaves per treelet, SAH constants Ci=1.2, Ct=1.0 (clheader.h:14-15). */
+int ptgpu_clbvh_optimize(ptgpu_clbvh *h, int treelet_size);
+/* CLBVH::getTrees (CLBVH.cpp:357-366): blocking readback into host mirrors owned by the handle
+ * (borrowed pointers, valid until the next build call or destroy). */
+int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl);
+/* Zero-copy access to the device trees (N-1 internal nodes, N leaves). */
+int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl);
+/* Device pointer to sorted_geometry_indices (CLBVH.cpp:257), N ints. */
+int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted);
+/* Radix-sort digit width for the next build_bvh_tree (4 or 8 bits; default 8). */
+int ptgpu_clbvh_set_digit_bits(ptgpu_clbvh *h, int bits);
+/* Per-stage device times of the last build in ms: morton, sort, topology, refit, optimize. */
+int ptgpu_clbvh_stage_times(ptgpu_clbvh *h, float *ms5);
+int ptgpu_clbvh_destroy(ptgpu_clbvh *h);
+
+/* ------------------------------------------------------------------------------------------ */
+/* KD tree -- replaces class KDTree (KDTree.h:88-89, KDTree.cpp:89-270)                        */
+/* ------------------------------------------------------------------------------------------ */
+
+/* KDTree::build (KDTree.cpp:89-182): the reference's mean-split object partition, built on the
+ * host with the reference's floating-point order; also prepares the device traversal layout.
+ * ctx may be NULL: the tree is then built on the host only (exports work, rendering does not). */
+int ptgpu_kdtree_build(ptgpu_ctx *ctx, const Shape *h_shapes, int n, ptgpu_kdtree **out);
+/* SAH variant with a depth cap (extension; no reference counterpart in the mean-split builder). */
+int ptgpu_kdtree_build_sah(ptgpu_ctx *ctx, const Shape *h_shapes, int n, int max_depth, int max_leaf,
+                           ptgpu_kdtree **out);
+/* KDTree::getTrees (KDTree.cpp:257-270): BFS heap export of 2^(maxdepth+1) KDNodeGPU plus the
+ * primitive list.  Ownership of both buffers passes to the caller (free with ptgpu_host_free).
+ * Returns PTGPU_ERR_UNSUPPORTED when the tree does not fit the int16 fields of KDNodeGPU. */
+int ptgpu_kdtree_get_trees(ptgpu_kdtree *kd, KDNodeGPU **h_kng, int *kngCnt, int **h_kn, int *knCnt);
+/* Depth-first export of the tree in int32 form: node5[k] = {leaf, left, right, start, count} and
+ * box[k] for node k (pre-order, left before right); kn lists shape indices of the leaves in order. */
+int ptgpu_kdtree_export_dfs(ptgpu_kdtree *kd, int *node5, Bound *box, int *kn, int cap_nodes, int cap_kn);
+/* Tree statistics: max depth, number of nodes, number of leaves, primitive-list length. */
+int ptgpu_kdtree_stats(ptgpu_kdtree *kd, int *max_depth, int *nodes, int *leaves, int *kn_len);
+int ptgpu_kdtree_destroy(ptgpu_kdtree *kd);
+
+/* ------------------------------------------------------------------------------------------ */
+/* Rendering -- replaces RadianceGPU (rendering_kernel.cl:1072-1150) and DrawFrame             */
+/* (smallptGPU.cpp:1010-1087)                                                                   */
+/* ------------------------------------------------------------------------------------------ */
+
+/* The RadianceGPU argument list in its original order (smallptGPU.cpp:1013-1031).  All pointers
+ * are device pointers.  btn/btl are used when accel == PTGPU_ACCEL_BVH, kng/kngCnt/kn/knCnt when
+ * accel == PTGPU_ACCEL_KD. */
+typedef struct {
+    Vec *colors;
+    unsigned int *seeds;
+    const Camera *camera;
+    const Shape *shapes;
+    unsigned int shapeCnt;
+    unsigned int lightCnt;
+    const BVHNodeGPU *btn;
+    const BVHNodeGPU *btl;
+    const KDNodeGPU *kng;
+    int kngCnt;
+    const int *kn;
+    int knCnt;
+    int width;
+    int height;
+    int currentSample;
+    unsigned int *pixels;
+} ptgpu_radiance_args;
+
+/* One RadianceGPU launch: one sample per pixel, running-mean accumulation into colors, RGBA8 pack
+ * into pixels, seeds written back.  Trees are read in the reference layout. */
+int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *args);
+
+/* Fast path: render `spp` consecutive samples (currentSample .. currentSample+spp-1) in one
+ * launch, traversing the handle's prepared device layout.  Results are identical to `spp`
+ * successive ptgpu_radiance_gpu calls.  Exactly one of bvh/kd may be non-NULL, or both NULL for
+ * the brute-force mode.  d_camera is a device Camera; d_shapes the device Shape array. */
+typedef struct {
+    Vec *colors;
+    unsigned int *seeds;
+    const Camera *camera;
+    const Shape *shapes;
+    unsigned int shapeCnt;
+    unsigned int lightCnt;
+    int width;
+    int height;
+    int currentSample;
+    int spp;
+    int max_depth;            /* MAX_DEPTH (geom.h:32); paths have max_depth+1 segments */
+    unsigned int *pixels;     /* may be NULL: skip the RGBA8 pack */
+    unsigned long long *ray_counter; /* may be NULL: device u64[2] += {closest, any-hit} queries */
+    unsigned long long *trav_stats;  /* may be NULL: device u64[2] += {node visits, primitive tests};
+                                        selects a counting variant of the kernel (slower) */
+} ptgpu_render_args;
+int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *args);
+
+/* Batch ray queries over the same traversal engine (the reference's Intersect and IntersectP,
+ * rendering_kernel.cl:194-395).  d_rays: n Rays.  Closest: d_t[i], d_id[i] (id = original shape
+ * index, t = 0 and id = -1 on a miss; ties in t resolve to the higher index).  Any-hit: d_hit[i] =
+ * 1 if a shape with index < shapeCnt-1 is hit at 0 != t < d_maxt[i]. */
+int ptgpu_trace_closest(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes,
+                        unsigned int shapeCnt, const Ray *d_rays, int n, float *d_t, int *d_id);
+int ptgpu_trace_any(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes,
+                    unsigned int shapeCnt, const Ray *d_rays, const float *d_maxt, int n, int *d_hit);
+
+/* UpdateCamera (displayfunc.cpp:732-756) on the host: fills dir/x/y from yaw, pitch, width, height. */
+int ptgpu_update_camera(Camera *cam);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PTGPU_H */

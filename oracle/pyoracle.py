@@ -1,0 +1,261 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and to the reference's own host path built
+from the reference sources (oracle/_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by the CPU-baseline
+leg of bench.py -- never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "liboracle.so"
+REF_SO = HERE / "_ref" / "libref.so"
+
+vp, i32, u32, f32 = C.c_void_p, C.c_int, C.c_uint, C.c_float
+
+
+class OrAccel(C.Structure):
+    _fields_ = [("mode", i32), ("btn", vp), ("btl", vp), ("kng", vp), ("kn", vp), ("kd_node5", vp),
+                ("kd_box", vp), ("quirks", i32)]
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    if a.dtype.names and "morton_code" in a.dtype.names and a.dtype.itemsize != 104:
+        raise ValueError("shape array is not in the 104-byte Shape layout")
+    return a.ctypes.data
+
+
+_OR = None
+_REF = None
+
+
+def lib():
+    global _OR
+    if _OR is None:
+        if not ORACLE_SO.exists():
+            raise RuntimeError(f"{ORACLE_SO} missing (make -C oracle)")
+        L = C.CDLL(str(ORACLE_SO))
+        sig = {
+            "or_get_random": (f32, [vp, vp]),
+            "or_clbvh_construct": (None, [vp, i32, vp]),
+            "or_bubblesort": (None, [vp, i32, vp]),
+            "or_stable_sort_desc": (None, [vp, i32, vp]),
+            "or_radix_tree": (None, [i32, vp, vp]),
+            "or_topology_closed_form": (None, [i32, vp, vp, vp, vp, vp, vp]),
+            "or_refit": (None, [vp, vp, i32, vp, vp]),
+            "or_kd_build": (vp, [vp, i32]),
+            "or_kd_free": (None, [vp]),
+            "or_kd_stats": (None, [vp, vp, vp, vp, vp]),
+            "or_kd_flatten_dfs": (i32, [vp, vp, vp, vp, i32, C.c_longlong]),
+            "or_kd_export_heap": (i32, [vp, vp, i32, vp, C.c_longlong]),
+            "or_sphere_intersect": (f32, [vp, vp]),
+            "or_triangle_intersect": (f32, [vp, vp, i32]),
+            "or_bound_test": (i32, [vp, vp]),
+            "or_intersect": (i32, [vp, u32, C.POINTER(OrAccel), vp, vp, vp, i32]),
+            "or_intersect_p": (i32, [vp, u32, vp, f32, i32]),
+            "or_radiance_frame": (None, [vp, vp, vp, vp, u32, u32, C.POINTER(OrAccel), i32, i32, i32, i32, vp,
+                                         i32, i32, vp, i32, i32, vp]),
+            "or_trace_closest": (None, [vp, u32, C.POINTER(OrAccel), vp, i32, vp, vp, i32, i32]),
+            "or_trace_any": (None, [vp, u32, vp, vp, i32, vp, i32, i32]),
+            "or_update_camera": (None, [vp]),
+            "or_accel_size": (C.c_size_t, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        if L.or_accel_size() != C.sizeof(OrAccel):
+            raise RuntimeError("OrAccel layout mismatch")
+        _OR = L
+    return _OR
+
+
+def ref_lib():
+    """The reference's geomfunc.h host path; None when it was not built (no reference tree)."""
+    global _REF
+    if _REF is None and REF_SO.exists():
+        _REF = C.CDLL(str(REF_SO))
+    return _REF
+
+
+_REF_SIGS = {
+    "ref_get_random": (f32, [vp, vp]),
+    "ref_sphere_intersect": (f32, [vp, vp]),
+    "ref_triangle_intersect": (f32, [vp, vp]),
+    "ref_bound_test": (i32, [vp, vp]),
+    "ref_intersect": (i32, [vp, u32, vp, vp, i32, i32, vp, vp, vp]),
+    "ref_intersect_p": (i32, [vp, u32, vp, f32]),
+    "ref_radiance_path": (None, [vp, u32, u32, vp, vp, i32, i32, vp, vp, vp, vp]),
+    "ref_render": (None, [vp, vp, vp, vp, u32, u32, vp, vp, i32, i32, i32, i32, i32, i32, vp, i32, i32]),
+}
+
+
+def ref_fn(name: str, accel: int, depth: int):
+    L = ref_lib()
+    if L is None:
+        return None
+    fn = getattr(L, f"{name}_a{accel}_d{depth}")
+    res, args = _REF_SIGS[name]
+    fn.restype = res
+    fn.argtypes = args
+    return fn
+
+
+# ------------------------------------------------------------------------------------------
+def get_random(s0: int, s1: int, count: int):
+    a, b = C.c_uint(s0), C.c_uint(s1)
+    out = np.empty(count, dtype=np.float32)
+    for k in range(count):
+        out[k] = lib().or_get_random(C.byref(a), C.byref(b))
+    return out, a.value, b.value
+
+
+def clbvh_construct(shapes: np.ndarray):
+    """Returns (shapes copy with index/b/morton_code written, scene6)."""
+    s = np.ascontiguousarray(shapes).copy()
+    scene = np.empty(6, dtype=np.float32)
+    lib().or_clbvh_construct(_p(s), len(s), _p(scene))
+    return s, scene
+
+
+def bubblesort(keys: np.ndarray) -> np.ndarray:
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    out = np.empty(len(k), dtype=np.int32)
+    lib().or_bubblesort(_p(k), len(k), _p(out))
+    return out
+
+
+def stable_sort_desc(keys: np.ndarray) -> np.ndarray:
+    k = np.ascontiguousarray(keys, dtype=np.uint32)
+    out = np.empty(len(k), dtype=np.int32)
+    lib().or_stable_sort_desc(_p(k), len(k), _p(out))
+    return out
+
+
+def radix_tree(n: int):
+    from ptgpu_pc_amd.layout import BVHNODE_DTYPE
+    btn = np.zeros(max(n - 1, 0), dtype=BVHNODE_DTYPE)
+    btl = np.zeros(n, dtype=BVHNODE_DTYPE)
+    lib().or_radix_tree(n, _p(btn) if n > 1 else None, _p(btl))
+    return btn, btl
+
+
+def topology_closed_form(n: int):
+    m = max(n - 1, 1)
+    left, right, lo, hi, pint = (np.full(m, -7, dtype=np.int32) for _ in range(5))
+    pleaf = np.full(n, -7, dtype=np.int32)
+    lib().or_topology_closed_form(n, _p(left), _p(right), _p(lo), _p(hi), _p(pint), _p(pleaf))
+    return left, right, lo, hi, pint, pleaf
+
+
+def refit(shapes: np.ndarray, sorted_idx: np.ndarray, btn: np.ndarray, btl: np.ndarray):
+    btn = btn.copy()
+    btl = btl.copy()
+    s = np.ascontiguousarray(sorted_idx, dtype=np.int32)
+    lib().or_refit(_p(np.ascontiguousarray(shapes)), _p(s), len(btl), _p(btn) if len(btn) else None, _p(btl))
+    return btn, btl
+
+
+def build_lbvh(shapes: np.ndarray, literal_sort: bool = False):
+    """Full reference LBVH build: (shapes with Morton fields, sorted, btn, btl)."""
+    s, _ = clbvh_construct(shapes)
+    keys = s["morton_code"].copy()
+    srt = bubblesort(keys) if literal_sort else stable_sort_desc(keys)
+    btn, btl = radix_tree(len(s))
+    btn, btl = refit(s, srt, btn, btl)
+    return s, srt, btn, btl
+
+
+class KD:
+    """The oracle's KDTree::build result."""
+
+    def __init__(self, shapes: np.ndarray):
+        self.shapes = np.ascontiguousarray(shapes).copy()
+        self.shapes["index"] = np.arange(len(shapes), dtype=np.int32)   # BuildKDtree, smallptGPU.cpp:1482
+        self.h = lib().or_kd_build(_p(self.shapes), len(self.shapes))
+
+    def stats(self):
+        md, nodes, leaves = i32(), i32(), i32()
+        sz = C.c_longlong()
+        lib().or_kd_stats(self.h, C.byref(md), C.byref(sz), C.byref(nodes), C.byref(leaves))
+        return dict(max_depth=md.value, kn_len=sz.value, nodes=nodes.value, leaves=leaves.value)
+
+    def flatten_dfs(self):
+        st = self.stats()
+        node5 = np.empty((st["nodes"], 5), dtype=np.int32)
+        box = np.empty((st["nodes"], 6), dtype=np.float32)
+        kn = np.empty(max(st["kn_len"], 1), dtype=np.int32)
+        cnt = lib().or_kd_flatten_dfs(self.h, _p(node5), _p(box), _p(kn), st["nodes"], len(kn))
+        assert cnt == st["nodes"], cnt
+        return node5, box, kn[:st["kn_len"]]
+
+    def export_heap(self):
+        from ptgpu_pc_amd.layout import KDNODE_DTYPE
+        st = self.stats()
+        size = 1 << (st["max_depth"] + 1)
+        heap = np.zeros(size, dtype=KDNODE_DTYPE)
+        kn = np.empty(max(st["kn_len"], 1), dtype=np.int32)
+        r = lib().or_kd_export_heap(self.h, _p(heap), size, _p(kn), len(kn))
+        assert r == size, r
+        return heap, kn[:st["kn_len"]]
+
+    def __del__(self):
+        try:
+            lib().or_kd_free(self.h)
+        except Exception:
+            pass
+
+
+def accel_none():
+    return OrAccel(0, None, None, None, None, None, None, 0)
+
+
+def accel_bvh(btn, btl, quirks=0):
+    return OrAccel(1, _p(btn) if len(btn) else None, _p(btl), None, None, None, None, quirks)
+
+
+def accel_kd_heap(kng, kn, quirks=0):
+    return OrAccel(2, None, None, _p(kng), _p(kn), None, None, quirks)
+
+
+def accel_kd_dfs(node5, box, kn):
+    return OrAccel(3, None, None, None, _p(kn), _p(node5), _p(box), 0)
+
+
+def trace_closest(shapes, acc: OrAccel, rays, sem=0, threads=8):
+    t = np.empty(len(rays), dtype=np.float32)
+    ids = np.empty(len(rays), dtype=np.int32)
+    lib().or_trace_closest(_p(shapes), len(shapes), C.byref(acc), _p(rays), len(rays), _p(t), _p(ids), sem, threads)
+    return t, ids
+
+
+def trace_any(shapes, rays, maxt, sem=0, threads=8):
+    hit = np.empty(len(rays), dtype=np.int32)
+    lib().or_trace_any(_p(shapes), len(shapes), _p(rays), _p(np.ascontiguousarray(maxt, dtype=np.float32)),
+                       len(rays), _p(hit), sem, threads)
+    return hit
+
+
+def radiance_frame(colors, seeds, cam, shapes, light_cnt, acc: OrAccel, width, height, current_sample, spp,
+                   pixels, max_depth, sem=0, pix_list=None, threads=8):
+    """`spp` RadianceGPU passes (in place on colors/seeds/pixels).  Returns (closest, any-hit) counts."""
+    stats = np.zeros(2, dtype=np.uint64)
+    pl = np.ascontiguousarray(pix_list, dtype=np.int32) if pix_list is not None else None
+    lib().or_radiance_frame(_p(colors), _p(seeds), _p(cam), _p(shapes), len(shapes), light_cnt, C.byref(acc),
+                            width, height, current_sample, spp, _p(pixels) if pixels is not None else None,
+                            max_depth, sem, _p(pl) if pl is not None else None, len(pl) if pl is not None else 0,
+                            threads, _p(stats))
+    return int(stats[0]), int(stats[1])
+
+
+def update_camera(cam: np.ndarray) -> np.ndarray:
+    lib().or_update_camera(_p(cam))
+    return cam

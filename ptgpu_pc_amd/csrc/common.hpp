@@ -1,0 +1,140 @@
+// common.hpp -- shared definitions for the gfx950 ray-acceleration and radiance library.
+//
+// Device layouts used by the traversal engine (derived from the reference layouts at build time):
+//   TNode (64 B)  one internal node of a binary tree with BOTH child boxes stored in the parent, so
+//                 one 64-byte fetch decides which children a ray enters.
+//   Prim  (48 B)  one primitive in traversal (leaf) order: triangle v0, e1 = v1-v0, e2 = v2-v0
+//                 (the same FP32 differences TriangleIntersect forms at rendering_kernel.cl:72-74),
+//                 or sphere centre + radius; the original shape index rides in p0.w.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ptgpu.h"
+
+// ---- layout checks against the reference declarations (offsets computed from geom.h etc.) ----
+static_assert(sizeof(Vec) == 12, "Vec");
+static_assert(sizeof(Ray) == 24, "Ray");
+static_assert(sizeof(Bound) == 24, "Bound");
+static_assert(sizeof(Shape) == 104, "Shape");
+static_assert(offsetof(Shape, t) == 4 && offsetof(Shape, e) == 40 && offsetof(Shape, c) == 52, "Shape");
+static_assert(offsetof(Shape, refl) == 64 && offsetof(Shape, index) == 68, "Shape");
+static_assert(offsetof(Shape, morton_code) == 72 && offsetof(Shape, b) == 76, "Shape");
+static_assert(offsetof(Shape, area) == 100, "Shape");
+static_assert(sizeof(Camera) == 76 && offsetof(Camera, dir) == 40 && offsetof(Camera, y) == 64, "Camera");
+static_assert(sizeof(BVHNodeGPU) == 60 && offsetof(BVHNodeGPU, bound) == 28, "BVHNodeGPU");
+static_assert(offsetof(BVHNodeGPU, cost) == 52 && offsetof(BVHNodeGPU, area) == 56, "BVHNodeGPU");
+static_assert(sizeof(KDNodeGPU) == 36 && offsetof(KDNodeGPU, bound) == 8, "KDNodeGPU");
+static_assert(offsetof(KDNodeGPU, min) == 32 && offsetof(KDNodeGPU, max) == 34, "KDNodeGPU");
+
+namespace ptgpu {
+
+constexpr float kEpsilon = 0.01f;                       // geom.h:29
+constexpr float kPi = 3.14159265358979323846f;          // geom.h:30
+
+struct alignas(16) TNode {
+    float4 a;   // c0.min_x c0.min_y c0.min_z c0.max_x
+    float4 b;   // c0.max_y c0.max_z c1.min_x c1.min_y
+    float4 c;   // c1.min_z c1.max_x c1.max_y c1.max_z
+    int4 r;     // ref0 ref1 cnt0 cnt1 : ref >= 0 internal node, ref < 0 leaf with first prim ~ref
+};
+static_assert(sizeof(TNode) == 64, "TNode");
+
+struct alignas(16) Prim {
+    float4 p0;  // v0.xyz | centre.xyz, w = shape index (bits)
+    float4 p1;  // e1.xyz | (rad, 0, 0), w = type (bits)
+    float4 p2;  // e2.xyz
+};
+static_assert(sizeof(Prim) == 48, "Prim");
+
+// A prepared traversal structure on the device.
+struct DevAccel {
+    const TNode *nodes = nullptr;
+    const Prim *prims = nullptr;
+    int root = 0;          // >= 0 internal node; < 0: the whole scene is one leaf ~root
+    int root_cnt = 0;      // primitive count when the root is a leaf
+    int nprims = 0;
+    int depth = 0;         // max internal-node depth (root = 0); stack bound
+    int mode = PTGPU_ACCEL_NONE;
+};
+
+// ---- errors ----
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define PT_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) return ::ptgpu::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define PT_CHECK(cond, code, ...)            \
+    do {                                     \
+        if (!(cond)) {                       \
+            ::ptgpu::set_error(__VA_ARGS__); \
+            return (code);                   \
+        }                                    \
+    } while (0)
+
+#define PT_TRY(expr)                  \
+    do {                              \
+        int _s = (expr);              \
+        if (_s != PTGPU_OK) return _s; \
+    } while (0)
+
+// Growable device buffer owned by a handle.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace ptgpu
+
+struct ptgpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev[16] = {};
+    int cu_count = 0;
+    int max_depth = 6;   // MAX_DEPTH (geom.h:32) for ptgpu_radiance_gpu
+    // scratch for raw-layout renders and light lists
+    ptgpu::DevBuf tmp_nodes, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
+};
+
+namespace ptgpu {
+// light list: indices of the first `light_cnt` shapes with non-zero emission, in index order
+// (the scan order of SampleLights, rendering_kernel.cl:415-423).  Writes *found on the host.
+int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt, DevBuf &out_lights,
+                     DevBuf &scan_tmp, int *found);
+// brute-force layout: prims in shape-index order
+int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_prims);
+// reference-layout trees -> traversal layout
+int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn,
+                    const BVHNodeGPU *d_btl, DevBuf &nodes, DevBuf &prims, DevAccel *acc);
+int convert_kd_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU *d_kng, int kngCnt,
+                   const int *d_kn, int knCnt, DevBuf &nodes, DevBuf &prims, DevAccel *acc);
+// render / trace over a prepared structure
+struct RenderLaunch {
+    Vec *colors;
+    unsigned int *seeds;
+    const Camera *camera;
+    const Shape *shapes;
+    unsigned int shapeCnt;
+    const int *lights;
+    int nlights;
+    int width, height, currentSample, spp, max_depth;
+    unsigned int *pixels;
+    unsigned long long *ray_counter;
+    unsigned long long *trav_stats;
+};
+int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
+int launch_trace(ptgpu_ctx *ctx, const DevAccel &acc, unsigned int shapeCnt, const Ray *d_rays,
+                 const float *d_maxt, int n, float *d_t, int *d_id, int *d_hit, bool any);
+}  // namespace ptgpu

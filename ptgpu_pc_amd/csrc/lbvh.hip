@@ -1,0 +1,768 @@
+// lbvh.hip -- LBVH build for gfx950: scene bounds + 30-bit Morton codes, onesweep LSD radix sort
+// (stable, descending by Morton code), Karras radix-tree topology over leaf indices, bottom-up
+// bound refit with atomic visit flags, and the C ABI that replaces class CLBVH (CLBVH.h:39-47).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "device_util.hpp"
+#include "lbvh.hpp"
+
+namespace ptgpu {
+
+// -------------------------------------------------------------------------------------------
+// utilities
+// -------------------------------------------------------------------------------------------
+__global__ void k_fill_u32(unsigned *p, unsigned v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words) {
+    if (n_words == 0) return PTGPU_OK;
+    size_t blocks = (n_words + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, (unsigned *)p, v, n_words);
+    PT_HIP(hipGetLastError());
+    return PTGPU_OK;
+}
+
+__device__ __forceinline__ float shape_f(const Shape *s, int word) {
+    return reinterpret_cast<const float *>(s)[word];
+}
+__device__ __forceinline__ int shape_i(const Shape *s, int word) {
+    return reinterpret_cast<const int *>(s)[word];
+}
+
+// CLBVH::getBound CLBVH.cpp:49-75 from the Shape words (type @0, p1..p3 / rad,p @1..9).
+__device__ __forceinline__ Bound shape_bound(const Shape *s, int *type_out) {
+    const float *w = reinterpret_cast<const float *>(s);
+    int type = reinterpret_cast<const int *>(s)[0];
+    *type_out = type;
+    Bound b;
+    if (type == SPHERE) {
+        float rad = w[1], px = w[2], py = w[3], pz = w[4];
+        b.min_x = px - rad; b.max_x = px + rad;
+        b.min_y = py - rad; b.max_y = py + rad;
+        b.min_z = pz - rad; b.max_z = pz + rad;
+    } else {
+        float x1 = w[1], y1 = w[2], z1 = w[3], x2 = w[4], y2 = w[5], z2 = w[6], x3 = w[7], y3 = w[8], z3 = w[9];
+        b.min_x = sel_min(sel_min(x1, x2), x3); b.max_x = sel_max(sel_max(x1, x2), x3);
+        b.min_y = sel_min(sel_min(y1, y2), y3); b.max_y = sel_max(sel_max(y1, y2), y3);
+        b.min_z = sel_min(sel_min(z1, z2), z3); b.max_z = sel_max(sel_max(z1, z2), z3);
+    }
+    return b;
+}
+
+// -------------------------------------------------------------------------------------------
+// Stage 1: AABBs, scene centroid bounds, Morton codes (CLBVH.cpp:90-164)
+// -------------------------------------------------------------------------------------------
+constexpr int kBoundsThreads = 256;
+
+__device__ __forceinline__ void reduce6(float *v, float *sh /*6*kBoundsThreads*/) {
+    const int tid = threadIdx.x;
+    for (int k = 0; k < 6; k++) sh[k * kBoundsThreads + tid] = v[k];
+    __syncthreads();
+    for (int s = kBoundsThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            for (int k = 0; k < 6; k++) {
+                float a = sh[k * kBoundsThreads + tid], b = sh[k * kBoundsThreads + tid + s];
+                sh[k * kBoundsThreads + tid] = (k & 1) ? sel_max(a, b) : sel_min(a, b);
+            }
+        }
+        __syncthreads();
+    }
+    for (int k = 0; k < 6; k++) v[k] = sh[k * kBoundsThreads];
+}
+
+// Per shape: bound -> Shape.b, index -> Shape.index, centroid; block partial of the centroid
+// min/max (max starts at FLT_MIN as CLBVH.cpp:92-96 does); the last block to finish folds the
+// partials into scene6 = {min_x, max_x, min_y, max_y, min_z, max_z}.
+__global__ __launch_bounds__(kBoundsThreads) void k_bounds(Shape *shapes, int n, float *partials,
+                                                           unsigned *done_counter, float *scene6) {
+    __shared__ float sh[6 * kBoundsThreads];
+    __shared__ bool s_last;
+    float acc[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        Shape *s = shapes + i;
+        int type;
+        Bound b = shape_bound(s, &type);
+        int *wi = reinterpret_cast<int *>(s);
+        float *wf = reinterpret_cast<float *>(s);
+        wi[17] = i;                                    // index @68
+        wf[19] = b.min_x; wf[20] = b.max_x; wf[21] = b.min_y;   // b @76
+        wf[22] = b.max_y; wf[23] = b.min_z; wf[24] = b.max_z;
+        float px, py, pz;
+        if (type == SPHERE) {
+            px = wf[2]; py = wf[3]; pz = wf[4];
+        } else {
+            px = (b.min_x + b.max_x) / 2.0f;
+            py = (b.min_y + b.max_y) / 2.0f;
+            pz = (b.min_z + b.max_z) / 2.0f;
+        }
+        if (px < acc[0]) acc[0] = px;
+        if (px > acc[1]) acc[1] = px;
+        if (py < acc[2]) acc[2] = py;
+        if (py > acc[3]) acc[3] = py;
+        if (pz < acc[4]) acc[4] = pz;
+        if (pz > acc[5]) acc[5] = pz;
+    }
+    reduce6(acc, sh);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) partials[blockIdx.x * 6 + k] = acc[k];
+        unsigned prev = add_acq_rel(done_counter, 1u);
+        s_last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    float fin[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        for (int k = 0; k < 6; k++) {
+            float v = __hip_atomic_load(partials + b * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            fin[k] = (k & 1) ? sel_max(fin[k], v) : sel_min(fin[k], v);
+        }
+    }
+    reduce6(fin, sh);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 6; k++) scene6[k] = fin[k];
+        *done_counter = 0;   // ready for the next build
+    }
+}
+
+__device__ __forceinline__ unsigned f2u_code(float v) { return v >= 0.f ? (unsigned)v : 0u; }
+
+__device__ __forceinline__ unsigned expand10(unsigned x) {
+    x &= 0x3FFu;
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// CLBVH.cpp:125-164: quantise each centroid to 10 bits per axis (1024 wraps to 0 through the
+// low-10-bit interleave) and interleave x above y above z.  Also emits the sort key ~code (30 bits)
+// so that an ascending stable sort yields the reference's stable descending order.
+__global__ __launch_bounds__(256) void k_morton(Shape *shapes, int n, const float *scene6, unsigned *keys) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float mnx = scene6[0], mxx = scene6[1], mny = scene6[2], mxy = scene6[3], mnz = scene6[4],
+                mxz = scene6[5];
+    Shape *s = shapes + i;
+    float *wf = reinterpret_cast<float *>(s);
+    int type = reinterpret_cast<int *>(s)[0];
+    float px, py, pz;
+    if (type == SPHERE) {
+        px = wf[2]; py = wf[3]; pz = wf[4];
+    } else {
+        px = (wf[19] + wf[20]) / 2.0f;
+        py = (wf[21] + wf[22]) / 2.0f;
+        pz = (wf[23] + wf[24]) / 2.0f;
+    }
+    unsigned a = f2u_code(((px - mnx) / (mxx - mnx)) * 1024.f);
+    unsigned b = f2u_code(((py - mny) / (mxy - mny)) * 1024.f);
+    unsigned c = f2u_code(((pz - mnz) / (mxz - mnz)) * 1024.f);
+    unsigned code = (expand10(a) << 2) | (expand10(b) << 1) | expand10(c);
+    reinterpret_cast<unsigned *>(s)[18] = code;   // morton_code @72
+    keys[i] = (~code) & 0x3FFFFFFFu;
+}
+
+// -------------------------------------------------------------------------------------------
+// Stage 2: onesweep LSD radix sort (Adinets & Merrill 2022) for wave64.
+//   k_sort_hist:   one read of the keys builds the digit histograms of every pass.
+//   k_sort_scan:   exclusive scan of each pass histogram (global digit bases).
+//   k_sort_pass:   per 4096-key tile: wave-level match ranking in input order (stable), tile
+//                  histogram, decoupled look-back over earlier tiles per digit, LDS shuffle into
+//                  digit order, coalesced scatter.  Tile ids come from an atomic counter so every
+//                  tile a block waits on has already started.
+// -------------------------------------------------------------------------------------------
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / kWave;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr unsigned kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kValMask = (1u << 30) - 1;
+constexpr unsigned kSpinLimit = 1u << 26;
+
+template <int BITS> struct SortCfg {
+    static constexpr int kRadix = 1 << BITS;
+    static constexpr unsigned kMask = (unsigned)kRadix - 1;
+    static constexpr int kPasses = (30 + BITS - 1) / BITS;
+};
+
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned *keys, int n, unsigned *hist) {
+    using C = SortCfg<BITS>;
+    __shared__ unsigned sh[C::kPasses * C::kRadix];
+    for (int j = threadIdx.x; j < C::kPasses * C::kRadix; j += kSortThreads) sh[j] = 0;
+    __syncthreads();
+    for (int i = blockIdx.x * kSortThreads + threadIdx.x; i < n; i += gridDim.x * kSortThreads) {
+        unsigned k = keys[i];
+#pragma unroll
+        for (int p = 0; p < C::kPasses; p++) atomicAdd(&sh[p * C::kRadix + ((k >> (p * BITS)) & C::kMask)], 1u);
+    }
+    __syncthreads();
+    for (int j = threadIdx.x; j < C::kPasses * C::kRadix; j += kSortThreads)
+        if (sh[j]) atomicAdd(&hist[j], sh[j]);
+}
+
+// exclusive scan of `v` across the 256 threads of a block; returns the exclusive prefix
+__device__ __forceinline__ unsigned block_exscan256(unsigned v, unsigned *s_w, unsigned *total) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    unsigned x = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        unsigned y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) s_w[w] = x;
+    __syncthreads();
+    unsigned pre = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < kSortWaves; k++) {
+        unsigned t = s_w[k];
+        if (k < w) pre += t;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + x - v;
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_sort_scan(unsigned *hist) {
+    using C = SortCfg<BITS>;
+    __shared__ unsigned s_w[kSortWaves];
+    for (int p = 0; p < C::kPasses; p++) {
+        for (int base = 0; base < C::kRadix; base += kSortThreads) {
+            int d = base + threadIdx.x;
+            unsigned v = d < C::kRadix ? hist[p * C::kRadix + d] : 0u;
+            unsigned tot;
+            unsigned ex = block_exscan256(v, s_w, &tot);
+            if (d < C::kRadix) hist[p * C::kRadix + d] = ex;   // kRadix <= 256: one round
+        }
+    }
+}
+
+template <int BITS>
+__global__ __launch_bounds__(kSortThreads) void k_sort_pass(const unsigned *__restrict__ keys_in,
+                                                            const unsigned *__restrict__ vals_in,
+                                                            unsigned *__restrict__ keys_out,
+                                                            unsigned *__restrict__ vals_out, int n, int shift,
+                                                            const unsigned *__restrict__ digit_base,
+                                                            unsigned *status, unsigned *tile_counter,
+                                                            unsigned *err) {
+    using C = SortCfg<BITS>;
+    __shared__ unsigned s_keys[kSortTile];
+    __shared__ unsigned s_vals[kSortTile];
+    __shared__ unsigned s_wcnt[kSortWaves][C::kRadix];
+    __shared__ unsigned s_lofs[C::kRadix];
+    __shared__ unsigned s_gofs[C::kRadix];
+    __shared__ unsigned s_w[kSortWaves];
+    __shared__ unsigned s_tile;
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int j = tid; j < kSortWaves * C::kRadix; j += kSortThreads) (&s_wcnt[0][0])[j] = 0;
+    __syncthreads();
+    const unsigned tile = s_tile;
+    const long long base = (long long)tile * kSortTile;
+    const long long wbase = base + (long long)w * kWave * kSortItems;
+
+    unsigned k[kSortItems], v[kSortItems], rank[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        long long idx = wbase + j * kWave + lane;
+        bool valid = idx < n;
+        k[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
+        v[j] = valid ? (vals_in ? vals_in[idx] : (unsigned)idx) : 0u;
+    }
+    const unsigned long long lt = lanemask_lt();
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        long long idx = wbase + j * kWave + lane;
+        bool valid = idx < n;
+        unsigned digit = (k[j] >> shift) & C::kMask;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < BITS; b++) {
+            bool bit = (digit >> b) & 1u;
+            unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        unsigned pre = 0;
+        if (valid) pre = s_wcnt[w][digit];
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+            rank[j] = pre + (unsigned)__popcll(peers & lt);
+            if ((peers & lt) == 0) s_wcnt[w][digit] = pre + (unsigned)__popcll(peers);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // per digit: wave-exclusive offsets, tile count, tile-local offsets, global base (look-back)
+    for (int d0 = 0; d0 < C::kRadix; d0 += kSortThreads) {
+        const int d = d0 + tid;
+        unsigned cnt = 0;
+        if (d < C::kRadix) {
+            unsigned run = 0;
+#pragma unroll
+            for (int q = 0; q < kSortWaves; q++) {
+                unsigned c = s_wcnt[q][d];
+                s_wcnt[q][d] = run;
+                run += c;
+            }
+            cnt = run;
+        }
+        unsigned tot;
+        unsigned lofs = block_exscan256(cnt, s_w, &tot);
+        if (d < C::kRadix) {
+            s_lofs[d] = lofs;
+            unsigned *my = status + (size_t)tile * C::kRadix + d;
+            unsigned prefix = 0;
+            if (tile == 0) {
+                st_relaxed(my, kFlagPrefix | cnt);
+            } else {
+                st_relaxed(my, kFlagAgg | cnt);
+                long long t = (long long)tile - 1;
+                unsigned spins = 0;
+                while (t >= 0) {
+                    unsigned s = ld_relaxed(status + (size_t)t * C::kRadix + d);
+                    unsigned flag = s & ~kValMask;
+                    if (flag == 0) {
+                        if (++spins > kSpinLimit) { atomicOr(err, 1u); break; }
+                        continue;
+                    }
+                    prefix += s & kValMask;
+                    if (flag == kFlagPrefix) break;
+                    t--;
+                }
+                st_relaxed(my, kFlagPrefix | ((prefix + cnt) & kValMask));
+            }
+            s_gofs[d] = digit_base[d] + prefix;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortItems; j++) {
+        long long idx = wbase + j * kWave + lane;
+        if (idx < n) {
+            unsigned digit = (k[j] >> shift) & C::kMask;
+            unsigned lpos = s_lofs[digit] + s_wcnt[w][digit] + rank[j];
+            s_keys[lpos] = k[j];
+            s_vals[lpos] = v[j];
+        }
+    }
+    __syncthreads();
+    long long rem = (long long)n - base;
+    const int count = rem < kSortTile ? (int)rem : kSortTile;
+    for (int i = tid; i < count; i += kSortThreads) {
+        unsigned key = s_keys[i];
+        unsigned digit = (key >> shift) & C::kMask;
+        unsigned dest = s_gofs[digit] + (unsigned)i - s_lofs[digit];
+        keys_out[dest] = key;
+        vals_out[dest] = s_vals[i];
+    }
+}
+
+// -------------------------------------------------------------------------------------------
+// Stage 3: Karras radix tree over leaf indices (BVH.cl:127-208).  The prefix length of two leaves
+// is clz(i ^ j); the tree is therefore the binary radix tree of 0..N-1.
+// -------------------------------------------------------------------------------------------
+__device__ __forceinline__ int delta_idx(int i, int j, int n) {
+    return (j >= 0 && j < n) ? __clz(i ^ j) : -1;
+}
+
+__global__ __launch_bounds__(256) void k_karras(int n, BVHNodeGPU *btn, BVHNodeGPU *btl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta_idx(i, i + 1, n) - delta_idx(i, i - 1, n)) > 0 ? 1 : -1;
+    const int dmin = delta_idx(i, i - d, n);
+    int lmax = 2;
+    while (delta_idx(i, i + lmax * d, n) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta_idx(i, i + (l + t) * d, n) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta_idx(i, j, n);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta_idx(i, i + (s + t) * d, n) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    BVHNodeGPU *cur = btn + i;
+    int leafflag = 0;
+    cur->nLeft = gamma;
+    if (lo == gamma) { leafflag = 2; btl[gamma].nParent = i; }
+    else btn[gamma].nParent = i;
+    cur->nRight = gamma + 1;
+    if (hi == gamma + 1) { leafflag = 2; btl[gamma + 1].nParent = i; }
+    else btn[gamma + 1].nParent = i;
+    cur->leaf = leafflag;
+    cur->min = lo;
+    cur->max = hi;
+}
+
+// -------------------------------------------------------------------------------------------
+// Stage 4: leaves + bottom-up refit (BVH.cl:214-258 with a parent visit flag).  Leaf k takes the
+// AABB and index of shape sorted[k]; the second child to arrive at a parent merges the two child
+// bounds (left, right), where a child is a leaf iff nLeft == min / nRight == max.  The flag
+// hand-off is an acquire-release device-scope atomic, so the second arriver sees the first
+// child's bound.  Also writes the traversal layout (TNode per internal node, Prim per leaf).
+// -------------------------------------------------------------------------------------------
+__device__ __forceinline__ Prim make_prim(const Shape *s, int id) {
+    const float *w = reinterpret_cast<const float *>(s);
+    int type = reinterpret_cast<const int *>(s)[0];
+    Prim p;
+    if (type == SPHERE) {
+        p.p0 = make_float4(w[2], w[3], w[4], __int_as_float(id));
+        p.p1 = make_float4(w[1], 0.f, 0.f, __int_as_float(type));
+        p.p2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    } else {
+        float x1 = w[1], y1 = w[2], z1 = w[3];
+        p.p0 = make_float4(x1, y1, z1, __int_as_float(id));
+        p.p1 = make_float4(w[4] - x1, w[5] - y1, w[6] - z1, __int_as_float(type));
+        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, 0.f);
+    }
+    return p;
+}
+
+__device__ __forceinline__ void store_bound(BVHNodeGPU *node, const Bound &b) {
+    float *f = reinterpret_cast<float *>(&node->bound);
+    f[0] = b.min_x; f[1] = b.max_x; f[2] = b.min_y; f[3] = b.max_y; f[4] = b.min_z; f[5] = b.max_z;
+}
+
+__device__ __forceinline__ TNode make_tnode(const Bound &L, const Bound &R, int rl, int rr, int cl, int cr) {
+    TNode t;
+    t.a = make_float4(L.min_x, L.min_y, L.min_z, L.max_x);
+    t.b = make_float4(L.max_y, L.max_z, R.min_x, R.min_y);
+    t.c = make_float4(R.min_z, R.max_x, R.max_y, R.max_z);
+    t.r = make_int4(rl, rr, cl, cr);
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_refit(const Shape *shapes, const unsigned *sorted, int n,
+                                               BVHNodeGPU *btn, BVHNodeGPU *btl, unsigned *flags,
+                                               TNode *tnodes, Prim *prims) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int sid = (int)sorted[k];
+    const Shape *s = shapes + sid;
+    Bound b = load_bound(&s->b);
+    store_bound(btl + k, b);
+    btl[k].nShape = sid;
+    prims[k] = make_prim(s, sid);
+    if (n < 2) return;
+    int p = btl[k].nParent;
+    for (;;) {
+        unsigned old = add_acq_rel(flags + p, 1u);
+        if (old == 0) return;
+        const int nl = btn[p].nLeft, nr = btn[p].nRight, mn = btn[p].min, mx = btn[p].max;
+        const bool lleaf = nl == mn, rleaf = nr == mx;
+        Bound L = load_bound(lleaf ? &btl[nl].bound : &btn[nl].bound);
+        Bound R = load_bound(rleaf ? &btl[nr].bound : &btn[nr].bound);
+        store_bound(btn + p, merge_bounds(L, R));
+        tnodes[p] = make_tnode(L, R, lleaf ? ~nl : nl, rleaf ? ~nr : nr, lleaf ? 1 : 0, rleaf ? 1 : 0);
+        if (p == 0) return;
+        p = btn[p].nParent;
+    }
+}
+
+}  // namespace ptgpu
+
+// ---------------------------------------------------------------------------------------------
+// C ABI (class CLBVH)
+// ---------------------------------------------------------------------------------------------
+using namespace ptgpu;
+
+static int clbvh_alloc(ptgpu_clbvh *h, int n) {
+    const size_t N = (size_t)n;
+    const size_t tiles = (N + kSortTile - 1) / kSortTile;
+    PT_TRY(h->keysA.ensure(N * 4));
+    PT_TRY(h->keysB.ensure(N * 4));
+    PT_TRY(h->valsA.ensure(N * 4));
+    PT_TRY(h->valsB.ensure(N * 4));
+    PT_TRY(h->btn.ensure((N > 1 ? N - 1 : 1) * sizeof(BVHNodeGPU)));
+    PT_TRY(h->btl.ensure(N * sizeof(BVHNodeGPU)));
+    PT_TRY(h->flags.ensure(N * 4));
+    PT_TRY(h->tnodes.ensure((N > 1 ? N - 1 : 1) * sizeof(TNode)));
+    PT_TRY(h->prims.ensure(N * sizeof(Prim)));
+    // sort scratch: histograms (<= 8 passes x 256), status words for every pass, counters
+    PT_TRY(h->hist.ensure(8 * 256 * 4));
+    PT_TRY(h->status.ensure(8 * tiles * 256 * 4 + 64));
+    PT_TRY(h->misc.ensure(4096 * 6 * 4 + 256));
+    return PTGPU_OK;
+}
+
+static inline float *misc_scene6(ptgpu_clbvh *h) { return h->misc.as<float>(); }
+static inline unsigned *misc_counters(ptgpu_clbvh *h) { return h->misc.as<unsigned>() + 8; }  // [0]=bounds done, [1..8] tile counters, [9] err
+static inline float *misc_partials(ptgpu_clbvh *h) { return h->misc.as<float>() + 64; }
+
+static int clbvh_construct(ptgpu_clbvh *h) {
+    hipStream_t st = h->ctx->stream;
+    const int n = h->n;
+    PT_HIP(hipEventRecord(h->ev[0], st));
+    int blocks = (n + kBoundsThreads - 1) / kBoundsThreads;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBoundsThreads), 0, st, h->d_shapes, n, misc_partials(h),
+                       misc_counters(h), misc_scene6(h));
+    PT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, h->d_shapes, n, misc_scene6(h),
+                       h->keysA.as<unsigned>());
+    PT_HIP(hipGetLastError());
+    PT_HIP(hipEventRecord(h->ev[1], st));
+    h->timed[0] = true;
+    h->state = 1;
+    return PTGPU_OK;
+}
+
+template <int BITS>
+static int clbvh_sort(ptgpu_clbvh *h) {
+    using C = SortCfg<BITS>;
+    hipStream_t st = h->ctx->stream;
+    const int n = h->n;
+    const size_t tiles = ((size_t)n + kSortTile - 1) / kSortTile;
+    unsigned *hist = h->hist.as<unsigned>();
+    unsigned *status = h->status.as<unsigned>();
+    unsigned *ctr = misc_counters(h);
+    PT_TRY(fill_u32(st, hist, 0u, (size_t)C::kPasses * C::kRadix));
+    PT_TRY(fill_u32(st, status, 0u, (size_t)C::kPasses * tiles * C::kRadix));
+    PT_TRY(fill_u32(st, ctr + 1, 0u, 9));
+    int hblocks = (int)((n + kSortThreads * 8 - 1) / (kSortThreads * 8));
+    if (hblocks > 2048) hblocks = 2048;
+    if (hblocks < 1) hblocks = 1;
+    hipLaunchKernelGGL(k_sort_hist<BITS>, dim3(hblocks), dim3(kSortThreads), 0, st, h->keysA.as<unsigned>(), n, hist);
+    PT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_sort_scan<BITS>, dim3(1), dim3(kSortThreads), 0, st, hist);
+    PT_HIP(hipGetLastError());
+    unsigned *kin = h->keysA.as<unsigned>(), *vin = nullptr;
+    unsigned *kout = h->keysB.as<unsigned>(), *vout = h->valsB.as<unsigned>();
+    for (int p = 0; p < C::kPasses; p++) {
+        hipLaunchKernelGGL(k_sort_pass<BITS>, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
+                           p * BITS, hist + p * C::kRadix, status + (size_t)p * tiles * C::kRadix, ctr + 1 + p,
+                           ctr + 9);
+        PT_HIP(hipGetLastError());
+        unsigned *tk = kin, *tv = vin;
+        kin = kout; vin = vout;
+        kout = tk; vout = (tv ? tv : h->valsA.as<unsigned>());
+    }
+    h->sorted = vin;
+    return PTGPU_OK;
+}
+
+extern "C" {
+
+static int clbvh_new(ptgpu_ctx *ctx, int n, ptgpu_clbvh **out) {
+    ptgpu_clbvh *h = new ptgpu_clbvh();
+    h->ctx = ctx;
+    h->n = n;
+    h->cap = n;
+    for (auto &e : h->ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) {
+            ptgpu_clbvh_destroy(h);
+            return hip_fail(r, "hipEventCreate", __FILE__, __LINE__);
+        }
+    }
+    int s = clbvh_alloc(h, n);
+    if (s == PTGPU_OK) s = fill_u32(ctx->stream, h->misc.p, 0u, h->misc.bytes / 4);
+    if (s != PTGPU_OK) {
+        ptgpu_clbvh_destroy(h);
+        return s;
+    }
+    *out = h;
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_create(ptgpu_ctx *ctx, Shape *h_shapes, int n, ptgpu_clbvh **out) {
+    PT_CHECK(ctx && h_shapes && out, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(n >= 1 && n < (1 << 30), PTGPU_ERR_INVALID, "shape count %d out of range [1, 2^30)", n);
+    *out = nullptr;
+    PT_HIP(hipSetDevice(ctx->device));
+    ptgpu_clbvh *h = nullptr;
+    PT_TRY(clbvh_new(ctx, n, &h));
+    int s = h->own_shapes.ensure((size_t)n * sizeof(Shape));
+    if (s == PTGPU_OK) {
+        h->d_shapes = h->own_shapes.as<Shape>();
+        hipError_t e = hipMemcpyAsync(h->d_shapes, h_shapes, (size_t)n * sizeof(Shape), hipMemcpyHostToDevice,
+                                      ctx->stream);
+        if (e != hipSuccess) s = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    }
+    if (s == PTGPU_OK) s = clbvh_construct(h);
+    if (s == PTGPU_OK) {
+        // the reference writes index, morton_code and b into the caller's array (CLBVH.cpp:102-159)
+        hipError_t e = hipMemcpyAsync(h_shapes, h->d_shapes, (size_t)n * sizeof(Shape), hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) s = hip_fail(e, "hipMemcpyAsync", __FILE__, __LINE__);
+    }
+    if (s != PTGPU_OK) {
+        ptgpu_clbvh_destroy(h);
+        return s;
+    }
+    *out = h;
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_create_device(ptgpu_ctx *ctx, Shape *d_shapes, int n, ptgpu_clbvh **out) {
+    PT_CHECK(ctx && d_shapes && out, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(n >= 1 && n < (1 << 30), PTGPU_ERR_INVALID, "shape count %d out of range [1, 2^30)", n);
+    *out = nullptr;
+    PT_HIP(hipSetDevice(ctx->device));
+    ptgpu_clbvh *h = nullptr;
+    PT_TRY(clbvh_new(ctx, n, &h));
+    h->d_shapes = d_shapes;
+    int s = clbvh_construct(h);
+    if (s != PTGPU_OK) {
+        ptgpu_clbvh_destroy(h);
+        return s;
+    }
+    *out = h;
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_reset(ptgpu_clbvh *h, int n) {
+    PT_CHECK(h, PTGPU_ERR_INVALID, "null handle");
+    PT_CHECK(n >= 1 && n <= h->cap, PTGPU_ERR_INVALID, "n=%d exceeds the handle capacity %d", n, h->cap);
+    h->n = n;
+    h->optimized = false;
+    return clbvh_construct(h);
+}
+
+int ptgpu_clbvh_build_radix_tree(ptgpu_clbvh *h) {
+    PT_CHECK(h && h->state >= 1, PTGPU_ERR_INVALID, "handle not constructed");
+    hipStream_t st = h->ctx->stream;
+    const int n = h->n;
+    PT_HIP(hipEventRecord(h->ev[2], st));
+    // CLBVH.cpp:83-88: internal nodes zeroed, leaves filled with 0xFF bytes
+    if (n > 1) PT_TRY(fill_u32(st, h->btn.p, 0u, (size_t)(n - 1) * sizeof(BVHNodeGPU) / 4));
+    PT_TRY(fill_u32(st, h->btl.p, 0xFFFFFFFFu, (size_t)n * sizeof(BVHNodeGPU) / 4));
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, h->btn.as<BVHNodeGPU>(),
+                           h->btl.as<BVHNodeGPU>());
+        PT_HIP(hipGetLastError());
+    }
+    PT_HIP(hipEventRecord(h->ev[3], st));
+    h->timed[2] = true;
+    h->state = 2;
+    h->optimized = false;
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
+    PT_CHECK(h && h->state >= 2, PTGPU_ERR_INVALID, "build_radix_tree must run first");
+    hipStream_t st = h->ctx->stream;
+    const int n = h->n;
+    PT_HIP(hipEventRecord(h->ev[4], st));
+    if (h->digit_bits == 4) PT_TRY(clbvh_sort<4>(h));
+    else PT_TRY(clbvh_sort<8>(h));
+    PT_HIP(hipEventRecord(h->ev[5], st));
+    if (n > 1) PT_TRY(fill_u32(st, h->flags.p, 0u, (size_t)(n - 1)));
+    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, st, h->d_shapes, h->sorted, n,
+                       h->btn.as<BVHNodeGPU>(), h->btl.as<BVHNodeGPU>(), h->flags.as<unsigned>(),
+                       h->tnodes.as<TNode>(), h->prims.as<Prim>());
+    PT_HIP(hipGetLastError());
+    PT_HIP(hipEventRecord(h->ev[6], st));
+    h->timed[1] = true;
+    h->timed[3] = true;
+    h->state = 3;
+    h->optimized = false;
+    h->acc.nodes = h->tnodes.as<TNode>();
+    h->acc.prims = h->prims.as<Prim>();
+    h->acc.nprims = n;
+    h->acc.mode = PTGPU_ACCEL_BVH;
+    if (n >= 2) {
+        h->acc.root = 0;
+        h->acc.root_cnt = 0;
+        int d = 0;
+        while ((1ll << d) < n) d++;
+        h->acc.depth = d;
+    } else {
+        h->acc.root = ~0;
+        h->acc.root_cnt = 1;
+        h->acc.depth = 0;
+    }
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_set_digit_bits(ptgpu_clbvh *h, int bits) {
+    PT_CHECK(h, PTGPU_ERR_INVALID, "null handle");
+    PT_CHECK(bits == 4 || bits == 8, PTGPU_ERR_INVALID, "digit width %d (4 or 8)", bits);
+    h->digit_bits = bits;
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_optimize(ptgpu_clbvh *h, int treelet_size) {
+    PT_CHECK(h, PTGPU_ERR_INVALID, "null handle");
+    (void)treelet_size;
+    ptgpu::set_error("treelet optimisation is not built yet");
+    return PTGPU_ERR_UNSUPPORTED;
+}
+
+int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl) {
+    PT_CHECK(h && h_btn && h_btl, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(h->state >= 2, PTGPU_ERR_INVALID, "no tree built");
+    const int n = h->n;
+    h->h_btn.resize(n > 1 ? n - 1 : 0);
+    h->h_btl.resize(n);
+    if (n > 1)
+        PT_HIP(hipMemcpyAsync(h->h_btn.data(), h->btn.p, (size_t)(n - 1) * sizeof(BVHNodeGPU), hipMemcpyDeviceToHost,
+                              h->ctx->stream));
+    PT_HIP(hipMemcpyAsync(h->h_btl.data(), h->btl.p, (size_t)n * sizeof(BVHNodeGPU), hipMemcpyDeviceToHost,
+                          h->ctx->stream));
+    unsigned err = 0;
+    PT_HIP(hipMemcpyAsync(&err, misc_counters(h) + 9, 4, hipMemcpyDeviceToHost, h->ctx->stream));
+    PT_HIP(hipStreamSynchronize(h->ctx->stream));
+    PT_CHECK(err == 0, PTGPU_ERR_HIP, "radix sort look-back exceeded its spin limit");
+    *h_btn = h->h_btn.data();
+    *h_btl = h->h_btl.data();
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl) {
+    PT_CHECK(h && d_btn && d_btl, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(h->state >= 2, PTGPU_ERR_INVALID, "no tree built");
+    *d_btn = h->btn.as<BVHNodeGPU>();
+    *d_btl = h->btl.as<BVHNodeGPU>();
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted) {
+    PT_CHECK(h && d_sorted, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(h->state >= 3, PTGPU_ERR_INVALID, "build_bvh_tree has not run");
+    *d_sorted = reinterpret_cast<const int *>(h->sorted);
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_stage_times(ptgpu_clbvh *h, float *ms5) {
+    PT_CHECK(h && ms5, PTGPU_ERR_INVALID, "null argument");
+    PT_HIP(hipStreamSynchronize(h->ctx->stream));
+    const int pairs[5][2] = {{0, 1}, {4, 5}, {2, 3}, {5, 6}, {7, 8}};
+    for (int k = 0; k < 5; k++) {
+        ms5[k] = 0.f;
+        if (!h->timed[k]) continue;
+        PT_HIP(hipEventElapsedTime(&ms5[k], h->ev[pairs[k][0]], h->ev[pairs[k][1]]));
+    }
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
+    if (!h) return PTGPU_OK;
+    if (h->ctx && h->ctx->stream) (void)hipStreamSynchronize(h->ctx->stream);
+    h->keysA.release(); h->keysB.release(); h->valsA.release(); h->valsB.release();
+    h->btn.release(); h->btl.release(); h->flags.release(); h->tnodes.release(); h->prims.release();
+    h->hist.release(); h->status.release(); h->misc.release(); h->own_shapes.release();
+    h->lights.release(); h->scan.release(); h->opt_tmp.release();
+    for (auto &e : h->ev)
+        if (e) (void)hipEventDestroy(e);
+    delete h;
+    return PTGPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,27 @@
+// lbvh.hpp -- state of one LBVH build (the object behind the C ABI's ptgpu_clbvh handle).
+#pragma once
+
+#include <vector>
+
+#include "common.hpp"
+
+struct ptgpu_clbvh {
+    ptgpu_ctx *ctx = nullptr;
+    int n = 0, cap = 0;
+    int state = 0;            // 1 constructed (bounds+Morton), 2 radix tree, 3 BVH refit done
+    int digit_bits = 8;       // radix-sort digit width (4 or 8)
+    bool optimized = false;
+    Shape *d_shapes = nullptr;
+    ptgpu::DevBuf own_shapes;
+    ptgpu::DevBuf keysA, keysB, valsA, valsB;
+    ptgpu::DevBuf btn, btl, flags, tnodes, prims;
+    ptgpu::DevBuf hist, status, misc;
+    ptgpu::DevBuf lights, scan, opt_tmp;
+    unsigned *sorted = nullptr;
+    int lights_for = -1, nlights = 0;
+    ptgpu::DevAccel acc;
+    std::vector<BVHNodeGPU> h_btn, h_btl;
+    // events: 0/1 morton, 2/3 topology, 4/5 sort, 5/6 refit, 7/8 optimize
+    hipEvent_t ev[10] = {};
+    bool timed[5] = {false, false, false, false, false};
+};

@@ -1,0 +1,413 @@
+"""ctypes host mirror of the reference's interface for the ray-acceleration and radiance path.
+
+`CLBVH` keeps the method names and call order of the reference class (CLBVH.h:39-47):
+    bvh = CLBVH(ctx, shapes)          # ctor: AABBs + Morton codes, written back into `shapes`
+    bvh.buildRadixTree()              # Karras topology
+    bvh.buildBVHTree()                # sort + refit
+    bvh.optimize(7)                   # treelet restructuring
+    btn, btl = bvh.getTrees()
+`KDTree` mirrors KDTree::build / getTrees (KDTree.h:88-89).  `radiance_gpu` takes the RadianceGPU
+argument list in its original order (rendering_kernel.cl:1072-1099).
+
+Everything runs through libptgpu.so; there is no CPU fallback.  If the library is missing the
+import of this module still works, but every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+from .layout import (BVHNODE_DTYPE, CAMERA_DTYPE, KDNODE_DTYPE, RAY_DTYPE, SHAPE_DTYPE)
+
+LIB_PATH = Path(__file__).resolve().parent / "libptgpu.so"
+
+ACCEL_NONE, ACCEL_BVH, ACCEL_KD = 0, 1, 2
+STATUS_NAMES = {0: "OK", 1: "INVALID", 2: "HIP", 3: "OOM", 4: "UNSUPPORTED", 5: "NO_DEVICE"}
+
+_LIB = None
+vp = C.c_void_p
+i32 = C.c_int
+u32 = C.c_uint
+f32 = C.c_float
+
+
+class PtgpuError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"ptgpu error {STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class RenderArgs(C.Structure):
+    _fields_ = [("colors", vp), ("seeds", vp), ("camera", vp), ("shapes", vp), ("shapeCnt", u32),
+                ("lightCnt", u32), ("width", i32), ("height", i32), ("currentSample", i32), ("spp", i32),
+                ("max_depth", i32), ("pixels", vp), ("ray_counter", vp), ("trav_stats", vp)]
+
+
+class RadianceArgs(C.Structure):
+    _fields_ = [("colors", vp), ("seeds", vp), ("camera", vp), ("shapes", vp), ("shapeCnt", u32),
+                ("lightCnt", u32), ("btn", vp), ("btl", vp), ("kng", vp), ("kngCnt", i32), ("kn", vp),
+                ("knCnt", i32), ("width", i32), ("height", i32), ("currentSample", i32), ("pixels", vp)]
+
+
+_SIGS = {
+    "ptgpu_layout_sizes": [C.POINTER(i32)] * 6,
+    "ptgpu_last_error": [],
+    "ptgpu_device_count": [C.POINTER(i32)],
+    "ptgpu_ctx_create": [i32, vp, C.POINTER(vp)],
+    "ptgpu_ctx_destroy": [vp],
+    "ptgpu_ctx_set_max_depth": [vp, i32],
+    "ptgpu_synchronize": [vp],
+    "ptgpu_malloc": [vp, C.c_size_t, C.POINTER(vp)],
+    "ptgpu_free": [vp, vp],
+    "ptgpu_memcpy_h2d": [vp, vp, vp, C.c_size_t],
+    "ptgpu_memcpy_d2h": [vp, vp, vp, C.c_size_t],
+    "ptgpu_memset_d": [vp, vp, i32, C.c_size_t],
+    "ptgpu_memcpy_d2d": [vp, vp, vp, C.c_size_t],
+    "ptgpu_host_free": [vp],
+    "ptgpu_event_record": [vp, i32],
+    "ptgpu_event_elapsed_ms": [vp, i32, i32, C.POINTER(f32)],
+    "ptgpu_clbvh_create": [vp, vp, i32, C.POINTER(vp)],
+    "ptgpu_clbvh_create_device": [vp, vp, i32, C.POINTER(vp)],
+    "ptgpu_clbvh_reset": [vp, i32],
+    "ptgpu_clbvh_build_radix_tree": [vp],
+    "ptgpu_clbvh_build_bvh_tree": [vp],
+    "ptgpu_clbvh_optimize": [vp, i32],
+    "ptgpu_clbvh_set_digit_bits": [vp, i32],
+    "ptgpu_clbvh_get_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
+    "ptgpu_clbvh_device_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
+    "ptgpu_clbvh_device_sorted": [vp, C.POINTER(vp)],
+    "ptgpu_clbvh_stage_times": [vp, C.POINTER(f32)],
+    "ptgpu_clbvh_destroy": [vp],
+    "ptgpu_kdtree_build": [vp, vp, i32, C.POINTER(vp)],
+    "ptgpu_kdtree_build_sah": [vp, vp, i32, i32, i32, C.POINTER(vp)],
+    "ptgpu_kdtree_get_trees": [vp, C.POINTER(vp), C.POINTER(i32), C.POINTER(vp), C.POINTER(i32)],
+    "ptgpu_kdtree_export_dfs": [vp, vp, vp, vp, i32, i32],
+    "ptgpu_kdtree_stats": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)],
+    "ptgpu_kdtree_destroy": [vp],
+    "ptgpu_radiance_gpu": [vp, i32, C.POINTER(RadianceArgs)],
+    "ptgpu_render": [vp, vp, vp, C.POINTER(RenderArgs)],
+    "ptgpu_trace_closest": [vp, vp, vp, vp, u32, vp, i32, vp, vp],
+    "ptgpu_trace_any": [vp, vp, vp, vp, u32, vp, vp, i32, vp],
+    "ptgpu_update_camera": [vp],
+}
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libptgpu.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = C.CDLL(str(p))
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "ptgpu_last_error" else (None if name == "ptgpu_host_free" else i32)
+    sizes = [i32() for _ in range(6)]
+    lib.ptgpu_layout_sizes(*[C.byref(s) for s in sizes])
+    got = [s.value for s in sizes]
+    want = [SHAPE_DTYPE.itemsize, CAMERA_DTYPE.itemsize, BVHNODE_DTYPE.itemsize, KDNODE_DTYPE.itemsize,
+            RAY_DTYPE.itemsize, 24]
+    if got != want:
+        raise RuntimeError(f"layout mismatch between libptgpu.so {got} and layout.py {want}")
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = _LIB.ptgpu_last_error().decode(errors="replace") if _LIB else "library not loaded"
+        raise PtgpuError(status, msg)
+
+
+def _ptr(a: np.ndarray) -> int:
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data
+
+
+def device_count() -> int:
+    lib = load_library()
+    c = i32()
+    _check(lib.ptgpu_device_count(C.byref(c)))
+    return c.value
+
+
+def update_camera(cam: np.ndarray) -> np.ndarray:
+    """UpdateCamera (displayfunc.cpp:732-756) in place on a 1-element CAMERA_DTYPE array."""
+    lib = load_library()
+    _check(lib.ptgpu_update_camera(_ptr(cam)))
+    return cam
+
+
+class DeviceArray:
+    """A device allocation owned by a Context."""
+
+    def __init__(self, ctx: "Context", nbytes: int, dtype=np.uint8, shape=None):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // max(1, self.dtype.itemsize),)
+        p = vp()
+        _check(ctx.lib.ptgpu_malloc(ctx.handle, max(4, self.nbytes), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("host array larger than the allocation")
+        _check(self.ctx.lib.ptgpu_memcpy_h2d(self.ctx.handle, self.ptr, _ptr(a), a.nbytes))
+        return self
+
+    def download(self, dtype=None, shape=None) -> np.ndarray:
+        dt = np.dtype(dtype) if dtype is not None else self.dtype
+        shp = shape if shape is not None else (self.nbytes // dt.itemsize,)
+        out = np.empty(shp, dtype=dt)
+        _check(self.ctx.lib.ptgpu_memcpy_d2h(self.ctx.handle, _ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def copy_from(self, other: "DeviceArray") -> "DeviceArray":
+        """Stream-ordered device-to-device copy."""
+        _check(self.ctx.lib.ptgpu_memcpy_d2d(self.ctx.handle, self.ptr, other.ptr, min(self.nbytes, other.nbytes)))
+        return self
+
+    def zero(self) -> "DeviceArray":
+        _check(self.ctx.lib.ptgpu_memset_d(self.ctx.handle, self.ptr, 0, self.nbytes))
+        return self
+
+    def free(self) -> None:
+        if self.ptr:
+            _check(self.ctx.lib.ptgpu_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.handle:
+                self.ctx.lib.ptgpu_free(self.ctx.handle, self.ptr)
+        except Exception:
+            pass
+
+
+class Context:
+    """One device + one HIP stream (replaces the OpenCL context/queue, smallptGPU.cpp:286-441)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = vp()
+        _check(self.lib.ptgpu_ctx_create(device, None, C.byref(h)))
+        self.handle = h.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.ptgpu_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_max_depth(self, d: int) -> None:
+        _check(self.lib.ptgpu_ctx_set_max_depth(self.handle, d))
+
+    def synchronize(self) -> None:
+        _check(self.lib.ptgpu_synchronize(self.handle))
+
+    def alloc(self, nbytes: int, dtype=np.uint8, shape=None) -> DeviceArray:
+        return DeviceArray(self, nbytes, dtype, shape)
+
+    def to_device(self, a: np.ndarray) -> DeviceArray:
+        a = np.ascontiguousarray(a)
+        return DeviceArray(self, a.nbytes, a.dtype, a.shape).upload(a)
+
+    def record(self, slot: int) -> None:
+        _check(self.lib.ptgpu_event_record(self.handle, slot))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = f32()
+        _check(self.lib.ptgpu_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
+        return ms.value
+
+
+class CLBVH:
+    """Mirror of class CLBVH (CLBVH.h:39-47).
+
+    shapes: a host SHAPE_DTYPE array (index/morton_code/b are written back into it, as the
+    reference does) or a DeviceArray holding n shapes (updated in place on the device)."""
+
+    def __init__(self, ctx: Context, shapes, n: int | None = None, digit_bits: int = 8):
+        self.ctx = ctx
+        h = vp()
+        if isinstance(shapes, DeviceArray):
+            self.n = int(n if n is not None else shapes.nbytes // SHAPE_DTYPE.itemsize)
+            _check(ctx.lib.ptgpu_clbvh_create_device(ctx.handle, shapes.ptr, self.n, C.byref(h)))
+            self.d_shapes = shapes
+        else:
+            if shapes.dtype != SHAPE_DTYPE or not shapes.flags["C_CONTIGUOUS"]:
+                raise ValueError("shapes must be a C-contiguous SHAPE_DTYPE array")
+            self.n = int(n if n is not None else len(shapes))
+            _check(ctx.lib.ptgpu_clbvh_create(ctx.handle, _ptr(shapes), self.n, C.byref(h)))
+            self.d_shapes = None
+        self.handle = h.value
+        if digit_bits != 8:
+            _check(ctx.lib.ptgpu_clbvh_set_digit_bits(self.handle, digit_bits))
+
+    def set_digit_bits(self, bits: int) -> None:
+        _check(self.ctx.lib.ptgpu_clbvh_set_digit_bits(self.handle, bits))
+
+    def reset(self, n: int | None = None) -> None:
+        _check(self.ctx.lib.ptgpu_clbvh_reset(self.handle, int(n if n is not None else self.n)))
+
+    def buildRadixTree(self) -> None:
+        _check(self.ctx.lib.ptgpu_clbvh_build_radix_tree(self.handle))
+
+    def buildBVHTree(self) -> None:
+        _check(self.ctx.lib.ptgpu_clbvh_build_bvh_tree(self.handle))
+
+    def optimize(self, treelet_size: int = 7) -> None:
+        _check(self.ctx.lib.ptgpu_clbvh_optimize(self.handle, treelet_size))
+
+    def getTrees(self):
+        """Returns copies of (btn[N-1], btl[N]) as BVHNODE_DTYPE arrays."""
+        a, b = vp(), vp()
+        _check(self.ctx.lib.ptgpu_clbvh_get_trees(self.handle, C.byref(a), C.byref(b)))
+        nn = max(self.n - 1, 0)
+        btn = np.empty(nn, dtype=BVHNODE_DTYPE)
+        btl = np.empty(self.n, dtype=BVHNODE_DTYPE)
+        if nn:
+            C.memmove(_ptr(btn), a.value, btn.nbytes)
+        C.memmove(_ptr(btl), b.value, btl.nbytes)
+        return btn, btl
+
+    def device_trees(self):
+        a, b = vp(), vp()
+        _check(self.ctx.lib.ptgpu_clbvh_device_trees(self.handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def sorted_indices(self) -> np.ndarray:
+        p = vp()
+        _check(self.ctx.lib.ptgpu_clbvh_device_sorted(self.handle, C.byref(p)))
+        out = np.empty(self.n, dtype=np.int32)
+        _check(self.ctx.lib.ptgpu_memcpy_d2h(self.ctx.handle, _ptr(out), p.value, out.nbytes))
+        return out
+
+    def stage_times(self) -> dict:
+        ms = (f32 * 5)()
+        _check(self.ctx.lib.ptgpu_clbvh_stage_times(self.handle, ms))
+        return dict(zip(["morton", "sort", "topology", "refit", "optimize"], list(ms)))
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.ptgpu_clbvh_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class KDTree:
+    """Mirror of class KDTree (KDTree.h:47-90): build() then getTrees()."""
+
+    def __init__(self, ctx: Context | None, shapes: np.ndarray, sah: bool = False, max_depth: int = 24,
+                 max_leaf: int = 4):
+        """ctx=None builds the host tree only (exports work, rendering does not)."""
+        if shapes.dtype != SHAPE_DTYPE:
+            raise ValueError("shapes must be a SHAPE_DTYPE array")
+        self.ctx = ctx
+        self.lib = ctx.lib if ctx is not None else load_library()
+        self.n = len(shapes)
+        shapes = np.ascontiguousarray(shapes)
+        h = vp()
+        ch = ctx.handle if ctx is not None else None
+        if sah:
+            _check(self.lib.ptgpu_kdtree_build_sah(ch, _ptr(shapes), self.n, max_depth, max_leaf, C.byref(h)))
+        else:
+            _check(self.lib.ptgpu_kdtree_build(ch, _ptr(shapes), self.n, C.byref(h)))
+        self.handle = h.value
+
+    def stats(self) -> dict:
+        v = [i32() for _ in range(4)]
+        _check(self.lib.ptgpu_kdtree_stats(self.handle, *[C.byref(x) for x in v]))
+        return dict(zip(["max_depth", "nodes", "leaves", "kn_len"], [x.value for x in v]))
+
+    def getTrees(self):
+        """KDTree::getTrees: (kng heap as KDNODE_DTYPE array, kn int32 array)."""
+        pk, pn = vp(), vp()
+        ck, cn = i32(), i32()
+        _check(self.lib.ptgpu_kdtree_get_trees(self.handle, C.byref(pk), C.byref(ck), C.byref(pn), C.byref(cn)))
+        kng = np.empty(ck.value, dtype=KDNODE_DTYPE)
+        kn = np.empty(cn.value, dtype=np.int32)
+        C.memmove(_ptr(kng), pk.value, kng.nbytes)
+        C.memmove(_ptr(kn), pn.value, kn.nbytes)
+        self.lib.ptgpu_host_free(pk.value)
+        self.lib.ptgpu_host_free(pn.value)
+        return kng, kn
+
+    def export_dfs(self):
+        st = self.stats()
+        node5 = np.empty((st["nodes"], 5), dtype=np.int32)
+        box = np.empty((st["nodes"], 6), dtype=np.float32)
+        kn = np.empty(max(st["kn_len"], 1), dtype=np.int32)
+        _check(self.lib.ptgpu_kdtree_export_dfs(self.handle, _ptr(node5), _ptr(box), _ptr(kn), st["nodes"],
+                                                len(kn)))
+        return node5, box, kn[:st["kn_len"]]
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.ptgpu_kdtree_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _h(x):
+    return x.handle if x is not None else None
+
+
+def render(ctx: Context, bvh: CLBVH | None, kd: KDTree | None, *, colors: DeviceArray, seeds: DeviceArray,
+           camera: DeviceArray, shapes: DeviceArray, shapeCnt: int, lightCnt: int, width: int, height: int,
+           currentSample: int = 0, spp: int = 1, max_depth: int = 6, pixels: DeviceArray | None = None,
+           ray_counter: DeviceArray | None = None, trav_stats: DeviceArray | None = None) -> None:
+    """`spp` RadianceGPU passes in one launch over a prepared BVH / KD tree (or brute force)."""
+    a = RenderArgs(colors.ptr, seeds.ptr, camera.ptr, shapes.ptr, shapeCnt, lightCnt, width, height, currentSample,
+                   spp, max_depth, pixels.ptr if pixels else None, ray_counter.ptr if ray_counter else None,
+                   trav_stats.ptr if trav_stats else None)
+    _check(ctx.lib.ptgpu_render(ctx.handle, _h(bvh), _h(kd), C.byref(a)))
+
+
+def radiance_gpu(ctx: Context, accel: int, colors, seeds, camera, shapes, shapeCnt, lightCnt, btn=None, btl=None,
+                 kng=None, kngCnt=0, kn=None, knCnt=0, width=0, height=0, currentSample=0, pixels=None) -> None:
+    """One RadianceGPU launch with the reference argument list (device pointers or DeviceArrays)."""
+    def p(x):
+        if x is None:
+            return None
+        return x.ptr if isinstance(x, DeviceArray) else int(x)
+    a = RadianceArgs(p(colors), p(seeds), p(camera), p(shapes), shapeCnt, lightCnt, p(btn), p(btl), p(kng), kngCnt,
+                     p(kn), knCnt, width, height, currentSample, p(pixels))
+    _check(ctx.lib.ptgpu_radiance_gpu(ctx.handle, accel, C.byref(a)))
+
+
+def trace_closest(ctx: Context, bvh, kd, shapes: DeviceArray, shapeCnt: int, rays: DeviceArray, n: int,
+                  t_out: DeviceArray, id_out: DeviceArray) -> None:
+    _check(ctx.lib.ptgpu_trace_closest(ctx.handle, _h(bvh), _h(kd), shapes.ptr, shapeCnt, rays.ptr, n, t_out.ptr,
+                                       id_out.ptr))
+
+
+def trace_any(ctx: Context, bvh, kd, shapes: DeviceArray, shapeCnt: int, rays: DeviceArray, maxt: DeviceArray,
+              n: int, hit_out: DeviceArray) -> None:
+    _check(ctx.lib.ptgpu_trace_any(ctx.handle, _h(bvh), _h(kd), shapes.ptr, shapeCnt, rays.ptr, maxt.ptr, n,
+                                   hit_out.ptr))
