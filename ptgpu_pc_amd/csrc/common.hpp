@@ -58,6 +58,8 @@ struct DevAccel {
     int root = 0;          // >= 0 internal node; < 0: the whole scene is one leaf ~root
     int root_cnt = 0;      // primitive count when the root is a leaf
     int nprims = 0;
+    int nnodes = 0;        // number of TNodes (node refs are checked against it)
+    int nshapes = 0;       // shapeCnt: any-hit queries ignore shape nshapes-1 (IntersectP, Q10)
     int depth = 0;         // max internal-node depth (root = 0); stack bound
     int mode = PTGPU_ACCEL_NONE;
 };

@@ -345,7 +345,9 @@ int upload_device(ptgpu_kdtree *T) {
     DevAccel &A = T->acc;
     A.nodes = T->d_nodes.as<TNode>();
     A.prims = T->d_prims.as<Prim>();
-    A.nprims = T->n;
+    A.nprims = (int)T->kn.size();
+    A.nnodes = ni;
+    A.nshapes = T->n;
     A.mode = PTGPU_ACCEL_KD;
     const KNode &root = T->nodes[0];
     if (root.leaf) {

@@ -676,6 +676,8 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     h->acc.nodes = h->tnodes.as<TNode>();
     h->acc.prims = h->prims.as<Prim>();
     h->acc.nprims = n;
+    h->acc.nnodes = n - 1;
+    h->acc.nshapes = n;
     h->acc.mode = PTGPU_ACCEL_BVH;
     if (n >= 2) {
         h->acc.root = 0;

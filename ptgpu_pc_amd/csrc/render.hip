@@ -25,6 +25,7 @@ namespace ptgpu {
 
 constexpr float kSlack = 1.0f + 1.0e-5f;   // relative slack on the slab test's exit distance
 constexpr float kInf = 1e20f;              // the reference's initial t (rendering_kernel.cl:228)
+constexpr unsigned kMaxVisits = 1u << 24;  // node visits per query; only a cyclic tree reaches it
 
 struct f3 { float x, y, z; };
 __device__ __forceinline__ f3 mk(float x, float y, float z) { f3 v; v.x = x; v.y = y; v.z = z; return v; }
@@ -106,6 +107,7 @@ __device__ int trace_closest(const DevAccel &A, f3 o, f3 d, float *tout, int *st
     int bid = -1;
     const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     auto test_leaf = [&](int first, int cnt) {
+        if (first < 0 || cnt > A.nprims - first) return;   // malformed caller tree
         for (int q = 0; q < cnt; q++) {
             int id;
             const float t = prim_isect(A.prims, first + q, o, d, &id);
@@ -119,7 +121,8 @@ __device__ int trace_closest(const DevAccel &A, f3 o, f3 d, float *tout, int *st
         return bid;
     }
     int node = A.root, sp = 0;
-    for (;;) {
+    for (unsigned guard = 0; guard < kMaxVisits; guard++) {
+        if ((unsigned)node >= (unsigned)A.nnodes) break;   // malformed caller tree
         const TNode *np = A.nodes + node;
         const float4 a = np->a, b = np->b, c = np->c;
         const int4 r = np->r;
@@ -153,6 +156,7 @@ __device__ bool trace_any(const DevAccel &A, f3 o, f3 d, float maxt, int skip_id
                           unsigned *nodes_visited, unsigned *prims_tested) {
     const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     auto test_leaf = [&](int first, int cnt) -> bool {
+        if (first < 0 || cnt > A.nprims - first) return false;   // malformed caller tree
         for (int q = 0; q < cnt; q++) {
             int id;
             const float t = prim_isect(A.prims, first + q, o, d, &id);
@@ -163,7 +167,8 @@ __device__ bool trace_any(const DevAccel &A, f3 o, f3 d, float maxt, int skip_id
     };
     if (A.root < 0) return test_leaf(~A.root, A.root_cnt);
     int node = A.root, sp = 0;
-    for (;;) {
+    for (unsigned guard = 0; guard < kMaxVisits; guard++) {
+        if ((unsigned)node >= (unsigned)A.nnodes) break;   // malformed caller tree
         const TNode *np = A.nodes + node;
         const float4 a = np->a, b = np->b, c = np->c;
         const int4 r = np->r;
@@ -219,7 +224,7 @@ __device__ __forceinline__ int closest(const DevAccel &A, f3 o, f3 d, float *t, 
 template <int STACK>
 __device__ __forceinline__ bool anyhit(const DevAccel &A, f3 o, f3 d, float maxt, int *stk, unsigned *nv, unsigned *pt) {
     if (A.mode == PTGPU_ACCEL_NONE) return brute_any(A.prims, A.nprims, o, d, maxt, pt);
-    return trace_any<STACK>(A, o, d, maxt, A.nprims - 1, stk, nv, pt);
+    return trace_any<STACK>(A, o, d, maxt, A.nshapes - 1, stk, nv, pt);
 }
 
 // ---- camera ray, rendering_kernel.cl:1042-1070 ----
@@ -697,6 +702,8 @@ int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeG
     acc->nodes = nodes.as<TNode>();
     acc->prims = prims.as<Prim>();
     acc->nprims = n;
+    acc->nnodes = n - 1;
+    acc->nshapes = n;
     acc->mode = PTGPU_ACCEL_BVH;
     if (n >= 2) {
         acc->root = 0;
@@ -726,7 +733,9 @@ int convert_kd_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU
     PT_HIP(hipStreamSynchronize(ctx->stream));
     acc->nodes = nodes.as<TNode>();
     acc->prims = prims.as<Prim>();
-    acc->nprims = n;
+    acc->nprims = knCnt;
+    acc->nnodes = kngCnt;
+    acc->nshapes = n;
     acc->mode = PTGPU_ACCEL_KD;
     if (root.leaf == 1) {
         acc->root = ~(int)root.min;
@@ -826,14 +835,16 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
     }
     if (kd) {
         PT_TRY(ptgpu_kdtree_device_accel(kd, acc));
-        PT_CHECK((unsigned)acc->nprims == n, PTGPU_ERR_INVALID, "shapeCnt %u differs from the KD tree's %d", n,
-                 acc->nprims);
+        PT_CHECK((unsigned)acc->nshapes == n, PTGPU_ERR_INVALID, "shapeCnt %u differs from the KD tree's %d", n,
+                 acc->nshapes);
         return PTGPU_OK;
     }
     PT_TRY(build_index_prims(ctx, d_shapes, (int)n, ctx->tmp_prims));
     acc->nodes = nullptr;
     acc->prims = ctx->tmp_prims.as<Prim>();
     acc->nprims = (int)n;
+    acc->nnodes = 0;
+    acc->nshapes = (int)n;
     acc->root = ~0;
     acc->root_cnt = (int)n;
     acc->depth = 0;

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -157,6 +158,7 @@ class DeviceArray:
         p = vp()
         _check(ctx.lib.ptgpu_malloc(ctx.handle, max(4, self.nbytes), C.byref(p)))
         self.ptr = p.value
+        ctx._adopt(self)
 
     def upload(self, a: np.ndarray) -> "DeviceArray":
         a = np.ascontiguousarray(a)
@@ -182,14 +184,15 @@ class DeviceArray:
         return self
 
     def free(self) -> None:
-        if self.ptr:
+        if self.ptr and self.ctx.handle:
             _check(self.ctx.lib.ptgpu_free(self.ctx.handle, self.ptr))
-            self.ptr = None
+        self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
-            if self.ptr and self.ctx.handle:
-                self.ctx.lib.ptgpu_free(self.ctx.handle, self.ptr)
+            self.free()
         except Exception:
             pass
 
@@ -199,14 +202,31 @@ class Context:
 
     def __init__(self, device: int = 0):
         self.lib = load_library()
+        self.handle = None
+        self._children = weakref.WeakSet()
         h = vp()
         _check(self.lib.ptgpu_ctx_create(device, None, C.byref(h)))
         self.handle = h.value
 
+    def _adopt(self, obj) -> None:
+        self._children.add(obj)
+
     def close(self) -> None:
+        """Releases every handle and allocation made through this context, then the context."""
         if self.handle:
+            for child in list(self._children):
+                try:
+                    child.close()
+                except Exception:
+                    pass
             self.lib.ptgpu_ctx_destroy(self.handle)
             self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def __enter__(self):
         return self
@@ -256,6 +276,7 @@ class CLBVH:
             _check(ctx.lib.ptgpu_clbvh_create(ctx.handle, _ptr(shapes), self.n, C.byref(h)))
             self.d_shapes = None
         self.handle = h.value
+        ctx._adopt(self)
         if digit_bits != 8:
             _check(ctx.lib.ptgpu_clbvh_set_digit_bits(self.handle, digit_bits))
 
@@ -304,9 +325,9 @@ class CLBVH:
         return dict(zip(["morton", "sort", "topology", "refit", "optimize"], list(ms)))
 
     def close(self) -> None:
-        if self.handle:
+        if self.handle and self.ctx.handle:
             self.ctx.lib.ptgpu_clbvh_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -334,6 +355,8 @@ class KDTree:
         else:
             _check(self.lib.ptgpu_kdtree_build(ch, _ptr(shapes), self.n, C.byref(h)))
         self.handle = h.value
+        if ctx is not None:
+            ctx._adopt(self)
 
     def stats(self) -> dict:
         v = [i32() for _ in range(4)]
@@ -363,9 +386,9 @@ class KDTree:
         return node5, box, kn[:st["kn_len"]]
 
     def close(self) -> None:
-        if self.handle:
+        if self.handle and (self.ctx is None or self.ctx.handle):
             self.lib.ptgpu_kdtree_destroy(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:

@@ -168,7 +168,7 @@ def test_closest_hit_c2_camera_rays(ctx):
     say("case c2 camera rays")
     t_o, id_o = O.trace_closest(s, O.accel_none(), rays)
     t_g, id_g = gpu_trace(ctx, bvh, None, ctx.to_device(s), len(s), rays)
-    assert (id_o >= 0).mean() > 0.5
+    assert (id_o >= 0).mean() > 0.4
     assert_closest_parity(t_g, id_g, t_o, id_o)
 
 
