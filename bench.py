@@ -74,13 +74,60 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
                       f"shadow rays over 65,538 shapes), {elapsed:.1f} s"}
 
 
+# algorithmic bytes per primitive of each build stage (each logical load/store counted once at its
+# natural width, atomics 8 B): Morton 36+36+4, 8-bit onesweep 4 + 16*4, topology 28, refit 156
+STAGE_BYTES = {"morton": 76, "sort": 68, "topology": 28, "refit": 156}
+
+
+def build_sweep(sizes, reps=5):
+    """LBVH build throughput and achieved algorithmic GB/s per stage on large soups (working sets
+    beyond the 256 MB MALL from about 2^21 primitives)."""
+    import ptgpu_pc_amd as pt
+    rng = np.random.default_rng(11)
+    ctx = pt.Context(0)
+    for n in sizes:
+        cen = rng.random((n, 3), dtype=np.float32)
+        from ptgpu_pc_amd.layout import SHAPE_DTYPE, TRIANGLE
+        sh = np.zeros(n, dtype=SHAPE_DTYPE)
+        sh["type"] = TRIANGLE
+        for k in ("p1", "p2", "p3"):
+            sh[k] = cen + rng.uniform(-0.01, 0.01, size=(n, 3)).astype(np.float32)
+        d_sh = ctx.to_device(sh)
+        del sh
+        bvh = pt.CLBVH(ctx, d_sh, n)
+        best = None
+        for _ in range(reps):
+            bvh.reset()
+            bvh.buildRadixTree()
+            bvh.buildBVHTree()
+            st = bvh.stage_times()
+            tot = st["morton"] + st["sort"] + st["topology"] + st["refit"]
+            if best is None or tot < best[0]:
+                best = (tot, st)
+        tot, st = best
+        row = {"n": n, "build_ms": round(tot, 4), "mtri_s": round(n / tot / 1e3, 1),
+               "working_set_mb": round(n * (104 + 120 + 64 + 48 + 16) / 2 ** 20, 1)}
+        for k, b in STAGE_BYTES.items():
+            row[f"{k}_ms"] = round(st[k], 4)
+            row[f"{k}_gbs"] = round(b * n / (st[k] * 1e-3) / 1e9, 1) if st[k] > 0 else None
+        print(json.dumps(row), flush=True)
+        bvh.close()
+        d_sh.free()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--build-sweep", type=str, default=None,
+                    help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
+    if args.build_sweep:
+        build_sweep([1 << int(x) for x in args.build_sweep.split(",")])
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by scene or "
               f"pixel tile, see the design notes)", file=sys.stderr)

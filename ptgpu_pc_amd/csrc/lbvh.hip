@@ -473,6 +473,85 @@ __global__ __launch_bounds__(256) void k_refit(const Shape *shapes, const unsign
     }
 }
 
+// -------------------------------------------------------------------------------------------
+// Stage 4 (index-keyed trees): fence-free refit by level passes.
+// The Karras tree over leaf indices is the binary radix tree of 0..N-1, so every internal node
+// covers an aligned block [j*2^s, min((j+1)*2^s, N) - 1] of leaves, with s = ceil(log2(len)),
+// and its Karras index is a closed form of (lo, hi, N): the node is the LEFT child of its parent
+// (index hi) iff bit s of lo is 0 and leaves remain after the block, otherwise it is the right
+// child or the root (index lo).  One workgroup reduces 1024 consecutive items (leaves, or subtree
+// roots of the previous pass) bottom-up in LDS, merging (left, right) exactly as BVH.cl:112-122,
+// writes every internal node's bound and TNode on the way, and hands its block root to the next
+// pass.  No atomics, no cross-workgroup visibility, deterministic; identical results to the
+// visit-flag refit (k_refit), which remains for trees whose topology is not index-keyed.
+// -------------------------------------------------------------------------------------------
+constexpr int kRefitLog = 10;
+constexpr int kRefitItems = 1 << kRefitLog;
+
+__device__ __forceinline__ int karras_index(long long lo, long long hi, int n) {
+    const int len = (int)(hi - lo + 1);
+    const int s = 32 - __clz(len - 1);   // ceil(log2(len)), len >= 2
+    return (lo + (1ll << s) <= n - 1 && !((lo >> s) & 1)) ? (int)hi : (int)lo;
+}
+__device__ __forceinline__ int child_ref(long long lo, long long hi, int n) {
+    return lo == hi ? ~(int)lo : karras_index(lo, hi, n);
+}
+
+template <bool LEAF>
+__global__ __launch_bounds__(256) void k_refit_levels(const Shape *shapes, const unsigned *sorted, int n, int s0,
+                                                      int levels, const float *in_bounds, float *out_bounds,
+                                                      BVHNodeGPU *btn, BVHNodeGPU *btl, TNode *tnodes, Prim *prims) {
+    __shared__ float sb[6][kRefitItems];
+    const long long item_base = (long long)blockIdx.x * kRefitItems;
+    const long long m = ((long long)n + (1ll << s0) - 1) >> s0;   // items at level s0
+    for (int q = 0; q < kRefitItems / 256; q++) {
+        const int j = threadIdx.x + q * 256;
+        const long long it = item_base + j;
+        if (it >= m) continue;
+        Bound B;
+        if (LEAF) {
+            const int sid = (int)sorted[it];
+            const Shape *sh = shapes + sid;
+            B = load_bound(&sh->b);
+            store_bound(btl + it, B);
+            btl[it].nShape = sid;
+            prims[it] = make_prim(sh, sid);
+        } else {
+            const float *f = in_bounds + it * 6;
+            B.min_x = f[0]; B.max_x = f[1]; B.min_y = f[2]; B.max_y = f[3]; B.min_z = f[4]; B.max_z = f[5];
+        }
+        sb[0][j] = B.min_x; sb[1][j] = B.max_x; sb[2][j] = B.min_y;
+        sb[3][j] = B.max_y; sb[4][j] = B.min_z; sb[5][j] = B.max_z;
+    }
+    __syncthreads();
+    for (int lev = 1; lev <= levels; lev++) {
+        const int s = s0 + lev;
+        const int nodes_here = kRefitItems >> lev;
+        for (int j = threadIdx.x; j < nodes_here; j += 256) {
+            const long long lo = (item_base + ((long long)j << lev)) << s0;
+            const long long mid = lo + (1ll << (s - 1));
+            if (mid > (long long)n - 1) continue;   // right half empty (or block past the end): collapsed
+            const long long hi_full = lo + (1ll << s) - 1;
+            const long long hi = hi_full < n - 1 ? hi_full : (long long)n - 1;
+            const int ls = j << lev, rs = ls + (1 << (lev - 1));
+            Bound L, R;
+            L.min_x = sb[0][ls]; L.max_x = sb[1][ls]; L.min_y = sb[2][ls];
+            L.max_y = sb[3][ls]; L.min_z = sb[4][ls]; L.max_z = sb[5][ls];
+            R.min_x = sb[0][rs]; R.max_x = sb[1][rs]; R.min_y = sb[2][rs];
+            R.max_y = sb[3][rs]; R.min_z = sb[4][rs]; R.max_z = sb[5][rs];
+            const Bound M = merge_bounds(L, R);
+            sb[0][ls] = M.min_x; sb[1][ls] = M.max_x; sb[2][ls] = M.min_y;
+            sb[3][ls] = M.max_y; sb[4][ls] = M.min_z; sb[5][ls] = M.max_z;
+            const int idx = karras_index(lo, hi, n);
+            store_bound(btn + idx, M);
+            const int rl = child_ref(lo, mid - 1, n), rr = child_ref(mid, hi, n);
+            tnodes[idx] = make_tnode(L, R, rl, rr, rl < 0 ? 1 : 0, rr < 0 ? 1 : 0);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 6 && out_bounds) out_bounds[(long long)blockIdx.x * 6 + threadIdx.x] = sb[threadIdx.x][0];
+}
+
 }  // namespace ptgpu
 
 // ---------------------------------------------------------------------------------------------
@@ -519,6 +598,38 @@ static int clbvh_construct(ptgpu_clbvh *h) {
     PT_HIP(hipEventRecord(h->ev[1], st));
     h->timed[0] = true;
     h->state = 1;
+    return PTGPU_OK;
+}
+
+// Refit of the index-keyed tree: ceil(log2 N) levels in passes of kRefitLog levels.
+static int clbvh_refit_levels(ptgpu_clbvh *h) {
+    hipStream_t st = h->ctx->stream;
+    const int n = h->n;
+    int L = 0;
+    while ((1ll << L) < n) L++;
+    const size_t per = (((size_t)n + kRefitItems - 1) / kRefitItems + 1) * 6;
+    PT_TRY(h->opt_tmp.ensure(2 * per * sizeof(float)));
+    float *buf[2] = {h->opt_tmp.as<float>(), h->opt_tmp.as<float>() + per};
+    int s0 = 0, pass = 0;
+    const float *in = nullptr;
+    do {
+        const int levels = (L - s0) < kRefitLog ? (L - s0) : kRefitLog;
+        const long long m = ((long long)n + (1ll << s0) - 1) >> s0;
+        const unsigned blocks = (unsigned)((m + kRefitItems - 1) / kRefitItems);
+        float *out = (s0 + levels < L) ? buf[pass & 1] : nullptr;
+        if (s0 == 0)
+            hipLaunchKernelGGL(k_refit_levels<true>, dim3(blocks), dim3(256), 0, st, h->d_shapes, h->sorted, n, 0,
+                               levels, (const float *)nullptr, out, h->btn.as<BVHNodeGPU>(), h->btl.as<BVHNodeGPU>(),
+                               h->tnodes.as<TNode>(), h->prims.as<Prim>());
+        else
+            hipLaunchKernelGGL(k_refit_levels<false>, dim3(blocks), dim3(256), 0, st, h->d_shapes, h->sorted, n, s0,
+                               levels, in, out, h->btn.as<BVHNodeGPU>(), h->btl.as<BVHNodeGPU>(),
+                               h->tnodes.as<TNode>(), h->prims.as<Prim>());
+        PT_HIP(hipGetLastError());
+        in = out;
+        s0 += levels;
+        pass++;
+    } while (s0 < L);
     return PTGPU_OK;
 }
 
@@ -663,11 +774,7 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     if (h->digit_bits == 4) PT_TRY(clbvh_sort<4>(h));
     else PT_TRY(clbvh_sort<8>(h));
     PT_HIP(hipEventRecord(h->ev[5], st));
-    if (n > 1) PT_TRY(fill_u32(st, h->flags.p, 0u, (size_t)(n - 1)));
-    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, st, h->d_shapes, h->sorted, n,
-                       h->btn.as<BVHNodeGPU>(), h->btl.as<BVHNodeGPU>(), h->flags.as<unsigned>(),
-                       h->tnodes.as<TNode>(), h->prims.as<Prim>());
-    PT_HIP(hipGetLastError());
+    PT_TRY(clbvh_refit_levels(h));
     PT_HIP(hipEventRecord(h->ev[6], st));
     h->timed[1] = true;
     h->timed[3] = true;
