@@ -154,6 +154,14 @@ int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl
 int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl);
 /* Device pointer to sorted_geometry_indices (CLBVH.cpp:257), N ints. */
 int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted);
+/* Tree used by the traversal layout (ptgpu_render / ptgpu_trace_*) after the next build_bvh_tree:
+ * 0 = the exported index-keyed Karras tree (BVH.cl:127-134), 1 = the Morton-keyed LBVH over the same
+ * sorted leaves (default; about half the SAH cost).  btn/btl are the index-keyed tree either way. */
+int ptgpu_clbvh_set_traversal_topology(ptgpu_clbvh *h, int mode);
+/* Read back the traversal tree (N-1 internal nodes): refs4[i] = {left ref, right ref, lo, hi} (ref < 0 is
+ * leaf ~k; [lo, hi] the node's leaf range) and boxes12[i] = left child Bound, right child Bound; *height =
+ * root height.  For tests and tools. */
+int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12, int *height);
 /* Radix-sort digit width for the next build_bvh_tree (4 or 8 bits; default 8). */
 int ptgpu_clbvh_set_digit_bits(ptgpu_clbvh *h, int bits);
 /* Per-stage device times of the last build in ms: morton, sort, topology, refit, optimize. */

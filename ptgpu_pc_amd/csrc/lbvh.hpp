@@ -17,6 +17,9 @@ struct ptgpu_clbvh {
     ptgpu::DevBuf btn, btl, flags, tnodes, prims;
     ptgpu::DevBuf hist, status, misc;
     ptgpu::DevBuf lights, scan, opt_tmp;
+    ptgpu::DevBuf mnode, mpar, gbi, xlist;   // Morton-keyed traversal tree: topology, parents, refit scratch
+    int trav_topology = 1;    // traversal tree: 0 = the exported index-keyed tree, 1 = Morton-keyed LBVH
+    bool height_pending = false;
     unsigned *sorted = nullptr;
     int lights_for = -1, nlights = 0;
     ptgpu::DevAccel acc;

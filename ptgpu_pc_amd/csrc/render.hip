@@ -823,6 +823,7 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
                          DevAccel *acc);
 
 extern "C" int ptgpu_kdtree_device_accel(ptgpu_kdtree *kd, ptgpu::DevAccel *acc);
+namespace ptgpu { int clbvh_prepare_traversal(ptgpu_clbvh *h); }
 
 static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes, unsigned n,
                          DevAccel *acc) {
@@ -830,6 +831,7 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
     if (bvh) {
         PT_CHECK(bvh->state >= 3, PTGPU_ERR_INVALID, "BVH not built (call build_bvh_tree)");
         PT_CHECK((unsigned)bvh->n == n, PTGPU_ERR_INVALID, "shapeCnt %u differs from the BVH's %d", n, bvh->n);
+        PT_TRY(clbvh_prepare_traversal(bvh));
         *acc = bvh->acc;
         return PTGPU_OK;
     }

@@ -77,6 +77,8 @@ _SIGS = {
     "ptgpu_clbvh_build_bvh_tree": [vp],
     "ptgpu_clbvh_optimize": [vp, i32],
     "ptgpu_clbvh_set_digit_bits": [vp, i32],
+    "ptgpu_clbvh_set_traversal_topology": [vp, i32],
+    "ptgpu_clbvh_export_traversal": [vp, vp, vp, C.POINTER(i32)],
     "ptgpu_clbvh_get_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
     "ptgpu_clbvh_device_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
     "ptgpu_clbvh_device_sorted": [vp, C.POINTER(vp)],
@@ -262,7 +264,7 @@ class CLBVH:
     shapes: a host SHAPE_DTYPE array (index/morton_code/b are written back into it, as the
     reference does) or a DeviceArray holding n shapes (updated in place on the device)."""
 
-    def __init__(self, ctx: Context, shapes, n: int | None = None, digit_bits: int = 8):
+    def __init__(self, ctx: Context, shapes, n: int | None = None, digit_bits: int = 8, traversal: str = "morton"):
         self.ctx = ctx
         h = vp()
         if isinstance(shapes, DeviceArray):
@@ -279,6 +281,12 @@ class CLBVH:
         ctx._adopt(self)
         if digit_bits != 8:
             _check(ctx.lib.ptgpu_clbvh_set_digit_bits(self.handle, digit_bits))
+        if traversal != "morton":
+            self.set_traversal(traversal)
+
+    def set_traversal(self, topology: str) -> None:
+        """'morton' (Morton-keyed LBVH, default) or 'reference' (the exported index-keyed tree)."""
+        _check(self.ctx.lib.ptgpu_clbvh_set_traversal_topology(self.handle, {"reference": 0, "morton": 1}[topology]))
 
     def set_digit_bits(self, bits: int) -> None:
         _check(self.ctx.lib.ptgpu_clbvh_set_digit_bits(self.handle, bits))
@@ -306,6 +314,15 @@ class CLBVH:
             C.memmove(_ptr(btn), a.value, btn.nbytes)
         C.memmove(_ptr(btl), b.value, btl.nbytes)
         return btn, btl
+
+    def export_traversal(self):
+        """(refs4 [N-1,4] = left, right, lo, hi; boxes [N-1,2,6]; height) of the traversal tree."""
+        m = max(self.n - 1, 0)
+        refs = np.zeros((max(m, 1), 4), dtype=np.int32)
+        boxes = np.zeros((max(m, 1), 2, 6), dtype=np.float32)
+        hgt = i32()
+        _check(self.ctx.lib.ptgpu_clbvh_export_traversal(self.handle, _ptr(refs), _ptr(boxes), C.byref(hgt)))
+        return refs[:m], boxes[:m], hgt.value
 
     def device_trees(self):
         a, b = vp(), vp()

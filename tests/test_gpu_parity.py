@@ -81,6 +81,63 @@ def test_lbvh_bitexact_config_sizes(ctx, which):
     assert (np.diff(keys.astype(np.int64)) <= 0).all()
 
 
+def check_traversal_tree(bvh, s):
+    """Invariants of the traversal tree: a binary tree over all leaves, each child box the exact
+    union (select-merge) of its leaf range, children partition the parent's range."""
+    refs, boxes, height = bvh.export_traversal()
+    n = bvh.n
+    srt = bvh.sorted_indices()
+    lb = s["b"][srt]
+    seen = np.zeros(n, np.int32)
+    seen_int = np.zeros(max(n - 1, 1), np.int32)
+    seen_int[0] = 1
+    for i in range(n - 1):
+        l, r, lo, hi = refs[i]
+        llo, lhi = (~l, ~l) if l < 0 else (refs[l][2], refs[l][3])
+        rlo, rhi = (~r, ~r) if r < 0 else (refs[r][2], refs[r][3])
+        assert llo == lo and rhi == hi and lhi + 1 == rlo, (i, refs[i])
+        for side, (a, b) in enumerate(((llo, lhi), (rlo, rhi))):
+            seg = lb[a:b + 1]
+            exp = [seg[:, 0].min(), seg[:, 1].max(), seg[:, 2].min(), seg[:, 3].max(), seg[:, 4].min(), seg[:, 5].max()]
+            assert boxes[i, side].tolist() == [float(x) for x in exp], (i, side)
+        for c in (l, r):
+            if c < 0:
+                seen[~c] += 1
+            else:
+                seen_int[c] += 1
+    assert (seen == 1).all() and (seen_int == 1).all()
+    assert 0 < height <= 64
+    return height
+
+
+@pytest.mark.parametrize("topology", ["morton", "reference"])
+@pytest.mark.parametrize("n", [2, 3, 1000, 1025, 4097, 70000])
+def test_traversal_tree_invariants(ctx, topology, n):
+    shapes = scenes.soup(n, seed=40 + n)[0] if n > 2 else scenes.soup(n, seed=3)[0][:n]
+    say("case traversal", topology, n)
+    s = shapes.copy()
+    bvh = pt.CLBVH(ctx, s, traversal=topology)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    h = check_traversal_tree(bvh, s)
+    say("height", h)
+
+
+def test_traversal_tree_with_duplicate_codes(ctx):
+    # many shapes share a centroid (equal Morton codes): ties break on the leaf index
+    sph = np.zeros(3000, dtype=SHAPE_DTYPE)
+    sph["type"] = SPHERE
+    sph["rad"] = 0.1
+    rng = np.random.default_rng(2)
+    sph["p"] = np.round(rng.random((3000, 3)) * 4) / 4
+    say("case duplicates")
+    s = sph.copy()
+    bvh = pt.CLBVH(ctx, s)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    check_traversal_tree(bvh, s)
+
+
 def test_rebuild_is_deterministic(ctx):
     shapes = scenes.soup(100000, seed=9)[0]
     say("case rebuild x3")
