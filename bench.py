@@ -122,6 +122,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine", default="auto", choices=["auto", "megakernel", "wavefront", "lanes"])
+    ap.add_argument("--regroup", type=int, default=None)
+    ap.add_argument("--blocks-per-cu", type=int, default=None)
+    ap.add_argument("--wide", type=int, default=None)
     ap.add_argument("--build-sweep", type=str, default=None,
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
@@ -141,6 +145,13 @@ def main():
     pt.update_camera(cam)
     n = len(shapes)
     ctx = pt.Context(0)
+    ctx.set_engine(args.engine)
+    if args.regroup:
+        ctx.set_tuning("regroup", args.regroup)
+    if args.blocks_per_cu:
+        ctx.set_tuning("blocks_per_cu", args.blocks_per_cu)
+    if args.wide is not None:
+        ctx.set_tuning("wide", args.wide)
     d_shapes = ctx.to_device(shapes)
     d_cam = ctx.to_device(cam)
     d_col = ctx.alloc(W * H * 12, np.float32).zero()
@@ -163,13 +174,21 @@ def main():
                   ray_counter=ray_counter, trav_stats=trav_stats)
         ctx.record(2)
 
+    # counter calibration: one plain copy of a known 256 MiB (beyond the 256 MB MALL)
+    cal_src = ctx.alloc(256 << 20, np.uint8)
+    cal_dst = ctx.alloc(256 << 20, np.uint8)
+    ctx.copy_calibration(cal_dst, cal_src, 256 << 20)
+    ctx.synchronize()
+    cal_src.free()
+    cal_dst.free()
+
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
     # one counting frame (untimed) for the roofline's algorithmic bytes
     step(trav_stats=d_stats)
     ctx.synchronize()
-    node_visits, prim_tests = (int(x) for x in d_stats.download(np.uint64)[:2])
+    box_tests, prim_tests = (int(x) for x in d_stats.download(np.uint64)[:2])
 
     d_rays.zero()
     ctx.synchronize()
@@ -188,15 +207,15 @@ def main():
     render_avg = float(np.mean(render_ms))
     build_avg = float(np.mean(build_ms))
 
-    # algorithmic bytes of one render launch: two 24-B child boxes per node visit, 36 B of vertices
-    # per triangle test, and the frame buffers (colors 12 B read+write, seeds 8 B read+write,
-    # pixels 4 B write per pixel)
-    alg_bytes = 48 * node_visits + 36 * prim_tests + W * H * (24 + 16 + 4)
+    # algorithmic bytes of one render launch: 24 B per child box tested, 36 B of vertices per
+    # triangle test, and the frame buffers (colors 12 B read+write, seeds 8 B read+write, pixels
+    # 4 B write per pixel)
+    alg_bytes = 24 * box_tests + 36 * prim_tests + W * H * (24 + 16 + 4)
     achieved = alg_bytes / (render_avg * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_render (path-tracing megakernel)",
-                "per_launch": {"node_visits": node_visits, "prim_tests": prim_tests, "alg_bytes": alg_bytes,
+                "per_launch": {"box_tests": box_tests, "prim_tests": prim_tests, "alg_bytes": alg_bytes,
                                "avg_ms": round(render_avg, 4)}}
 
     cpu = None

@@ -112,6 +112,15 @@ int ptgpu_synchronize(ptgpu_ctx *ctx);
 /* MAX_DEPTH (geom.h:32) used by ptgpu_radiance_gpu: the reference fixes it at compile time (6);
  * here it is a per-context value, default 6. */
 int ptgpu_ctx_set_max_depth(ptgpu_ctx *ctx, int max_depth);
+/* Path-tracing schedule: 0 auto (3 when a tree is walked, else 1), 1 megakernel with wave-lockstep
+ * queries, 2 wavefront (needs a tree and lightCnt <= 8), 3 megakernel with per-lane states (a lane
+ * whose query ends waits for shading while the rest of its wave keeps traversing, over four-wide
+ * nodes).  All give the same bits. */
+int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine);
+/* Launch tuning, no effect on results: "regroup" (1..64, ready lanes that start a shading pass in
+ * engine 3, default 8), "blocks_per_cu" (1..64, persistent workgroups per CU, default 2), "wide"
+ * (0/1, engine 3 walks four-wide nodes, default 1). */
+int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value);
 int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);
 int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr);
 int ptgpu_memcpy_h2d(ptgpu_ctx *ctx, void *d_dst, const void *h_src, size_t bytes); /* synchronous */
@@ -121,6 +130,9 @@ int ptgpu_memcpy_d2d(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t byte
 /* Host buffers returned by ptgpu_kdtree_get_trees are released with this. */
 void ptgpu_host_free(void *h_ptr);
 
+/* A plain device copy kernel (bytes a multiple of 16) with known traffic, for calibrating hardware
+ * memory-traffic counters. */
+int ptgpu_copy_calibration(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 /* Stream-ordered timing: record a named event slot (0..15) and read the elapsed ms between two. */
 int ptgpu_event_record(ptgpu_ctx *ctx, int slot);
 int ptgpu_event_elapsed_ms(ptgpu_ctx *ctx, int slot_begin, int slot_end, float *ms);

@@ -44,16 +44,33 @@ struct alignas(16) TNode {
 };
 static_assert(sizeof(TNode) == 64, "TNode");
 
+// Four-wide traversal node, one per internal TNode: the children of binary node i's children
+// (a leaf child stays a leaf), so a walk from the root that always jumps two binary levels
+// touches only every other level.  Boxes are stored per axis, child k in component k.
+struct alignas(16) TNode4 {
+    float4 lox, loy, loz;
+    float4 hix, hiy, hiz;
+    int4 ref;   // >= 0 internal (TNode4 index), < 0 leaf with first prim ~ref, kEmptyRef unused
+    int4 cnt;   // leaf primitive counts
+};
+static_assert(sizeof(TNode4) == 128, "TNode4");
+constexpr int kEmptyRef = (int)0x80000000;
+constexpr float kFarBox = 1e30f;   // coordinates of an unused TNode4 slot
+
 struct alignas(16) Prim {
     float4 p0;  // v0.xyz | centre.xyz, w = shape index (bits)
-    float4 p1;  // e1.xyz | (rad, 0, 0), w = type (bits)
-    float4 p2;  // e2.xyz
+    float4 p1;  // e1.xyz | (rad, 0, 0), w = type (bits 0-7) | kLastInLeaf
+    float4 p2;  // e2.xyz, w = type again (the test branches on it, so all three loads issue together)
 };
 static_assert(sizeof(Prim) == 48, "Prim");
+// Set in Prim::p1.w on the last primitive of each leaf range when the leaf ranges are disjoint, so
+// a traversal can hold a whole leaf as the index of its first primitive.
+constexpr int kLastInLeaf = 0x100;
 
 // A prepared traversal structure on the device.
 struct DevAccel {
     const TNode *nodes = nullptr;
+    const TNode4 *nodes4 = nullptr;   // optional four-wide view of `nodes` (same indices)
     const Prim *prims = nullptr;
     int root = 0;          // >= 0 internal node; < 0: the whole scene is one leaf ~root
     int root_cnt = 0;      // primitive count when the root is a leaf
@@ -62,6 +79,7 @@ struct DevAccel {
     int nshapes = 0;       // shapeCnt: any-hit queries ignore shape nshapes-1 (IntersectP, Q10)
     int depth = 0;         // max internal-node depth (root = 0); stack bound
     int mode = PTGPU_ACCEL_NONE;
+    int leaf_runs = 0;     // 1: every leaf range is disjoint and its last Prim carries kLastInLeaf
 };
 
 // ---- errors ----
@@ -106,8 +124,13 @@ struct ptgpu_ctx {
     hipEvent_t ev[16] = {};
     int cu_count = 0;
     int max_depth = 6;   // MAX_DEPTH (geom.h:32) for ptgpu_radiance_gpu
+    int engine = 0;      // 0 auto, 1 megakernel (lockstep queries), 2 wavefront, 3 megakernel (per-lane states)
+    int regroup = 8;    // per-lane-state megakernel: ready lanes that trigger a shading pass
+    int blocks_per_cu = 2;
+    int wide = 1;        // per-lane-state megakernel: walk the four-wide nodes when they exist
+    void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
-    ptgpu::DevBuf tmp_nodes, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
+    ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
 };
 
 namespace ptgpu {
@@ -117,6 +140,8 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
                      DevBuf &scan_tmp, int *found);
 // brute-force layout: prims in shape-index order
 int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_prims);
+// four-wide nodes for a prepared binary structure (no-op without internal nodes); sets acc->nodes4
+int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4);
 // reference-layout trees -> traversal layout
 int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn,
                     const BVHNodeGPU *d_btl, DevBuf &nodes, DevBuf &prims, DevAccel *acc);
@@ -137,6 +162,9 @@ struct RenderLaunch {
     unsigned long long *trav_stats;
 };
 int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
+int launch_render_wavefront(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
+int wf_supported(int nlights);
+void wf_release(ptgpu_ctx *ctx);
 int launch_trace(ptgpu_ctx *ctx, const DevAccel &acc, unsigned int shapeCnt, const Ray *d_rays,
                  const float *d_maxt, int n, float *d_t, int *d_id, int *d_hit, bool any);
 }  // namespace ptgpu

@@ -110,10 +110,12 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->tmp_nodes.release();
+    ctx->tmp_nodes4.release();
     ctx->tmp_prims.release();
     ctx->tmp_lights.release();
     ctx->tmp_scan.release();
     ctx->tmp_counter.release();
+    wf_release(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->own_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -124,6 +126,31 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
 int ptgpu_synchronize(ptgpu_ctx *ctx) {
     PT_CHECK(ctx, PTGPU_ERR_INVALID, "ctx is NULL");
     PT_HIP(hipStreamSynchronize(ctx->stream));
+    return PTGPU_OK;
+}
+
+int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine) {
+    PT_CHECK(ctx, PTGPU_ERR_INVALID, "ctx is NULL");
+    PT_CHECK(engine >= 0 && engine <= 3, PTGPU_ERR_INVALID,
+             "engine %d (0 auto, 1 lockstep megakernel, 2 wavefront, 3 per-lane-state megakernel)", engine);
+    ctx->engine = engine;
+    return PTGPU_OK;
+}
+
+int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value) {
+    PT_CHECK(ctx && name, PTGPU_ERR_INVALID, "null argument");
+    if (!strcmp(name, "regroup")) {
+        PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "regroup %d not in [1, 64]", value);
+        ctx->regroup = value;
+    } else if (!strcmp(name, "blocks_per_cu")) {
+        PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "blocks_per_cu %d not in [1, 64]", value);
+        ctx->blocks_per_cu = value;
+    } else if (!strcmp(name, "wide")) {
+        PT_CHECK(value == 0 || value == 1, PTGPU_ERR_INVALID, "wide %d not in {0, 1}", value);
+        ctx->wide = value;
+    } else {
+        PT_CHECK(false, PTGPU_ERR_INVALID, "unknown tuning parameter '%s'", name);
+    }
     return PTGPU_OK;
 }
 

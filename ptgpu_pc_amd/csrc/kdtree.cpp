@@ -89,7 +89,7 @@ struct ptgpu_kdtree {
     std::vector<KNode> nodes;     // depth-first pre-order
     std::vector<int> kn;          // shape indices, leaves in depth-first order
     std::vector<Shape> shapes;    // host copy (for the heap export bounds and prims)
-    DevBuf d_nodes, d_prims;
+    DevBuf d_nodes, d_nodes4, d_prims;
     DevAccel acc;
     bool sah = false;
 };
@@ -288,20 +288,20 @@ void build_sah(ptgpu_kdtree *T, const std::vector<Bound> &bnd, int max_depth, in
     }
 }
 
-Prim host_prim(const Shape &s, int id) {
+Prim host_prim(const Shape &s, int id, bool last) {
     Prim p;
-    int type = (int)s.type;
+    int type = (int)s.type | (last ? kLastInLeaf : 0);
     float fid, ftype;
     memcpy(&fid, &id, 4);
     memcpy(&ftype, &type, 4);
     if (s.type == SPHERE) {
         p.p0 = make_float4(s.s.p.x, s.s.p.y, s.s.p.z, fid);
         p.p1 = make_float4(s.s.rad, 0.f, 0.f, ftype);
-        p.p2 = make_float4(0.f, 0.f, 0.f, 0.f);
+        p.p2 = make_float4(0.f, 0.f, 0.f, ftype);
     } else {
         p.p0 = make_float4(s.t.p1.x, s.t.p1.y, s.t.p1.z, fid);
         p.p1 = make_float4(s.t.p2.x - s.t.p1.x, s.t.p2.y - s.t.p1.y, s.t.p2.z - s.t.p1.z, ftype);
-        p.p2 = make_float4(s.t.p3.x - s.t.p1.x, s.t.p3.y - s.t.p1.y, s.t.p3.z - s.t.p1.z, 0.f);
+        p.p2 = make_float4(s.t.p3.x - s.t.p1.x, s.t.p3.y - s.t.p1.y, s.t.p3.z - s.t.p1.z, ftype);
     }
     return p;
 }
@@ -335,7 +335,11 @@ int upload_device(ptgpu_kdtree *T) {
         tn[inum[k]] = t;
     }
     std::vector<Prim> pr(T->kn.size() > 0 ? T->kn.size() : 1);
-    for (size_t q = 0; q < T->kn.size(); q++) pr[q] = host_prim(T->shapes[T->kn[q]], T->kn[q]);
+    // leaves own disjoint runs [start, start + count) of kn: flag the last primitive of each run
+    std::vector<char> last(T->kn.size() + 1, 0);
+    for (const KNode &nd : T->nodes)
+        if (nd.leaf && nd.count > 0) last[(size_t)nd.start + nd.count - 1] = 1;
+    for (size_t q = 0; q < T->kn.size(); q++) pr[q] = host_prim(T->shapes[T->kn[q]], T->kn[q], last[q] != 0);
     PT_TRY(T->d_nodes.ensure(tn.size() * sizeof(TNode)));
     PT_TRY(T->d_prims.ensure(pr.size() * sizeof(Prim)));
     hipStream_t st = T->ctx->stream;
@@ -348,6 +352,7 @@ int upload_device(ptgpu_kdtree *T) {
     A.nprims = (int)T->kn.size();
     A.nnodes = ni;
     A.nshapes = T->n;
+    A.leaf_runs = 1;
     A.mode = PTGPU_ACCEL_KD;
     const KNode &root = T->nodes[0];
     if (root.leaf) {
@@ -358,6 +363,8 @@ int upload_device(ptgpu_kdtree *T) {
         A.root_cnt = 0;
     }
     A.depth = T->maxdepth;
+    A.nodes4 = nullptr;
+    PT_TRY(build_wide(T->ctx, &A, T->d_nodes4));
     return PTGPU_OK;
 }
 
@@ -494,6 +501,7 @@ int ptgpu_kdtree_destroy(ptgpu_kdtree *kd) {
     if (!kd) return PTGPU_OK;
     if (kd->ctx && kd->ctx->stream) (void)hipStreamSynchronize(kd->ctx->stream);
     kd->d_nodes.release();
+    kd->d_nodes4.release();
     kd->d_prims.release();
     delete kd;
     return PTGPU_OK;

@@ -20,6 +20,14 @@ __global__ void k_fill_u32(unsigned *p, unsigned v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// Plain streaming copy (16-byte vectors, grid-stride): a kernel of known traffic for calibrating
+// the FETCH_SIZE / WRITE_SIZE counters.
+__global__ __launch_bounds__(256) void k_copy_calib(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words) {
     if (n_words == 0) return PTGPU_OK;
     size_t blocks = (n_words + 255) / 256;
@@ -417,17 +425,17 @@ __global__ __launch_bounds__(256) void k_karras(int n, BVHNodeGPU *btn, BVHNodeG
 // -------------------------------------------------------------------------------------------
 __device__ __forceinline__ Prim make_prim(const Shape *s, int id) {
     const float *w = reinterpret_cast<const float *>(s);
-    int type = reinterpret_cast<const int *>(s)[0];
+    const int type = reinterpret_cast<const int *>(s)[0] | kLastInLeaf;   // one primitive per leaf
     Prim p;
-    if (type == SPHERE) {
+    if ((type & 0xff) == SPHERE) {
         p.p0 = make_float4(w[2], w[3], w[4], __int_as_float(id));
         p.p1 = make_float4(w[1], 0.f, 0.f, __int_as_float(type));
-        p.p2 = make_float4(0.f, 0.f, 0.f, 0.f);
+        p.p2 = make_float4(0.f, 0.f, 0.f, __int_as_float(type));
     } else {
         float x1 = w[1], y1 = w[2], z1 = w[3];
         p.p0 = make_float4(x1, y1, z1, __int_as_float(id));
         p.p1 = make_float4(w[4] - x1, w[5] - y1, w[6] - z1, __int_as_float(type));
-        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, 0.f);
+        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, __int_as_float(type));
     }
     return p;
 }
@@ -880,17 +888,31 @@ namespace ptgpu {
 // The Morton-keyed tree's height is known on the device after the refit; the traversal stack is
 // sized from it, so the first use after a build reads it back (one 4-byte copy).
 int clbvh_prepare_traversal(ptgpu_clbvh *h) {
-    if (!h->height_pending) return PTGPU_OK;
-    unsigned height = 0;
-    PT_HIP(hipMemcpyAsync(&height, misc_counters(h) + 11, 4, hipMemcpyDeviceToHost, h->ctx->stream));
-    PT_HIP(hipStreamSynchronize(h->ctx->stream));
-    h->acc.depth = (int)height;
-    h->height_pending = false;
+    if (h->height_pending) {
+        unsigned height = 0;
+        PT_HIP(hipMemcpyAsync(&height, misc_counters(h) + 11, 4, hipMemcpyDeviceToHost, h->ctx->stream));
+        PT_HIP(hipStreamSynchronize(h->ctx->stream));
+        h->acc.depth = (int)height;
+        h->height_pending = false;
+    }
+    if (h->wide_pending) {
+        PT_TRY(build_wide(h->ctx, &h->acc, h->tnodes4));
+        h->wide_pending = false;
+    }
     return PTGPU_OK;
 }
 }  // namespace ptgpu
 
 extern "C" {
+
+int ptgpu_copy_calibration(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
+    PT_CHECK(ctx && d_dst && d_src && bytes % 16 == 0, PTGPU_ERR_INVALID, "bad calibration copy arguments");
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy_calib, dim3(ctx->cu_count * 8), dim3(256), 0, ctx->stream, (const uint4 *)d_src,
+                       (uint4 *)d_dst, n);
+    PT_HIP(hipGetLastError());
+    return PTGPU_OK;
+}
 
 int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12, int *height) {
     PT_CHECK(h && h_refs4 && h_boxes12, PTGPU_ERR_INVALID, "null argument");
@@ -1053,6 +1075,9 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     h->acc.nnodes = n - 1;
     h->acc.nshapes = n;
     h->acc.mode = PTGPU_ACCEL_BVH;
+    h->acc.leaf_runs = 1;
+    h->acc.nodes4 = nullptr;
+    h->wide_pending = n >= 2;   // four-wide nodes are derived at the first traversal after a build
     if (n >= 2) {
         h->acc.root = 0;
         h->acc.root_cnt = 0;
@@ -1135,7 +1160,7 @@ int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
     h->btn.release(); h->btl.release(); h->flags.release(); h->tnodes.release(); h->prims.release();
     h->hist.release(); h->status.release(); h->misc.release(); h->own_shapes.release();
     h->lights.release(); h->scan.release(); h->opt_tmp.release();
-    h->mnode.release(); h->mpar.release(); h->gbi.release(); h->xlist.release();
+    h->mnode.release(); h->mpar.release(); h->gbi.release(); h->xlist.release(); h->tnodes4.release();
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     delete h;

@@ -18,6 +18,8 @@ struct ptgpu_clbvh {
     ptgpu::DevBuf hist, status, misc;
     ptgpu::DevBuf lights, scan, opt_tmp;
     ptgpu::DevBuf mnode, mpar, gbi, xlist;   // Morton-keyed traversal tree: topology, parents, refit scratch
+    ptgpu::DevBuf tnodes4;                   // four-wide view of tnodes, rebuilt after each refit
+    bool wide_pending = false;
     int trav_topology = 1;    // traversal tree: 0 = the exported index-keyed tree, 1 = Morton-keyed LBVH
     bool height_pending = false;
     unsigned *sorted = nullptr;

@@ -18,232 +18,10 @@
 #include <cmath>
 #include <cstring>
 
-#include "device_util.hpp"
 #include "lbvh.hpp"
+#include "render_device.hpp"
 
 namespace ptgpu {
-
-constexpr float kSlack = 1.0f + 1.0e-5f;   // relative slack on the slab test's exit distance
-constexpr float kInf = 1e20f;              // the reference's initial t (rendering_kernel.cl:228)
-constexpr unsigned kMaxVisits = 1u << 24;  // node visits per query; only a cyclic tree reaches it
-
-struct f3 { float x, y, z; };
-__device__ __forceinline__ f3 mk(float x, float y, float z) { f3 v; v.x = x; v.y = y; v.z = z; return v; }
-__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ f3 smul(float k, f3 b) { return mk(k * b.x, k * b.y, k * b.z); }
-__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
-    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ f3 norm3(f3 v) { float l = 1.f / sqrtf(dot3(v, v)); return smul(l, v); }
-__device__ __forceinline__ bool iszero3(f3 v) { return v.x == 0.f && v.y == 0.f && v.z == 0.f; }
-__device__ __forceinline__ f3 ldvec(const Vec *p) {
-    const float *f = reinterpret_cast<const float *>(p);
-    return mk(f[0], f[1], f[2]);
-}
-
-// ---- GetRandom, rendering_kernel.cl:4-17 ----
-__device__ __forceinline__ float get_random(unsigned &s0, unsigned &s1) {
-    s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
-    s1 = 18000u * (s1 & 65535u) + (s1 >> 16);
-    unsigned ires = (s0 << 16) + s1;
-    float f = __uint_as_float((ires & 0x007fffffu) | 0x40000000u);
-    return (f - 2.f) / 2.f;
-}
-
-// ---- primitive tests (rendering_kernel.cl:26-125) on the Prim layout ----
-__device__ __forceinline__ float tri_isect(const float4 &p0, const float4 &p1, const float4 &p2, f3 o, f3 d) {
-    const f3 v0 = mk(p0.x, p0.y, p0.z), e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
-    const f3 pvec = cross3(d, e2);
-    const float det = dot3(e1, pvec);
-    if (det == 0.f) return 0.f;
-    const float inv = 1.0f / det;
-    const f3 tvec = sub3(o, v0);
-    const float u = dot3(tvec, pvec) * inv;
-    if (u < 0.0f || u > 1.0f) return 0.f;
-    const f3 qvec = cross3(tvec, e1);
-    const float v = dot3(d, qvec) * inv;
-    if (v < 0.0f || u + v > 1.0f) return 0.f;
-    const float t = dot3(e2, qvec) * inv;
-    return t > kEpsilon ? t : 0.f;
-}
-__device__ __forceinline__ float sph_isect(const float4 &p0, float rad, f3 o, f3 d) {
-    const f3 op = sub3(mk(p0.x, p0.y, p0.z), o);
-    const float b = dot3(op, d);
-    float det = b * b - dot3(op, op) + rad * rad;
-    if (det < 0.f) return 0.f;
-    det = sqrtf(det);
-    float t = b - det;
-    if (t > kEpsilon) return t;
-    t = b + det;
-    return t > kEpsilon ? t : 0.f;
-}
-__device__ __forceinline__ float prim_isect(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id) {
-    const float4 p0 = prims[k].p0, p1 = prims[k].p1;
-    *id = __float_as_int(p0.w);
-    if (__float_as_int(p1.w) == SPHERE) return sph_isect(p0, p1.x, o, d);
-    const float4 p2 = prims[k].p2;
-    return tri_isect(p0, p1, p2, o, d);
-}
-
-// slab test of one box; returns entry distance (>= 0) or +inf on a miss
-__device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
-                                      float tlimit) {
-    const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
-    const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
-    const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
-    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
-    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlimit));
-    return tn <= tf * kSlack ? tn : __builtin_inff();
-}
-
-// Closest hit over a prepared structure.  Returns the hit shape index or -1; t in *tout.
-template <int STACK>
-__device__ int trace_closest(const DevAccel &A, f3 o, f3 d, float *tout, int *stk, unsigned *nodes_visited,
-                             unsigned *prims_tested) {
-    float tb = kInf;
-    int bid = -1;
-    const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    auto test_leaf = [&](int first, int cnt) {
-        if (first < 0 || cnt > A.nprims - first) return;   // malformed caller tree
-        for (int q = 0; q < cnt; q++) {
-            int id;
-            const float t = prim_isect(A.prims, first + q, o, d, &id);
-            if (prims_tested) (*prims_tested)++;
-            if (t != 0.f && (t < tb || (t == tb && id > bid))) { tb = t; bid = id; }
-        }
-    };
-    if (A.root < 0) {
-        test_leaf(~A.root, A.root_cnt);
-        *tout = tb;
-        return bid;
-    }
-    int node = A.root, sp = 0;
-    for (unsigned guard = 0; guard < kMaxVisits; guard++) {
-        if ((unsigned)node >= (unsigned)A.nnodes) break;   // malformed caller tree
-        const TNode *np = A.nodes + node;
-        const float4 a = np->a, b = np->b, c = np->c;
-        const int4 r = np->r;
-        if (nodes_visited) (*nodes_visited)++;
-        float t0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tb);
-        float t1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tb);
-        if (r.x < 0 && t0 != __builtin_inff()) { test_leaf(~r.x, r.z); t0 = __builtin_inff(); }
-        if (r.y < 0 && t1 != __builtin_inff()) { test_leaf(~r.y, r.w); t1 = __builtin_inff(); }
-        const bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
-        if (h0 && h1) {
-            const bool first0 = t0 <= t1;
-            const int nearc = first0 ? r.x : r.y, farc = first0 ? r.y : r.x;
-            if (sp < STACK) stk[(sp++) * 256] = farc;
-            node = nearc;
-        } else if (h0) {
-            node = r.x;
-        } else if (h1) {
-            node = r.y;
-        } else {
-            if (sp == 0) break;
-            node = stk[(--sp) * 256];
-        }
-    }
-    *tout = tb;
-    return bid;
-}
-
-// Any hit with 0 != t < maxt over shapes other than `skip_id` (IntersectP semantics).
-template <int STACK>
-__device__ bool trace_any(const DevAccel &A, f3 o, f3 d, float maxt, int skip_id, int *stk,
-                          unsigned *nodes_visited, unsigned *prims_tested) {
-    const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    auto test_leaf = [&](int first, int cnt) -> bool {
-        if (first < 0 || cnt > A.nprims - first) return false;   // malformed caller tree
-        for (int q = 0; q < cnt; q++) {
-            int id;
-            const float t = prim_isect(A.prims, first + q, o, d, &id);
-            if (prims_tested) (*prims_tested)++;
-            if (t != 0.f && t < maxt && id != skip_id) return true;
-        }
-        return false;
-    };
-    if (A.root < 0) return test_leaf(~A.root, A.root_cnt);
-    int node = A.root, sp = 0;
-    for (unsigned guard = 0; guard < kMaxVisits; guard++) {
-        if ((unsigned)node >= (unsigned)A.nnodes) break;   // malformed caller tree
-        const TNode *np = A.nodes + node;
-        const float4 a = np->a, b = np->b, c = np->c;
-        const int4 r = np->r;
-        if (nodes_visited) (*nodes_visited)++;
-        float t0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, maxt);
-        float t1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, maxt);
-        if (r.x < 0 && t0 != __builtin_inff()) { if (test_leaf(~r.x, r.z)) return true; t0 = __builtin_inff(); }
-        if (r.y < 0 && t1 != __builtin_inff()) { if (test_leaf(~r.y, r.w)) return true; t1 = __builtin_inff(); }
-        const bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
-        if (h0 && h1) {
-            if (sp < STACK) stk[(sp++) * 256] = r.y;
-            node = r.x;
-        } else if (h0) {
-            node = r.x;
-        } else if (h1) {
-            node = r.y;
-        } else {
-            if (sp == 0) break;
-            node = stk[(--sp) * 256];
-        }
-    }
-    return false;
-}
-
-// Brute-force mode (ACCELSTR 0): prims in shape-index order.
-__device__ int brute_closest(const Prim *prims, int n, f3 o, f3 d, float *tout, unsigned *prims_tested) {
-    float tb = kInf;
-    int bid = -1;
-    for (int i = n - 1; i >= 0; i--) {
-        int id;
-        const float t = prim_isect(prims, i, o, d, &id);
-        if (t != 0.f && t < tb) { tb = t; bid = id; }
-    }
-    if (prims_tested) *prims_tested += (unsigned)n;
-    *tout = tb;
-    return bid;
-}
-__device__ bool brute_any(const Prim *prims, int n, f3 o, f3 d, float maxt, unsigned *prims_tested) {
-    for (int i = n - 2; i >= 0; i--) {
-        int id;
-        if (prims_tested) (*prims_tested)++;
-        const float t = prim_isect(prims, i, o, d, &id);
-        if (t != 0.f && t < maxt) return true;
-    }
-    return false;
-}
-
-template <int STACK>
-__device__ __forceinline__ int closest(const DevAccel &A, f3 o, f3 d, float *t, int *stk, unsigned *nv, unsigned *pt) {
-    if (A.mode == PTGPU_ACCEL_NONE) return brute_closest(A.prims, A.nprims, o, d, t, pt);
-    return trace_closest<STACK>(A, o, d, t, stk, nv, pt);
-}
-template <int STACK>
-__device__ __forceinline__ bool anyhit(const DevAccel &A, f3 o, f3 d, float maxt, int *stk, unsigned *nv, unsigned *pt) {
-    if (A.mode == PTGPU_ACCEL_NONE) return brute_any(A.prims, A.nprims, o, d, maxt, pt);
-    return trace_any<STACK>(A, o, d, maxt, A.nshapes - 1, stk, nv, pt);
-}
-
-// ---- camera ray, rendering_kernel.cl:1042-1070 ----
-__device__ __forceinline__ void camera_ray(const Camera *cam, unsigned &s0, unsigned &s1, int width, int height,
-                                           int x, int y, f3 &ro, f3 &rd) {
-    const float invW = 1.f / (float)width, invH = 1.f / (float)height;
-    const float r1 = get_random(s0, s1) - .5f;
-    const float r2 = get_random(s0, s1) - .5f;
-    const float kcx = ((float)x + r1) * invW - .5f;
-    const float kcy = ((float)y + r2) * invH - .5f;
-    const f3 cx = ldvec(&cam->x), cy = ldvec(&cam->y), cd = ldvec(&cam->dir), co = ldvec(&cam->orig);
-    f3 rdir = mk(cx.x * kcx + cy.x * kcy + cd.x, cx.y * kcx + cy.y * kcy + cd.y, cx.z * kcx + cy.z * kcy + cd.z);
-    ro = add3(smul(0.1f, rdir), co);
-    rd = norm3(rdir);
-}
-
-__device__ __forceinline__ int to_int(float x) {   // vec.h:60 with the OpenCL clamp builtin
-    return (int)(powf(fminf(fmaxf(x, 0.f), 1.f), 1.f / 2.2f) * 255.f + .5f);
-}
 
 struct RenderParams {
     DevAccel acc;
@@ -259,6 +37,7 @@ struct RenderParams {
     unsigned long long *trav_stats;
     unsigned *work_counter;
     int total;
+    int regroup;
 };
 
 // The path-tracing megakernel.  Lane state: one pixel (gid), its remaining samples, the RNG state
@@ -323,151 +102,35 @@ __global__ __launch_bounds__(256) void k_render(RenderParams P) {
                 ended = true;
             } else {
                 const f3 hp = add3(ro, smul(t, rd));
-                const Shape *sh = P.shapes + id;
-                const float *w = reinterpret_cast<const float *>(sh);
-                const int type = reinterpret_cast<const int *>(sh)[0];
-                f3 nrm;
-                if (type == SPHERE) {
-                    nrm = sub3(hp, mk(w[2], w[3], w[4]));
-                } else {
-                    const f3 p1 = mk(w[1], w[2], w[3]);
-                    nrm = cross3(sub3(mk(w[4], w[5], w[6]), p1), sub3(mk(w[7], w[8], w[9]), p1));
-                }
-                nrm = norm3(nrm);
-                const float dp = dot3(nrm, rd);
-                const float sgn = dp > 0.f ? 1.f : (dp < 0.f ? -1.f : (dp == dp ? dp : 0.f));  // OpenCL sign
-                const f3 nl = smul(-1.f * sgn, nrm);
-                const f3 e = mk(w[10], w[11], w[12]);
-                const f3 c = mk(w[13], w[14], w[15]);
-                const int refl = reinterpret_cast<const int *>(sh)[16];
-                if (!iszero3(e)) {
-                    if (spec) rad = add3(rad, mul3(thr, smul(fabsf(dp), e)));
+                const Surface S = surface_at(P.shapes + id, hp, rd);
+                if (!iszero3(S.e)) {
+                    if (spec) rad = add3(rad, mul3(thr, smul(fabsf(S.dp), S.e)));
                     ended = true;
-                } else if (refl == DIFF) {
+                } else if (S.refl == DIFF) {
                     spec = 0;
-                    thr = mul3(thr, c);
-                    // SampleLights
+                    thr = mul3(thr, S.c);
                     f3 Ld = mk(0.f, 0.f, 0.f);
                     for (int li = 0; li < P.nlights; li++) {
-                        const Shape *L = P.shapes + P.lights[li];
-                        const float *lw = reinterpret_cast<const float *>(L);
-                        const int ltype = reinterpret_cast<const int *>(L)[0];
-                        f3 lp, ln;
-                        if (ltype == TRIANGLE) {
-                            const float u1 = get_random(s0, s1);
-                            const float u2 = get_random(s0, s1);
-                            const float su1 = sqrtf(u1);
-                            const float bu = 1.0f - su1, bv = u2 * su1;
-                            const f3 p1 = mk(lw[1], lw[2], lw[3]);
-                            f3 e1 = sub3(mk(lw[4], lw[5], lw[6]), p1), e2 = sub3(mk(lw[7], lw[8], lw[9]), p1);
-                            e1 = smul(bu, e1);
-                            e2 = smul(bv, e2);
-                            lp = add3(add3(p1, e1), e2);
-                            ln = norm3(cross3(e1, e2));
-                        } else {
-                            const float u1 = get_random(s0, s1);
-                            const float u2 = get_random(s0, s1);
-                            const float zz = 1.f - 2.f * u1;
-                            const float q = 1.f - zz * zz;
-                            const float rr = sqrtf(0.f < q ? q : 0.f);
-                            const float phi = 2.f * kPi * u2;
-                            const f3 usp = mk(rr * cosf(phi), rr * sinf(phi), zz);
-                            lp = add3(smul(lw[1], usp), mk(lw[2], lw[3], lw[4]));
-                            ln = usp;
-                        }
-                        f3 sd = sub3(lp, hp);
-                        const float len = sqrtf(dot3(sd, sd));
-                        sd = smul(1.f / len, sd);
-                        float wo = dot3(sd, ln);
-                        if (wo > 0.f) continue;
-                        wo = -wo;
-                        wo = fminf(fmaxf(wo, 0.f), 1.f);
-                        const float wi = dot3(sd, nl);
-                        if (wi > 0.f) {
-                            n_any++;
-                            if (!anyhit<STACK>(A, hp, sd, len - kEpsilon, stk, pnv, ppt)) {
-                                const float s = lw[25] * wi * wo / (len * len);
-                                Ld = add3(Ld, smul(s, mk(lw[10], lw[11], lw[12])));
-                            }
-                        }
+                        f3 sd, contrib;
+                        float len;
+                        if (!sample_light(P.shapes + P.lights[li], s0, s1, hp, S.nl, &sd, &len, &contrib)) continue;
+                        n_any++;
+                        if (!anyhit<STACK>(A, hp, sd, len - kEpsilon, stk, pnv, ppt)) Ld = add3(Ld, contrib);
                     }
                     Ld = mul3(thr, Ld);
                     rad = add3(rad, Ld);
-                    // cosine-weighted bounce
-                    const float r1 = 2.f * kPi * get_random(s0, s1);
-                    const float r2 = get_random(s0, s1);
-                    const float r2s = sqrtf(r2);
-                    f3 ww = nl;
-                    const f3 aa = fabsf(ww.x) > .1f ? mk(0.f, 1.f, 0.f) : mk(1.f, 0.f, 0.f);
-                    f3 u = norm3(cross3(aa, ww));
-                    f3 v = cross3(ww, u);
-                    u = smul(cosf(r1) * r2s, u);
-                    v = smul(sinf(r1) * r2s, v);
-                    f3 nd = add3(u, v);
-                    ww = smul(sqrtf(1 - r2), ww);
-                    nd = add3(nd, ww);
+                    rd = diffuse_dir(S.nl, s0, s1);
                     ro = hp;
-                    rd = nd;
-                } else if (refl == SPEC) {
-                    spec = 1;
-                    f3 nd = smul(2.f * dot3(nrm, rd), nrm);
-                    nd = sub3(rd, nd);
-                    thr = mul3(thr, c);
-                    ro = hp;
-                    rd = nd;
                 } else {
                     spec = 1;
-                    f3 nd = smul(2.f * dot3(nrm, rd), nrm);
-                    nd = sub3(rd, nd);
-                    const bool into = dot3(nrm, nl) > 0;
-                    const float nc = 1.f, nt = 1.5f;
-                    const float nnt = into ? nc / nt : nt / nc;
-                    const float ddn = dot3(rd, nl);
-                    const float cos2t = 1.f - nnt * nnt * (1.f - ddn * ddn);
-                    if (cos2t < 0.f) {
-                        thr = mul3(thr, c);
-                        ro = hp;
-                        rd = nd;
-                    } else {
-                        const float kk = (float)(into ? 1 : -1) * (ddn * nnt + sqrtf(cos2t));
-                        const f3 nkk = smul(kk, nrm);
-                        f3 td = smul(nnt, rd);
-                        td = sub3(td, nkk);
-                        td = norm3(td);
-                        const float a = nt - nc, b = nt + nc;
-                        const float R0 = a * a / (b * b);
-                        const float cc = 1 - (into ? -ddn : dot3(td, nrm));
-                        const float Re = R0 + (1 - R0) * cc * cc * cc * cc * cc;
-                        const float Tr = 1.f - Re;
-                        const float Pp = .25f + .5f * Re;
-                        const float RP = Re / Pp;
-                        const float TP = Tr / (1.f - Pp);
-                        if (get_random(s0, s1) < Pp) {
-                            thr = smul(RP, thr);
-                            thr = mul3(thr, c);
-                            ro = hp;
-                            rd = nd;
-                        } else {
-                            thr = smul(TP, thr);
-                            thr = mul3(thr, c);
-                            ro = hp;
-                            rd = td;
-                        }
-                    }
+                    specular_or_refract(S, rd, thr, s0, s1);
+                    ro = hp;
                 }
                 depth++;
             }
         }
         if (ended) {
-            // RadianceGPU accumulation, rendering_kernel.cl:1127-1136
-            const int cs = P.currentSample + k;
-            if (cs == 0) {
-                col = rad;
-            } else {
-                const float k1 = (float)cs;
-                const float k2 = 1.f / ((float)cs + 1.f);
-                col = mk((col.x * k1 + rad.x) * k2, (col.y * k1 + rad.y) * k2, (col.z * k1 + rad.z) * k2);
-            }
+            col = running_mean(col, rad, P.currentSample + k);
             k++;
             if (k < P.spp) {
                 const int x = gid % P.width, y = gid / P.width;
@@ -478,8 +141,7 @@ __global__ __launch_bounds__(256) void k_render(RenderParams P) {
                 cp[0] = col.x; cp[1] = col.y; cp[2] = col.z;
                 if (P.pixels) {
                     const int x = gid % P.width, y = gid / P.width;
-                    P.pixels[y * P.width + x] = (unsigned)to_int(col.x) | ((unsigned)to_int(col.y) << 8) |
-                                                ((unsigned)to_int(col.z) << 16) | 0xff000000u;
+                    P.pixels[y * P.width + x] = pack_rgba(col);
                 }
                 P.seeds[2 * gid] = s0;
                 P.seeds[2 * gid + 1] = s1;
@@ -489,6 +151,311 @@ __global__ __launch_bounds__(256) void k_render(RenderParams P) {
     }
     if (P.ray_counter) {
         // wave reduction, one atomic per wave
+        unsigned long long a = n_closest, b = n_any;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, kWave);
+            b += __shfl_down(b, o, kWave);
+        }
+        if (__lane_id() == 0) {
+            atomicAdd(P.ray_counter, a);
+            atomicAdd(P.ray_counter + 1, b);
+        }
+    }
+    if (STATS && P.trav_stats) {
+        unsigned long long a = st_nodes, b = st_prims;
+        for (int o = 32; o > 0; o >>= 1) {
+            a += __shfl_down(a, o, kWave);
+            b += __shfl_down(b, o, kWave);
+        }
+        if (__lane_id() == 0) {
+            atomicAdd(P.trav_stats, a);
+            atomicAdd(P.trav_stats + 1, b);
+        }
+    }
+}
+
+// ---- megakernel with per-lane states ----
+//
+// The same paths as k_render, scheduled differently: every lane carries its own query (closest or
+// any hit) and the wave advances all traversing lanes by one node per loop iteration.  A lane whose
+// query ends does not hold the wave: it waits in a ready state until `regroup` lanes are ready (or
+// none is traversing), then the wave runs one shading pass for exactly those lanes, each of which
+// leaves it with its next query (or its pixel done).  The random numbers are drawn in the same
+// order as in k_render (shadow-ray outcomes never draw), so the bits are the same.  Needs a tree
+// whose root is an internal node (brute force and single-leaf trees run in k_render).
+enum : int {
+    kNeed = 0,    // wants a pixel
+    kTrav = 1,    // traversing
+    kCDone = 2,   // closest-hit query finished: tb / bid hold the result
+    kADone = 3,   // any-hit query finished: bid >= 0 means occluded
+    kLight = 4,   // next light of SampleLights (index li)
+    kSeg = 5,     // start of a path segment (depth checked here)
+    kEnded = 6,   // sample finished
+    kIdle = 7,    // no work left
+};
+
+template <int STACK, bool STATS, bool WIDE>
+__global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
+    __shared__ int s_stack[STACK * 256];
+    int *stk = s_stack + threadIdx.x;
+    const DevAccel &A = P.acc;
+    const unsigned long long lt = lanemask_lt();
+
+    int st = kNeed;
+    bool drained = false;
+    int gid = 0, k = 0, depth = 0, li = 0;
+    bool spec = true, qany = false;
+    unsigned s0 = 0, s1 = 0;
+    f3 col = mk(0.f, 0.f, 0.f), ro = col, rd = mk(0.f, 0.f, 1.f), rad = col, thr = mk(1.f, 1.f, 1.f);
+    f3 nl = col, Ld = col, contrib = col, inv = col;
+    float tb = kInf;
+    int bid = -1, node = 0, sp = 0;
+    unsigned nv = 0;
+    unsigned long long n_closest = 0, n_any = 0;
+    unsigned st_nodes = 0, st_prims = 0;
+
+    // the primitives of one leaf for the lane's current query; true when an any-hit query is decided
+    auto leaf = [&](int first, int cnt) -> bool {
+        if (first < 0 || cnt > A.nprims - first) return false;   // malformed caller tree
+        for (int q = 0; q < cnt; q++) {
+            int id;
+            const float t = prim_isect(A.prims, first + q, ro, rd, &id);
+            if (STATS) st_prims++;
+            if (qany) {
+                if (t != 0.f && t < tb && id != A.nshapes - 1) { bid = 1; return true; }
+            } else if (t != 0.f && (t < tb || (t == tb && id > bid))) {
+                tb = t;
+                bid = id;
+            }
+        }
+        return false;
+    };
+    // start the lane's query along (ro, rd); tb = maxt for any-hit
+    auto start = [&](bool any, float maxt) {
+        qany = any;
+        tb = any ? maxt : kInf;
+        bid = -1;
+        inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        node = A.root;
+        sp = 0;
+        nv = 0;
+        st = kTrav;
+    };
+
+    for (;;) {
+        // ---- traversal: one node of every traversing lane per iteration, until `regroup` lanes
+        // (or all that are left) have finished their queries ----
+        const unsigned long long live = __ballot(st != kIdle);
+        if (!live) break;
+        // a sparse wave (the end of the frame) shades sooner, so no lane's path waits long on others
+        const unsigned nlive = (unsigned)__popcll(live);
+        const unsigned quarter = (nlive + 3u) >> 2;
+        const unsigned regroup = (unsigned)P.regroup < quarter ? (unsigned)P.regroup : quarter;
+        for (;;) {
+            const unsigned long long trav = __ballot(st == kTrav);
+            if (!trav || (unsigned)__popcll(live & ~trav) >= regroup) break;
+            if (!WIDE && st == kTrav) {
+                bool done = false;
+                if ((unsigned)node >= (unsigned)A.nnodes || ++nv > kMaxVisits) {
+                    done = true;   // malformed caller tree
+                } else {
+                    const TNode *np = A.nodes + node;
+                    const float4 a = np->a, b = np->b, c = np->c;
+                    const int4 r = np->r;
+                    if (STATS) st_nodes += 2;
+                    float t0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, ro, inv, tb);
+                    float t1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, ro, inv, tb);
+                    if (r.x < 0 && t0 != __builtin_inff()) {
+                        done = leaf(~r.x, r.z);
+                        t0 = __builtin_inff();
+                    }
+                    if (!done && r.y < 0 && t1 != __builtin_inff()) {
+                        done = leaf(~r.y, r.w);
+                        t1 = __builtin_inff();
+                    }
+                    if (!done) {
+                        const bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
+                        if (h0 && h1) {
+                            const bool first0 = qany || t0 <= t1;
+                            if (sp < STACK) stk[(sp++) * 256] = first0 ? r.y : r.x;
+                            node = first0 ? r.x : r.y;
+                        } else if (h0) {
+                            node = r.x;
+                        } else if (h1) {
+                            node = r.y;
+                        } else if (sp == 0) {
+                            done = true;
+                        } else {
+                            node = stk[(--sp) * 256];
+                        }
+                    }
+                }
+                if (done) st = qany ? kADone : kCDone;
+            }
+            if (WIDE && st == kTrav) {
+                bool done = false;
+                if ((unsigned)node >= (unsigned)A.nnodes || ++nv > kMaxVisits) {
+                    done = true;   // malformed caller tree
+                } else {
+                    const TNode4 *np = A.nodes4 + node;
+                    const float4 lx = np->lox, ly = np->loy, lz = np->loz;
+                    const float4 hx = np->hix, hy = np->hiy, hz = np->hiz;
+                    const int4 rf = np->ref, cn = np->cnt;
+                    constexpr float inf = __builtin_inff();
+                    // branch-free: unused slots hold a box beyond every query's clip distance
+                    float t0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ro, inv, tb);
+                    float t1 = slab(lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, ro, inv, tb);
+                    float t2 = slab(lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, ro, inv, tb);
+                    float t3 = slab(lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, ro, inv, tb);
+                    if (STATS) st_nodes += (rf.x != kEmptyRef) + (rf.y != kEmptyRef) + (rf.z != kEmptyRef) +
+                                           (rf.w != kEmptyRef);
+                    // leaf children that are hit: their primitives now, one leaf per loop turn
+                    unsigned lm = (rf.x < 0 && t0 != inf ? 1u : 0u) | (rf.y < 0 && t1 != inf ? 2u : 0u) |
+                                  (rf.z < 0 && t2 != inf ? 4u : 0u) | (rf.w < 0 && t3 != inf ? 8u : 0u);
+                    if (lm & 1u) t0 = inf;
+                    if (lm & 2u) t1 = inf;
+                    if (lm & 4u) t2 = inf;
+                    if (lm & 8u) t3 = inf;
+                    while (lm && !done) {
+                        const unsigned k = (unsigned)__builtin_ctz(lm);
+                        lm &= lm - 1u;
+                        const int lr = k == 0 ? rf.x : k == 1 ? rf.y : k == 2 ? rf.z : rf.w;
+                        const int lc = k == 0 ? cn.x : k == 1 ? cn.y : k == 2 ? cn.z : cn.w;
+                        done = leaf(~lr, lc);
+                    }
+                    if (!done) {
+                        // inner children: nearest first (order only changes the visit count)
+                        int r0 = rf.x, r1 = rf.y, r2 = rf.z, r3 = rf.w;
+                        auto cswap = [](float &ta, int &ra, float &tb_, int &rb) {
+                            const bool sw = tb_ < ta;
+                            const float tt = sw ? tb_ : ta;
+                            tb_ = sw ? ta : tb_;
+                            ta = tt;
+                            const int rr = sw ? rb : ra;
+                            rb = sw ? ra : rb;
+                            ra = rr;
+                        };
+                        cswap(t0, r0, t1, r1);
+                        cswap(t2, r2, t3, r3);
+                        cswap(t0, r0, t2, r2);
+                        cswap(t1, r1, t3, r3);
+                        cswap(t1, r1, t2, r2);
+                        if (t3 != inf && sp < STACK) stk[(sp++) * 256] = r3;
+                        if (t2 != inf && sp < STACK) stk[(sp++) * 256] = r2;
+                        if (t1 != inf && sp < STACK) stk[(sp++) * 256] = r1;
+                        if (t0 != inf) node = r0;
+                        else if (sp == 0) done = true;
+                        else node = stk[(--sp) * 256];
+                    }
+                }
+                if (done) st = qany ? kADone : kCDone;
+            }
+        }
+        // ---- shading pass: pixels for lanes that need one (one atomic per wave) ----
+        const unsigned long long need = __ballot(st == kNeed);
+        if (need) {
+            const unsigned cnt = (unsigned)__popcll(need);
+            const int leader = __ffsll((long long)need) - 1;
+            unsigned base = 0;
+            if ((int)__lane_id() == leader) base = atomicAdd(P.work_counter, cnt);
+            base = __shfl(base, leader, kWave);
+            if (base + cnt >= (unsigned)P.total) drained = true;
+            if (st == kNeed) {
+                const unsigned my = base + (unsigned)__popcll(need & lt);
+                if (my < (unsigned)P.total) {
+                    gid = (int)my;
+                    const int x = gid % P.width, y = gid / P.width;
+                    s0 = P.seeds[2 * gid];
+                    s1 = P.seeds[2 * gid + 1];
+                    k = 0;
+                    if (P.currentSample > 0) col = ldvec(P.colors + (P.height - y - 1) * P.width + x);
+                    camera_ray(P.camera, s0, s1, P.width, P.height, x, y, ro, rd);
+                    rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
+                    st = kSeg;
+                } else {
+                    st = kIdle;
+                }
+            }
+        }
+        // ---- per-lane state machine until the lane holds a query or its pixel is done ----
+        while (st >= kCDone && st <= kEnded) {
+            if (st == kCDone) {
+                if (bid < 0) {
+                    st = kEnded;
+                } else {
+                    const f3 hp = add3(ro, smul(tb, rd));
+                    const Surface S = surface_at(P.shapes + bid, hp, rd);
+                    if (!iszero3(S.e)) {
+                        if (spec) rad = add3(rad, mul3(thr, smul(fabsf(S.dp), S.e)));
+                        st = kEnded;
+                    } else if (S.refl == DIFF) {
+                        spec = false;
+                        thr = mul3(thr, S.c);
+                        Ld = mk(0.f, 0.f, 0.f);
+                        nl = S.nl;
+                        ro = hp;
+                        li = 0;
+                        st = kLight;
+                    } else {
+                        spec = true;
+                        specular_or_refract(S, rd, thr, s0, s1);
+                        ro = hp;
+                        depth++;
+                        st = kSeg;
+                    }
+                }
+            }
+            if (st == kADone) {
+                if (bid < 0) Ld = add3(Ld, contrib);
+                li++;
+                st = kLight;
+            }
+            if (st == kLight) {
+                // lights whose sample needs no shadow ray only draw their random numbers
+                float len = 0.f;
+                bool shoot = false;
+                for (; li < P.nlights; li++) {
+                    if (sample_light(P.shapes + P.lights[li], s0, s1, ro, nl, &rd, &len, &contrib)) {
+                        shoot = true;
+                        break;
+                    }
+                }
+                if (shoot) {
+                    n_any++;
+                    start(true, len - kEpsilon);
+                    continue;
+                }
+                Ld = mul3(thr, Ld);
+                rad = add3(rad, Ld);
+                rd = diffuse_dir(nl, s0, s1);
+                depth++;
+                st = kSeg;
+            }
+            if (st == kSeg && depth > P.max_depth) st = kEnded;
+            if (st == kEnded) {
+                col = running_mean(col, rad, P.currentSample + k);
+                k++;
+                const int x = gid % P.width, y = gid / P.width;
+                if (k < P.spp) {
+                    camera_ray(P.camera, s0, s1, P.width, P.height, x, y, ro, rd);
+                    rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
+                    st = kSeg;
+                } else {
+                    float *cp = reinterpret_cast<float *>(P.colors + (P.height - y - 1) * P.width + x);
+                    cp[0] = col.x; cp[1] = col.y; cp[2] = col.z;
+                    if (P.pixels) P.pixels[y * P.width + x] = pack_rgba(col);
+                    P.seeds[2 * gid] = s0;
+                    P.seeds[2 * gid + 1] = s1;
+                    st = drained ? kIdle : kNeed;
+                }
+            }
+            if (st == kSeg) {
+                n_closest++;
+                start(false, 0.f);
+            }
+        }
+    }
+    if (P.ray_counter) {
         unsigned long long a = n_closest, b = n_any;
         for (int o = 32; o > 0; o >>= 1) {
             a += __shfl_down(a, o, kWave);
@@ -538,26 +505,26 @@ __global__ __launch_bounds__(256) void k_trace(DevAccel A, const Ray *rays, cons
 }
 
 // ---- scene preparation kernels ----
-__device__ __forceinline__ Prim prim_of(const Shape *s, int id) {
+__device__ __forceinline__ Prim prim_of(const Shape *s, int id, int flags) {
     const float *w = reinterpret_cast<const float *>(s);
-    const int type = reinterpret_cast<const int *>(s)[0];
+    const int type = reinterpret_cast<const int *>(s)[0] | flags;
     Prim p;
-    if (type == SPHERE) {
+    if ((type & 0xff) == SPHERE) {
         p.p0 = make_float4(w[2], w[3], w[4], __int_as_float(id));
         p.p1 = make_float4(w[1], 0.f, 0.f, __int_as_float(type));
-        p.p2 = make_float4(0.f, 0.f, 0.f, 0.f);
+        p.p2 = make_float4(0.f, 0.f, 0.f, __int_as_float(type));
     } else {
         const float x1 = w[1], y1 = w[2], z1 = w[3];
         p.p0 = make_float4(x1, y1, z1, __int_as_float(id));
         p.p1 = make_float4(w[4] - x1, w[5] - y1, w[6] - z1, __int_as_float(type));
-        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, 0.f);
+        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, __int_as_float(type));
     }
     return p;
 }
 
 __global__ void k_index_prims(const Shape *shapes, int n, Prim *prims) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) prims[i] = prim_of(shapes + i, i);
+    if (i < n) prims[i] = prim_of(shapes + i, i, kLastInLeaf);
 }
 
 __device__ __forceinline__ bool emits(const Shape *s) {
@@ -616,11 +583,54 @@ __global__ __launch_bounds__(256) void k_light_write(const Shape *shapes, int n,
     }
 }
 
+// TNode -> TNode4: the four grandchildren of binary node i (a leaf child is kept as it is).  Refs
+// keep the binary numbering, so every internal node has a four-wide twin and a walk from the root
+// that always takes two binary levels per step stays inside the array.
+__global__ __launch_bounds__(256) void k_collapse4(const TNode *__restrict__ bn, int nnodes, TNode4 *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    float lo[3][4], hi[3][4];
+    int ref[4], cnt[4], k = 0;
+    auto put = [&](float x0, float y0, float z0, float x1, float y1, float z1, int r, int c) {
+        lo[0][k] = x0; lo[1][k] = y0; lo[2][k] = z0;
+        hi[0][k] = x1; hi[1][k] = y1; hi[2][k] = z1;
+        ref[k] = r; cnt[k] = c;
+        k++;
+    };
+    auto put_child = [&](const TNode &t, int side) {
+        if (side == 0) put(t.a.x, t.a.y, t.a.z, t.a.w, t.b.x, t.b.y, t.r.x, t.r.z);
+        else put(t.b.z, t.b.w, t.c.x, t.c.y, t.c.z, t.c.w, t.r.y, t.r.w);
+    };
+    const TNode t = bn[i];
+    for (int side = 0; side < 2; side++) {
+        const int r = side == 0 ? t.r.x : t.r.y;
+        if (r < 0 || r >= nnodes) {
+            put_child(t, side);   // leaf (or a ref the traversal rejects)
+        } else {
+            const TNode g = bn[r];
+            put_child(g, 0);
+            put_child(g, 1);
+        }
+    }
+    // an unused slot is a point at 1e30: every query clips at t <= 1e20, so its slab test misses
+    for (; k < 4;) put(kFarBox, kFarBox, kFarBox, kFarBox, kFarBox, kFarBox, kEmptyRef, 0);
+    TNode4 o;
+    o.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    o.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    o.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    o.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    o.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    o.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    o.ref = make_int4(ref[0], ref[1], ref[2], ref[3]);
+    o.cnt = make_int4(cnt[0], cnt[1], cnt[2], cnt[3]);
+    out[i] = o;
+}
+
 // reference BVH layout -> TNode/Prim (child is a leaf iff nLeft == min / nRight == max)
 __global__ void k_convert_bvh(const Shape *shapes, int n, const BVHNodeGPU *btn, const BVHNodeGPU *btl,
                               TNode *nodes, Prim *prims) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) prims[i] = prim_of(shapes + btl[i].nShape, btl[i].nShape);
+    if (i < n) prims[i] = prim_of(shapes + btl[i].nShape, btl[i].nShape, kLastInLeaf);   // one per leaf
     if (i < n - 1) {
         const int nl = btn[i].nLeft, nr = btn[i].nRight;
         const bool lleaf = nl == btn[i].min, rleaf = nr == btn[i].max;
@@ -639,7 +649,7 @@ __global__ void k_convert_bvh(const Shape *shapes, int n, const BVHNodeGPU *btn,
 __global__ void k_convert_kd(const Shape *shapes, const KDNodeGPU *kng, int kngCnt, const int *kn, int knCnt,
                              TNode *nodes, Prim *prims) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < knCnt) prims[h] = prim_of(shapes + kn[h], kn[h]);
+    if (h < knCnt) prims[h] = prim_of(shapes + kn[h], kn[h], 0);   // caller ranges may overlap: no flags
     if (h >= 1 && h < kngCnt && kng[h].leaf == 0 && 2 * h + 1 < kngCnt) {
         const int cl = 2 * h, cr = 2 * h + 1;
         const bool lleaf = kng[cl].leaf == 1, rleaf = kng[cr].leaf == 1;
@@ -663,6 +673,17 @@ int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_
     hipLaunchKernelGGL(k_index_prims, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_shapes, n,
                        out_prims.as<Prim>());
     PT_HIP(hipGetLastError());
+    return PTGPU_OK;
+}
+
+int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4) {
+    acc->nodes4 = nullptr;
+    if (acc->mode == PTGPU_ACCEL_NONE || acc->root < 0 || acc->nnodes <= 0 || !acc->nodes) return PTGPU_OK;
+    PT_TRY(out_nodes4.ensure((size_t)acc->nnodes * sizeof(TNode4)));
+    hipLaunchKernelGGL(k_collapse4, dim3((acc->nnodes + 255) / 256), dim3(256), 0, ctx->stream, acc->nodes,
+                       acc->nnodes, out_nodes4.as<TNode4>());
+    PT_HIP(hipGetLastError());
+    acc->nodes4 = out_nodes4.as<TNode4>();
     return PTGPU_OK;
 }
 
@@ -705,6 +726,7 @@ int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeG
     acc->nnodes = n - 1;
     acc->nshapes = n;
     acc->mode = PTGPU_ACCEL_BVH;
+    acc->leaf_runs = 1;
     if (n >= 2) {
         acc->root = 0;
         acc->root_cnt = 0;
@@ -750,6 +772,14 @@ int convert_kd_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU
     return PTGPU_OK;
 }
 
+// four-wide walk: at most three pushes per visit and one visit per two binary levels
+static int stack_for_wide(const DevAccel &acc) {
+    const int need = 3 * ((acc.depth + 2) / 2) + 1;
+    if (need <= 32) return 32;
+    if (need <= 64) return 64;
+    return -1;
+}
+
 static int stack_for(const DevAccel &acc) {
     if (acc.mode == PTGPU_ACCEL_NONE) return 24;
     if (acc.depth + 2 <= 24) return 24;
@@ -760,6 +790,11 @@ static int stack_for(const DevAccel &acc) {
 int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     const int total = L.width * L.height;
     if (total <= 0) return PTGPU_OK;
+    const bool wf_ok = acc.mode != PTGPU_ACCEL_NONE && wf_supported(L.nlights) && !L.trav_stats;
+    if (ctx->engine == 2) PT_CHECK(wf_ok, PTGPU_ERR_UNSUPPORTED, "wavefront engine: needs a tree, <= 8 lights, no stats");
+    // the wavefront schedule only on request: one frame keeps at most W*H paths in flight, too few
+    // to amortise its per-bounce barriers (measured 2x slower than the megakernels at 512x512)
+    if (wf_ok && ctx->engine == 2) return launch_render_wavefront(ctx, acc, L);
     const int st = stack_for(acc);
     PT_CHECK(st > 0, PTGPU_ERR_UNSUPPORTED, "tree depth %d exceeds the 62-level traversal stack", acc.depth);
     PT_TRY(ctx->tmp_counter.ensure(64));
@@ -782,12 +817,35 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.trav_stats = L.trav_stats;
     P.work_counter = ctx->tmp_counter.as<unsigned>();
     P.total = total;
+    P.regroup = ctx->regroup;
     // persistent waves: enough workgroups to fill every CU several times, never more than needed
-    int blocks = ctx->cu_count * 8;
+    int blocks = ctx->cu_count * ctx->blocks_per_cu;
     const int need = (total + 255) / 256;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    if (L.trav_stats) {
+    // auto and engine 3: per-lane states whenever there is a tree to walk (measured fastest at every
+    // size tried); brute force and single-leaf scenes keep the lockstep megakernel
+    if ((ctx->engine == 0 || ctx->engine == 3) && acc.mode != PTGPU_ACCEL_NONE && acc.root >= 0) {
+        const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc) : -1;
+#define PT_LANES(ST, STATS_, WIDE_) hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_>), dim3(blocks), dim3(256), 0, \
+                                                       ctx->stream, P)
+        if (sw > 0) {
+            if (L.trav_stats) {
+                if (sw == 32) PT_LANES(32, true, true);
+                else PT_LANES(64, true, true);
+            } else {
+                if (sw == 32) PT_LANES(32, false, true);
+                else PT_LANES(64, false, true);
+            }
+        } else if (L.trav_stats) {
+            if (st == 24) PT_LANES(24, true, false);
+            else PT_LANES(64, true, false);
+        } else {
+            if (st == 24) PT_LANES(24, false, false);
+            else PT_LANES(64, false, false);
+        }
+#undef PT_LANES
+    } else if (L.trav_stats) {
         if (st == 24) hipLaunchKernelGGL((k_render<24, true>), dim3(blocks), dim3(256), 0, ctx->stream, P);
         else hipLaunchKernelGGL((k_render<64, true>), dim3(blocks), dim3(256), 0, ctx->stream, P);
     } else {
@@ -868,10 +926,12 @@ int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) 
     if (accel == PTGPU_ACCEL_BVH) {
         PT_CHECK(a->btn && a->btl, PTGPU_ERR_INVALID, "BVH mode needs btn and btl");
         PT_TRY(convert_bvh_ref(ctx, a->shapes, n, a->btn, a->btl, ctx->tmp_nodes, ctx->tmp_prims, &acc));
+        PT_TRY(build_wide(ctx, &acc, ctx->tmp_nodes4));
     } else if (accel == PTGPU_ACCEL_KD) {
         PT_CHECK(a->kng && a->kn, PTGPU_ERR_INVALID, "KD mode needs kng and kn");
         PT_TRY(convert_kd_ref(ctx, a->shapes, n, a->kng, a->kngCnt, a->kn, a->knCnt, ctx->tmp_nodes, ctx->tmp_prims,
                               &acc));
+        PT_TRY(build_wide(ctx, &acc, ctx->tmp_nodes4));
     } else {
         PT_CHECK(accel == PTGPU_ACCEL_NONE, PTGPU_ERR_INVALID, "unknown accel mode %d", accel);
         PT_TRY(resolve_accel(ctx, nullptr, nullptr, a->shapes, a->shapeCnt, &acc));

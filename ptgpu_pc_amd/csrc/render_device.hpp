@@ -1,0 +1,378 @@
+// render_device.hpp -- device math shared by the path-tracing engines (megakernel and wavefront).
+//
+// Every function restates a piece of rendering_kernel.cl in FP32 with the expression order of the
+// reference (the library is built with -ffp-contract=off and IEEE division/sqrt), so both engines
+// produce the same bits for the same path.
+#pragma once
+
+#include "device_util.hpp"
+
+namespace ptgpu {
+
+constexpr float kSlack = 1.0f + 1.0e-5f;   // relative slack on the slab test's exit distance
+constexpr float kInf = 1e20f;              // the reference's initial t (rendering_kernel.cl:228)
+constexpr unsigned kMaxVisits = 1u << 24;  // node visits per query; only a cyclic tree reaches it
+
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mk(float x, float y, float z) { f3 v; v.x = x; v.y = y; v.z = z; return v; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 smul(float k, f3 b) { return mk(k * b.x, k * b.y, k * b.z); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ f3 norm3(f3 v) { float l = 1.f / sqrtf(dot3(v, v)); return smul(l, v); }
+__device__ __forceinline__ bool iszero3(f3 v) { return v.x == 0.f && v.y == 0.f && v.z == 0.f; }
+__device__ __forceinline__ f3 ldvec(const Vec *p) {
+    const float *f = reinterpret_cast<const float *>(p);
+    return mk(f[0], f[1], f[2]);
+}
+
+// ---- GetRandom, rendering_kernel.cl:4-17 ----
+__device__ __forceinline__ float get_random(unsigned &s0, unsigned &s1) {
+    s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
+    s1 = 18000u * (s1 & 65535u) + (s1 >> 16);
+    unsigned ires = (s0 << 16) + s1;
+    float f = __uint_as_float((ires & 0x007fffffu) | 0x40000000u);
+    return (f - 2.f) / 2.f;
+}
+
+// ---- primitive tests (rendering_kernel.cl:26-125) on the Prim layout ----
+__device__ __forceinline__ float tri_isect(const float4 &p0, const float4 &p1, const float4 &p2, f3 o, f3 d) {
+    const f3 v0 = mk(p0.x, p0.y, p0.z), e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
+    const f3 pvec = cross3(d, e2);
+    const float det = dot3(e1, pvec);
+    if (det == 0.f) return 0.f;
+    const float inv = 1.0f / det;
+    const f3 tvec = sub3(o, v0);
+    const float u = dot3(tvec, pvec) * inv;
+    if (u < 0.0f || u > 1.0f) return 0.f;
+    const f3 qvec = cross3(tvec, e1);
+    const float v = dot3(d, qvec) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 0.f;
+    const float t = dot3(e2, qvec) * inv;
+    return t > kEpsilon ? t : 0.f;
+}
+__device__ __forceinline__ float sph_isect(const float4 &p0, float rad, f3 o, f3 d) {
+    const f3 op = sub3(mk(p0.x, p0.y, p0.z), o);
+    const float b = dot3(op, d);
+    float det = b * b - dot3(op, op) + rad * rad;
+    if (det < 0.f) return 0.f;
+    det = sqrtf(det);
+    float t = b - det;
+    if (t > kEpsilon) return t;
+    t = b + det;
+    return t > kEpsilon ? t : 0.f;
+}
+__device__ __forceinline__ float prim_isect(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id) {
+    const float4 p0 = prims[k].p0, p1 = prims[k].p1, p2 = prims[k].p2;
+    *id = __float_as_int(p0.w);
+    if ((__float_as_int(p2.w) & 0xff) == SPHERE) return sph_isect(p0, p1.x, o, d);
+    return tri_isect(p0, p1, p2, o, d);
+}
+
+// slab test of one box; returns entry distance (>= 0) or +inf on a miss
+__device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
+                                      float tlimit) {
+    const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+    const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+    const float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlimit));
+    return tn <= tf * kSlack ? tn : __builtin_inff();
+}
+
+// Closest hit over a prepared structure.  Returns the hit shape index or -1; t in *tout.
+template <int STACK>
+__device__ int trace_closest(const DevAccel &A, f3 o, f3 d, float *tout, int *stk, unsigned *nodes_visited,
+                             unsigned *prims_tested) {
+    float tb = kInf;
+    int bid = -1;
+    const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    auto test_leaf = [&](int first, int cnt) {
+        if (first < 0 || cnt > A.nprims - first) return;   // malformed caller tree
+        for (int q = 0; q < cnt; q++) {
+            int id;
+            const float t = prim_isect(A.prims, first + q, o, d, &id);
+            if (prims_tested) (*prims_tested)++;
+            if (t != 0.f && (t < tb || (t == tb && id > bid))) { tb = t; bid = id; }
+        }
+    };
+    if (A.root < 0) {
+        test_leaf(~A.root, A.root_cnt);
+        *tout = tb;
+        return bid;
+    }
+    int node = A.root, sp = 0;
+    for (unsigned guard = 0; guard < kMaxVisits; guard++) {
+        if ((unsigned)node >= (unsigned)A.nnodes) break;   // malformed caller tree
+        const TNode *np = A.nodes + node;
+        const float4 a = np->a, b = np->b, c = np->c;
+        const int4 r = np->r;
+        if (nodes_visited) (*nodes_visited) += 2;   // box tests
+        float t0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tb);
+        float t1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tb);
+        if (r.x < 0 && t0 != __builtin_inff()) { test_leaf(~r.x, r.z); t0 = __builtin_inff(); }
+        if (r.y < 0 && t1 != __builtin_inff()) { test_leaf(~r.y, r.w); t1 = __builtin_inff(); }
+        const bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
+        if (h0 && h1) {
+            const bool first0 = t0 <= t1;
+            const int nearc = first0 ? r.x : r.y, farc = first0 ? r.y : r.x;
+            if (sp < STACK) stk[(sp++) * 256] = farc;
+            node = nearc;
+        } else if (h0) {
+            node = r.x;
+        } else if (h1) {
+            node = r.y;
+        } else {
+            if (sp == 0) break;
+            node = stk[(--sp) * 256];
+        }
+    }
+    *tout = tb;
+    return bid;
+}
+
+// Any hit with 0 != t < maxt over shapes other than `skip_id` (IntersectP semantics).
+template <int STACK>
+__device__ bool trace_any(const DevAccel &A, f3 o, f3 d, float maxt, int skip_id, int *stk,
+                          unsigned *nodes_visited, unsigned *prims_tested) {
+    const f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    auto test_leaf = [&](int first, int cnt) -> bool {
+        if (first < 0 || cnt > A.nprims - first) return false;   // malformed caller tree
+        for (int q = 0; q < cnt; q++) {
+            int id;
+            const float t = prim_isect(A.prims, first + q, o, d, &id);
+            if (prims_tested) (*prims_tested)++;
+            if (t != 0.f && t < maxt && id != skip_id) return true;
+        }
+        return false;
+    };
+    if (A.root < 0) return test_leaf(~A.root, A.root_cnt);
+    int node = A.root, sp = 0;
+    for (unsigned guard = 0; guard < kMaxVisits; guard++) {
+        if ((unsigned)node >= (unsigned)A.nnodes) break;   // malformed caller tree
+        const TNode *np = A.nodes + node;
+        const float4 a = np->a, b = np->b, c = np->c;
+        const int4 r = np->r;
+        if (nodes_visited) (*nodes_visited) += 2;   // box tests
+        float t0 = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, maxt);
+        float t1 = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, maxt);
+        if (r.x < 0 && t0 != __builtin_inff()) { if (test_leaf(~r.x, r.z)) return true; t0 = __builtin_inff(); }
+        if (r.y < 0 && t1 != __builtin_inff()) { if (test_leaf(~r.y, r.w)) return true; t1 = __builtin_inff(); }
+        const bool h0 = t0 != __builtin_inff(), h1 = t1 != __builtin_inff();
+        if (h0 && h1) {
+            if (sp < STACK) stk[(sp++) * 256] = r.y;
+            node = r.x;
+        } else if (h0) {
+            node = r.x;
+        } else if (h1) {
+            node = r.y;
+        } else {
+            if (sp == 0) break;
+            node = stk[(--sp) * 256];
+        }
+    }
+    return false;
+}
+
+// Brute-force mode (ACCELSTR 0): prims in shape-index order.
+__device__ int brute_closest(const Prim *prims, int n, f3 o, f3 d, float *tout, unsigned *prims_tested) {
+    float tb = kInf;
+    int bid = -1;
+    for (int i = n - 1; i >= 0; i--) {
+        int id;
+        const float t = prim_isect(prims, i, o, d, &id);
+        if (t != 0.f && t < tb) { tb = t; bid = id; }
+    }
+    if (prims_tested) *prims_tested += (unsigned)n;
+    *tout = tb;
+    return bid;
+}
+__device__ bool brute_any(const Prim *prims, int n, f3 o, f3 d, float maxt, unsigned *prims_tested) {
+    for (int i = n - 2; i >= 0; i--) {
+        int id;
+        if (prims_tested) (*prims_tested)++;
+        const float t = prim_isect(prims, i, o, d, &id);
+        if (t != 0.f && t < maxt) return true;
+    }
+    return false;
+}
+
+template <int STACK>
+__device__ __forceinline__ int closest(const DevAccel &A, f3 o, f3 d, float *t, int *stk, unsigned *nv, unsigned *pt) {
+    if (A.mode == PTGPU_ACCEL_NONE) return brute_closest(A.prims, A.nprims, o, d, t, pt);
+    return trace_closest<STACK>(A, o, d, t, stk, nv, pt);
+}
+template <int STACK>
+__device__ __forceinline__ bool anyhit(const DevAccel &A, f3 o, f3 d, float maxt, int *stk, unsigned *nv, unsigned *pt) {
+    if (A.mode == PTGPU_ACCEL_NONE) return brute_any(A.prims, A.nprims, o, d, maxt, pt);
+    return trace_any<STACK>(A, o, d, maxt, A.nshapes - 1, stk, nv, pt);
+}
+
+// ---- camera ray, rendering_kernel.cl:1042-1070 ----
+__device__ __forceinline__ void camera_ray(const Camera *cam, unsigned &s0, unsigned &s1, int width, int height,
+                                           int x, int y, f3 &ro, f3 &rd) {
+    const float invW = 1.f / (float)width, invH = 1.f / (float)height;
+    const float r1 = get_random(s0, s1) - .5f;
+    const float r2 = get_random(s0, s1) - .5f;
+    const float kcx = ((float)x + r1) * invW - .5f;
+    const float kcy = ((float)y + r2) * invH - .5f;
+    const f3 cx = ldvec(&cam->x), cy = ldvec(&cam->y), cd = ldvec(&cam->dir), co = ldvec(&cam->orig);
+    f3 rdir = mk(cx.x * kcx + cy.x * kcy + cd.x, cx.y * kcx + cy.y * kcy + cd.y, cx.z * kcx + cy.z * kcy + cd.z);
+    ro = add3(smul(0.1f, rdir), co);
+    rd = norm3(rdir);
+}
+
+__device__ __forceinline__ int to_int(float x) {   // vec.h:60 with the OpenCL clamp builtin
+    return (int)(powf(fminf(fmaxf(x, 0.f), 1.f), 1.f / 2.2f) * 255.f + .5f);
+}
+
+
+// ---- surface at a hit (rendering_kernel.cl:550-598) ----
+struct Surface {
+    f3 nrm, nl, e, c;
+    float dp;
+    int refl;
+};
+__device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
+    const float *w = reinterpret_cast<const float *>(sh);
+    const int type = reinterpret_cast<const int *>(sh)[0];
+    Surface S;
+    f3 nrm;
+    if (type == SPHERE) {
+        nrm = sub3(hp, mk(w[2], w[3], w[4]));
+    } else {
+        const f3 p1 = mk(w[1], w[2], w[3]);
+        nrm = cross3(sub3(mk(w[4], w[5], w[6]), p1), sub3(mk(w[7], w[8], w[9]), p1));
+    }
+    nrm = norm3(nrm);
+    S.nrm = nrm;
+    S.dp = dot3(nrm, rd);
+    const float sgn = S.dp > 0.f ? 1.f : (S.dp < 0.f ? -1.f : (S.dp == S.dp ? S.dp : 0.f));  // OpenCL sign
+    S.nl = smul(-1.f * sgn, nrm);
+    S.e = mk(w[10], w[11], w[12]);
+    S.c = mk(w[13], w[14], w[15]);
+    S.refl = reinterpret_cast<const int *>(sh)[16];
+    return S;
+}
+
+// ---- one light of SampleLights (rendering_kernel.cl:421-490): draws two randoms (u1 first),
+// returns true when a shadow ray must decide the contribution s * e; sd/len describe that ray.
+__device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsigned &s1, f3 hp, f3 nl, f3 *sd_out,
+                                             float *len_out, f3 *contrib) {
+    const float *lw = reinterpret_cast<const float *>(L);
+    const int ltype = reinterpret_cast<const int *>(L)[0];
+    f3 lp, ln;
+    if (ltype == TRIANGLE) {
+        const float u1 = get_random(s0, s1);
+        const float u2 = get_random(s0, s1);
+        const float su1 = sqrtf(u1);
+        const float bu = 1.0f - su1, bv = u2 * su1;
+        const f3 p1 = mk(lw[1], lw[2], lw[3]);
+        f3 e1 = sub3(mk(lw[4], lw[5], lw[6]), p1), e2 = sub3(mk(lw[7], lw[8], lw[9]), p1);
+        e1 = smul(bu, e1);
+        e2 = smul(bv, e2);
+        lp = add3(add3(p1, e1), e2);
+        ln = norm3(cross3(e1, e2));
+    } else {
+        const float u1 = get_random(s0, s1);
+        const float u2 = get_random(s0, s1);
+        const float zz = 1.f - 2.f * u1;
+        const float q = 1.f - zz * zz;
+        const float rr = sqrtf(0.f < q ? q : 0.f);
+        const float phi = 2.f * kPi * u2;
+        const f3 usp = mk(rr * cosf(phi), rr * sinf(phi), zz);
+        lp = add3(smul(lw[1], usp), mk(lw[2], lw[3], lw[4]));
+        ln = usp;
+    }
+    f3 sd = sub3(lp, hp);
+    const float len = sqrtf(dot3(sd, sd));
+    sd = smul(1.f / len, sd);
+    float wo = dot3(sd, ln);
+    if (wo > 0.f) return false;
+    wo = -wo;
+    wo = fminf(fmaxf(wo, 0.f), 1.f);
+    const float wi = dot3(sd, nl);
+    if (!(wi > 0.f)) return false;
+    const float s = lw[25] * wi * wo / (len * len);
+    *sd_out = sd;
+    *len_out = len;
+    *contrib = smul(s, mk(lw[10], lw[11], lw[12]));
+    return true;
+}
+
+// ---- cosine-weighted diffuse bounce (rendering_kernel.cl:629-661) ----
+__device__ __forceinline__ f3 diffuse_dir(f3 nl, unsigned &s0, unsigned &s1) {
+    const float r1 = 2.f * kPi * get_random(s0, s1);
+    const float r2 = get_random(s0, s1);
+    const float r2s = sqrtf(r2);
+    f3 ww = nl;
+    const f3 aa = fabsf(ww.x) > .1f ? mk(0.f, 1.f, 0.f) : mk(1.f, 0.f, 0.f);
+    f3 u = norm3(cross3(aa, ww));
+    f3 v = cross3(ww, u);
+    u = smul(cosf(r1) * r2s, u);
+    v = smul(sinf(r1) * r2s, v);
+    f3 nd = add3(u, v);
+    ww = smul(sqrtf(1 - r2), ww);
+    return add3(nd, ww);
+}
+
+// ---- specular and refractive continuation (rendering_kernel.cl:666-752); updates thr/rd ----
+__device__ __forceinline__ void specular_or_refract(const Surface &S, f3 &rd, f3 &thr, unsigned &s0, unsigned &s1) {
+    f3 nd = smul(2.f * dot3(S.nrm, rd), S.nrm);
+    nd = sub3(rd, nd);
+    if (S.refl == SPEC) {
+        thr = mul3(thr, S.c);
+        rd = nd;
+        return;
+    }
+    const bool into = dot3(S.nrm, S.nl) > 0;
+    const float nc = 1.f, nt = 1.5f;
+    const float nnt = into ? nc / nt : nt / nc;
+    const float ddn = dot3(rd, S.nl);
+    const float cos2t = 1.f - nnt * nnt * (1.f - ddn * ddn);
+    if (cos2t < 0.f) {
+        thr = mul3(thr, S.c);
+        rd = nd;
+        return;
+    }
+    const float kk = (float)(into ? 1 : -1) * (ddn * nnt + sqrtf(cos2t));
+    const f3 nkk = smul(kk, S.nrm);
+    f3 td = smul(nnt, rd);
+    td = sub3(td, nkk);
+    td = norm3(td);
+    const float a = nt - nc, b = nt + nc;
+    const float R0 = a * a / (b * b);
+    const float cc = 1 - (into ? -ddn : dot3(td, S.nrm));
+    const float Re = R0 + (1 - R0) * cc * cc * cc * cc * cc;
+    const float Tr = 1.f - Re;
+    const float Pp = .25f + .5f * Re;
+    const float RP = Re / Pp;
+    const float TP = Tr / (1.f - Pp);
+    if (get_random(s0, s1) < Pp) {
+        thr = smul(RP, thr);
+        thr = mul3(thr, S.c);
+        rd = nd;
+    } else {
+        thr = smul(TP, thr);
+        thr = mul3(thr, S.c);
+        rd = td;
+    }
+}
+
+// ---- running mean of RadianceGPU (rendering_kernel.cl:1127-1136) ----
+__device__ __forceinline__ f3 running_mean(f3 col, f3 rad, int cs) {
+    if (cs == 0) return rad;
+    const float k1 = (float)cs;
+    const float k2 = 1.f / ((float)cs + 1.f);
+    return mk((col.x * k1 + rad.x) * k2, (col.y * k1 + rad.y) * k2, (col.z * k1 + rad.z) * k2);
+}
+
+__device__ __forceinline__ unsigned pack_rgba(f3 col) {
+    return (unsigned)to_int(col.x) | ((unsigned)to_int(col.y) << 8) | ((unsigned)to_int(col.z) << 16) | 0xff000000u;
+}
+
+}  // namespace ptgpu

@@ -60,6 +60,8 @@ _SIGS = {
     "ptgpu_ctx_create": [i32, vp, C.POINTER(vp)],
     "ptgpu_ctx_destroy": [vp],
     "ptgpu_ctx_set_max_depth": [vp, i32],
+    "ptgpu_ctx_set_engine": [vp, i32],
+    "ptgpu_ctx_set_tuning": [vp, C.c_char_p, i32],
     "ptgpu_synchronize": [vp],
     "ptgpu_malloc": [vp, C.c_size_t, C.POINTER(vp)],
     "ptgpu_free": [vp, vp],
@@ -69,6 +71,7 @@ _SIGS = {
     "ptgpu_memcpy_d2d": [vp, vp, vp, C.c_size_t],
     "ptgpu_host_free": [vp],
     "ptgpu_event_record": [vp, i32],
+    "ptgpu_copy_calibration": [vp, vp, vp, C.c_size_t],
     "ptgpu_event_elapsed_ms": [vp, i32, i32, C.POINTER(f32)],
     "ptgpu_clbvh_create": [vp, vp, i32, C.POINTER(vp)],
     "ptgpu_clbvh_create_device": [vp, vp, i32, C.POINTER(vp)],
@@ -199,6 +202,9 @@ class DeviceArray:
             pass
 
 
+ENGINES = {"auto": 0, "megakernel": 1, "wavefront": 2, "lanes": 3}
+
+
 class Context:
     """One device + one HIP stream (replaces the OpenCL context/queue, smallptGPU.cpp:286-441)."""
 
@@ -236,6 +242,13 @@ class Context:
     def __exit__(self, *exc):
         self.close()
 
+    def set_engine(self, engine: str) -> None:
+        """'auto', 'megakernel' or 'wavefront' (same results)."""
+        _check(self.lib.ptgpu_ctx_set_engine(self.handle, ENGINES[engine]))
+
+    def set_tuning(self, name: str, value: int) -> None:
+        _check(self.lib.ptgpu_ctx_set_tuning(self.handle, name.encode(), int(value)))
+
     def set_max_depth(self, d: int) -> None:
         _check(self.lib.ptgpu_ctx_set_max_depth(self.handle, d))
 
@@ -248,6 +261,9 @@ class Context:
     def to_device(self, a: np.ndarray) -> DeviceArray:
         a = np.ascontiguousarray(a)
         return DeviceArray(self, a.nbytes, a.dtype, a.shape).upload(a)
+
+    def copy_calibration(self, dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
+        _check(self.lib.ptgpu_copy_calibration(self.handle, dst.ptr, src.ptr, nbytes))
 
     def record(self, slot: int) -> None:
         _check(self.lib.ptgpu_event_record(self.handle, slot))

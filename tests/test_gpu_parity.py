@@ -298,6 +298,51 @@ def test_render_icosphere_kd_vs_oracle(ctx):
     image_checks(c_g, s_g, p_g, c_o, s_o, p_o)
 
 
+@pytest.mark.parametrize("scene", ["soup_bvh", "cornell_bvh", "icosphere_kd", "soup_kd_sah"])
+def test_wavefront_equals_megakernel(ctx, scene):
+    """The two schedules run the same device functions in the same order: same bits."""
+    say("case engines", scene)
+    if scene == "soup_bvh":
+        shapes, lc, cam = scenes.soup(20000, seed=12, width=96, height=64)
+    elif scene == "cornell_bvh":
+        shapes, lc, cam = scenes.cornell(64, 48)
+    else:
+        shapes, lc, cam = scenes.icosphere_in_box(4, seed=3, width=64, height=48)
+    if scene == "soup_kd_sah":
+        shapes, lc, cam = scenes.soup(20000, seed=13, width=64, height=64)
+    pt.update_camera(cam)
+    w, h = int(cam["width"][0]), int(cam["height"][0])
+    s = shapes.copy()
+    bvh = kd = None
+    if scene.endswith("bvh"):
+        bvh = pt.CLBVH(ctx, s)
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+    elif scene == "icosphere_kd":
+        kd = pt.KDTree(ctx, s)
+    else:
+        kd = pt.KDTree(ctx, s, sah=True, max_depth=24, max_leaf=2)
+    d_sh = ctx.to_device(s)
+    out = {}
+    for eng in ("megakernel", "wavefront", "lanes"):
+        ctx.set_engine(eng)
+        out[eng] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd, d_shapes=d_sh)[:3]
+    # per-lane states with few lanes (several pixels per lane) and extreme regroup thresholds
+    for regroup, bpc, wide in ((1, 1, 1), (64, 1, 0), (7, 2, 1), (8, 8, 0)):
+        ctx.set_tuning("regroup", regroup)
+        ctx.set_tuning("blocks_per_cu", bpc)
+        ctx.set_tuning("wide", wide)
+        out[f"lanes{regroup}/{bpc}/{wide}"] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd,
+                                                       d_shapes=d_sh)[:3]
+    ctx.set_tuning("regroup", 8)
+    ctx.set_tuning("blocks_per_cu", 2)
+    ctx.set_tuning("wide", 1)
+    ctx.set_engine("auto")
+    for eng, res in out.items():
+        for a, b in zip(out["megakernel"], res):
+            assert a.tobytes() == b.tobytes(), eng
+
+
 def test_fused_spp_equals_separate_launches(ctx):
     shapes, lc, cam = scenes.soup(5000, seed=6, width=32, height=32)
     pt.update_camera(cam)
