@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--regroup", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--wide", type=int, default=None)
+    ap.add_argument("--treelet", type=int, default=0, help="treelet size for CLBVH::optimize (0: no optimize)")
+    ap.add_argument("--treelet-passes", type=int, default=1)
     ap.add_argument("--build-sweep", type=str, default=None,
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
@@ -167,6 +169,8 @@ def main():
         bvh.reset()
         bvh.buildRadixTree()
         bvh.buildBVHTree()
+        for _ in range(args.treelet_passes if args.treelet else 0):
+            bvh.optimize(args.treelet)
         d_seeds.copy_from(seeds0)
         ctx.record(1)
         pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_shapes, shapeCnt=n,

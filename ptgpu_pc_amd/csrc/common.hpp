@@ -77,7 +77,7 @@ struct DevAccel {
     int nprims = 0;
     int nnodes = 0;        // number of TNodes (node refs are checked against it)
     int nshapes = 0;       // shapeCnt: any-hit queries ignore shape nshapes-1 (IntersectP, Q10)
-    int depth = 0;         // max internal-node depth (root = 0); stack bound
+    int depth = 0;         // tree height: deepest leaf in edges from the root (bounds the stack)
     int mode = PTGPU_ACCEL_NONE;
     int leaf_runs = 0;     // 1: every leaf range is disjoint and its last Prim carries kLastInLeaf
 };

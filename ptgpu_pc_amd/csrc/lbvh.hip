@@ -926,12 +926,12 @@ int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12,
     std::vector<TNode> tn((size_t)m);
     PT_HIP(hipMemcpyAsync(tn.data(), h->tnodes.p, (size_t)m * sizeof(TNode), hipMemcpyDeviceToHost, h->ctx->stream));
     std::vector<int4> mn;
-    if (h->trav_topology == 1) {
+    if (h->trav_topology == 1 && !h->optimized) {
         mn.resize((size_t)m);
         PT_HIP(hipMemcpyAsync(mn.data(), h->mnode.p, (size_t)m * sizeof(int4), hipMemcpyDeviceToHost, h->ctx->stream));
     }
     std::vector<BVHNodeGPU> bt;
-    if (h->trav_topology == 0) {
+    if (h->trav_topology == 0 && !h->optimized) {
         bt.resize((size_t)m);
         PT_HIP(hipMemcpyAsync(bt.data(), h->btn.p, (size_t)m * sizeof(BVHNodeGPU), hipMemcpyDeviceToHost,
                               h->ctx->stream));
@@ -941,8 +941,9 @@ int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12,
         const TNode &t = tn[i];
         h_refs4[i * 4 + 0] = t.r.x;
         h_refs4[i * 4 + 1] = t.r.y;
-        h_refs4[i * 4 + 2] = h->trav_topology == 1 ? mn[i].z : bt[i].min;
-        h_refs4[i * 4 + 3] = h->trav_topology == 1 ? mn[i].w : bt[i].max;
+        // leaf ranges [lo, hi] of the build's tree; an optimised tree has none (-1)
+        h_refs4[i * 4 + 2] = h->optimized ? -1 : h->trav_topology == 1 ? mn[i].z : bt[i].min;
+        h_refs4[i * 4 + 3] = h->optimized ? -1 : h->trav_topology == 1 ? mn[i].w : bt[i].max;
         const float v[12] = {t.a.x, t.a.w, t.a.y, t.b.x, t.a.z, t.b.y, t.b.z, t.c.y, t.b.w, t.c.z, t.c.x, t.c.w};
         for (int f = 0; f < 12; f++) h_boxes12[i * 12 + f] = v[f];   // two Bounds in reference order
     }
@@ -1031,6 +1032,7 @@ int ptgpu_clbvh_reset(ptgpu_clbvh *h, int n) {
     PT_CHECK(n >= 1 && n <= h->cap, PTGPU_ERR_INVALID, "n=%d exceeds the handle capacity %d", n, h->cap);
     h->n = n;
     h->optimized = false;
+    h->export_pending = false;
     return clbvh_construct(h);
 }
 
@@ -1051,6 +1053,7 @@ int ptgpu_clbvh_build_radix_tree(ptgpu_clbvh *h) {
     h->timed[2] = true;
     h->state = 2;
     h->optimized = false;
+    h->export_pending = false;
     return PTGPU_OK;
 }
 
@@ -1069,6 +1072,7 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     h->timed[3] = true;
     h->state = 3;
     h->optimized = false;
+    h->export_pending = false;
     h->acc.nodes = h->tnodes.as<TNode>();
     h->acc.prims = h->prims.as<Prim>();
     h->acc.nprims = n;
@@ -1099,16 +1103,10 @@ int ptgpu_clbvh_set_digit_bits(ptgpu_clbvh *h, int bits) {
     return PTGPU_OK;
 }
 
-int ptgpu_clbvh_optimize(ptgpu_clbvh *h, int treelet_size) {
-    PT_CHECK(h, PTGPU_ERR_INVALID, "null handle");
-    (void)treelet_size;
-    ptgpu::set_error("treelet optimisation is not built yet");
-    return PTGPU_ERR_UNSUPPORTED;
-}
-
 int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl) {
     PT_CHECK(h && h_btn && h_btl, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(h->state >= 2, PTGPU_ERR_INVALID, "no tree built");
+    PT_TRY(clbvh_export_optimized(h));
     const int n = h->n;
     h->h_btn.resize(n > 1 ? n - 1 : 0);
     h->h_btl.resize(n);
@@ -1129,6 +1127,7 @@ int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl
 int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl) {
     PT_CHECK(h && d_btn && d_btl, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(h->state >= 2, PTGPU_ERR_INVALID, "no tree built");
+    PT_TRY(clbvh_export_optimized(h));
     *d_btn = h->btn.as<BVHNodeGPU>();
     *d_btl = h->btl.as<BVHNodeGPU>();
     return PTGPU_OK;
@@ -1161,6 +1160,7 @@ int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
     h->hist.release(); h->status.release(); h->misc.release(); h->own_shapes.release();
     h->lights.release(); h->scan.release(); h->opt_tmp.release();
     h->mnode.release(); h->mpar.release(); h->gbi.release(); h->xlist.release(); h->tnodes4.release();
+    h->treelet.release();
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);
     delete h;

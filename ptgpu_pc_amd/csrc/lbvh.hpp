@@ -5,6 +5,11 @@
 
 #include "common.hpp"
 
+namespace ptgpu {
+int clbvh_prepare_traversal(ptgpu_clbvh *h);
+int clbvh_export_optimized(ptgpu_clbvh *h);
+}  // namespace ptgpu
+
 struct ptgpu_clbvh {
     ptgpu_ctx *ctx = nullptr;
     int n = 0, cap = 0;
@@ -20,12 +25,16 @@ struct ptgpu_clbvh {
     ptgpu::DevBuf mnode, mpar, gbi, xlist;   // Morton-keyed traversal tree: topology, parents, refit scratch
     ptgpu::DevBuf tnodes4;                   // four-wide view of tnodes, rebuilt after each refit
     bool wide_pending = false;
+    ptgpu::DevBuf treelet;                   // treelet optimiser state (kept for the lazy export)
+    bool export_pending = false;             // btn/btl still describe the tree before optimize
     int trav_topology = 1;    // traversal tree: 0 = the exported index-keyed tree, 1 = Morton-keyed LBVH
     bool height_pending = false;
     unsigned *sorted = nullptr;
     int lights_for = -1, nlights = 0;
     ptgpu::DevAccel acc;
     std::vector<BVHNodeGPU> h_btn, h_btl;
+    // device word holding the traversal tree's height (read back lazily by clbvh_prepare_traversal)
+    unsigned *height_word() const { return misc.as<unsigned>() + 8 + 11; }
     // events: 0/1 morton, 2/3 topology, 4/5 sort, 5/6 refit, 7/8 optimize
     hipEvent_t ev[10] = {};
     bool timed[5] = {false, false, false, false, false};
