@@ -74,6 +74,99 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
                       f"shadow rays over 65,538 shapes), {elapsed:.1f} s"}
 
 
+RENDER_KERNEL = "k_render_lanes"
+WORKLOAD = ("LBVH build + BVH traversal: 65,536-triangle soup (seed 1) + emissive quad, 512x512, 8 spp, depth 5, "
+            "30-bit Morton, one leaf per primitive")
+TRAFFIC_FILE = ROOT / "profiles" / "k_render_lanes_c2_traffic.json"
+CALIB_BYTES = 256 << 20      # the calibration copy reads and writes this many bytes
+
+
+def _db_rows(db, sql):
+    import sqlite3
+    con = sqlite3.connect(str(db))
+    try:
+        return list(con.execute(sql))
+    finally:
+        con.close()
+
+
+def _counter(db, name, kernel_like):
+    import sqlite3
+    con = sqlite3.connect(str(db))
+    try:
+        r = con.execute("select avg(value), count(*) from counters_collection where counter_name = ? and kernel_name "
+                        "like ? and kernel_name not like '%, true, %'", (name, f"%{kernel_like}%")).fetchone()
+    finally:
+        con.close()
+    return r
+
+
+def profile_summary(d: Path, out_md: Path) -> None:
+    """Summarise rocprofv3 results of the default workload into profiles/: kernel-trace statistics,
+    FETCH_SIZE / WRITE_SIZE calibrated against the known-size copy kernel, SQ/TCC ratios."""
+    d = Path(d)
+    kt = next(d.glob("kt/**/*results.db"))
+    rows = _db_rows(kt, "select name, total_calls, average, percentage from top_kernels order by total_duration desc "
+                        "limit 12")
+    fetch_db, write_db = next(d.glob("fetch/**/*results.db")), next(d.glob("write/**/*results.db"))
+    f_k, f_n = _counter(fetch_db, "FETCH_SIZE", RENDER_KERNEL)
+    w_k, w_n = _counter(write_db, "WRITE_SIZE", RENDER_KERNEL)
+    f_c = _counter(fetch_db, "FETCH_SIZE", "k_copy_calib")[0]
+    w_c = _counter(write_db, "WRITE_SIZE", "k_copy_calib")[0]
+    f_scale = CALIB_BYTES / (f_c * 1024.0)     # bytes per reported KiB, from the known copy
+    w_scale = CALIB_BYTES / (w_c * 1024.0)
+    traffic = f_k * 1024.0 * f_scale + w_k * 1024.0 * w_scale
+    sq = {}
+    for dbp in d.glob("sq*/**/*results.db"):
+        for name, val in _db_rows(dbp, "select counter_name, avg(value) from counters_collection where kernel_name "
+                                       f"like '%{RENDER_KERNEL}%' and kernel_name not like '%, true, %' group by "
+                                       "counter_name"):
+            sq[name] = val
+    render_avg = next((r[2] / 1e3 for r in rows if RENDER_KERNEL in r[0] and ", true," not in r[0]), None)
+    cmds = json.loads((d / "commands.json").read_text())
+    lines = [f"# {RENDER_KERNEL} — C2 (65,538-shape soup, 512x512, 8 spp, depth 5), MI355X", "",
+             "Commands (run from the repository root; rocprofv3 output directories as named):", ""]
+    lines += [f"    {c}" for c in cmds]
+    lines += ["", "## Kernel trace (rocprofv3 --kernel-trace --stats)", "",
+              "| kernel | calls | average ms | % of GPU time |", "|---|---|---|---|"]
+    lines += [f"| `{r[0][:90]}` | {r[1]} | {r[2] / 1e3:.4f} | {r[3]:.2f} |" for r in rows]   # rocpd: µs
+    lines += ["", "## HBM traffic per render launch", "",
+              f"* calibration: `k_copy_calib` reads and writes {CALIB_BYTES >> 20} MiB; FETCH_SIZE reports {f_c:.0f} KiB "
+              f"(true/reported = {f_scale:.4f}), WRITE_SIZE reports {w_c:.0f} KiB (true/reported = {w_scale:.4f})",
+              f"* {RENDER_KERNEL}: FETCH_SIZE {f_k:.0f} KiB ({f_n} dispatches), WRITE_SIZE {w_k:.0f} KiB",
+              f"* calibrated traffic: {traffic / 1e9:.3f} GB per launch "
+              f"({traffic / 1e9 / render_avg * 1e3 if render_avg else 0:.0f} GB/s at the traced average duration)"]
+    if sq:
+        lines += ["", "## SQ / TCC ratios", ""]
+        if "SQ_THREAD_CYCLES_VALU" in sq and "SQ_ACTIVE_INST_VALU" in sq:
+            lines.append(f"* VALU lane utilisation: {sq['SQ_THREAD_CYCLES_VALU'] / (sq['SQ_ACTIVE_INST_VALU'] * 64):.3f}")
+        if "SQ_ACTIVE_INST_VALU" in sq and "GRBM_GUI_ACTIVE" in sq:
+            simds, xcds = 1024, 8
+            busy = sq["SQ_ACTIVE_INST_VALU"] * 4 / (simds * sq["GRBM_GUI_ACTIVE"] / xcds)
+            lines.append(f"* VALU busy (quad-cycles x4 / (1024 SIMDs x GPU cycles)): {busy:.3f}")
+        if "TCC_HIT_sum" in sq:
+            lines.append(f"* L2 hit rate: {sq['TCC_HIT_sum'] / (sq['TCC_HIT_sum'] + sq['TCC_MISS_sum']):.4f}")
+        for k in sorted(sq):
+            lines.append(f"* {k}: {sq[k]:.4g}")
+    out_md.write_text("\n".join(lines) + "\n")
+    TRAFFIC_FILE.write_text(json.dumps({"kernel": RENDER_KERNEL, "workload": WORKLOAD, "traffic_bytes": traffic,
+                                        "fetch_kib": f_k, "write_kib": w_k, "fetch_factor": f_scale,
+                                        "write_factor": w_scale, "avg_ms_rocprof": render_avg,
+                                        "summary": out_md.name}, indent=1) + "\n")
+    print(out_md.read_text())
+
+
+def measured_traffic():
+    """HBM bytes per render launch from the committed rocprofv3 summary of this workload (or None)."""
+    try:
+        t = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != RENDER_KERNEL or t.get("workload") != WORKLOAD:
+        return None
+    return int(t["traffic_bytes"])
+
+
 # algorithmic bytes per primitive of each build stage (each logical load/store counted once at its
 # natural width, atomics 8 B): Morton 36+36+4, 8-bit onesweep 4 + 16*4, topology 28, refit 156
 STAGE_BYTES = {"morton": 76, "sort": 68, "topology": 28, "refit": 156}
@@ -128,9 +221,14 @@ def main():
     ap.add_argument("--wide", type=int, default=None)
     ap.add_argument("--treelet", type=int, default=0, help="treelet size for CLBVH::optimize (0: no optimize)")
     ap.add_argument("--treelet-passes", type=int, default=1)
+    ap.add_argument("--profile-summary", type=str, default=None,
+                    help="directory of rocprofv3 results (kt/, fetch/, write/, sq*/, commands.json) to summarise")
     ap.add_argument("--build-sweep", type=str, default=None,
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
+    if args.profile_summary:
+        profile_summary(Path(args.profile_summary), ROOT / "profiles" / "k_render_lanes_c2_soup65538_512x512x8spp.md")
+        return
     if args.build_sweep:
         build_sweep([1 << int(x) for x in args.build_sweep.split(",")])
         return
@@ -217,8 +315,9 @@ def main():
     alg_bytes = 24 * box_tests + 36 * prim_tests + W * H * (24 + 16 + 4)
     achieved = alg_bytes / (render_avg * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_render (path-tracing megakernel)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic() if args.engine in ("auto", "lanes") and not args.treelet else None,
+                "kernel": f"{RENDER_KERNEL} (path-tracing megakernel, per-lane states)",
                 "per_launch": {"box_tests": box_tests, "prim_tests": prim_tests, "alg_bytes": alg_bytes,
                                "avg_ms": round(render_avg, 4)}}
 
@@ -241,8 +340,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": "LBVH build + BVH traversal: 65,536-triangle soup (seed 1) + emissive quad, "
-                               "512x512, 8 spp, depth 5, 30-bit Morton, one leaf per primitive",
+        "config": {"workload": WORKLOAD,
                    "shapes": n, "width": W, "height": H, "spp": SPP, "max_depth": DEPTH},
         "rays_per_step": rays // args.steps,
         "closest_per_step": closest // args.steps,
