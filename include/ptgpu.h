@@ -256,6 +256,20 @@ typedef struct {
 } ptgpu_render_args;
 int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *args);
 
+/* Many frames in one launch (no reference counterpart: the batched form of the path).  Each job is
+ * one ptgpu_render call's work -- a prepared tree (exactly one of bvh / kd, each with at least two
+ * shapes), its scene and its frame buffers -- and the lanes of the launch take pixels from all jobs,
+ * so frames of different scenes, sizes and sample counts are traversed together.  Every job's
+ * colors / seeds / pixels come out bit-identical to its own ptgpu_render call.  Jobs must not share
+ * output buffers.  args.ray_counter / args.trav_stats of the jobs are ignored; d_ray_counter (may be
+ * NULL) receives the batch's {closest, any-hit} query counts. */
+typedef struct {
+    ptgpu_clbvh *bvh;
+    ptgpu_kdtree *kd;
+    ptgpu_render_args args;
+} ptgpu_render_job;
+int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter);
+
 /* Batch ray queries over the same traversal engine (the reference's Intersect and IntersectP,
  * rendering_kernel.cl:194-395).  d_rays: n Rays.  Closest: d_t[i], d_id[i] (id = original shape
  * index, t = 0 and id = -1 on a miss; ties in t resolve to the higher index).  Any-hit: d_hit[i] =

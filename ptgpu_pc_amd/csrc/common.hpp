@@ -131,6 +131,11 @@ struct ptgpu_ctx {
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
+    // batch launches: device job table, pinned staging copy, and the event that fences its reuse
+    ptgpu::DevBuf tmp_jobs;
+    void *job_pinned = nullptr;
+    size_t job_pinned_bytes = 0;
+    hipEvent_t job_event = nullptr;
 };
 
 namespace ptgpu {
@@ -142,6 +147,8 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
 int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_prims);
 // four-wide nodes for a prepared binary structure (no-op without internal nodes); sets acc->nodes4
 int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4);
+// light list cached in a KD handle (as the LBVH handle caches its own)
+int kdtree_lights(ptgpu_kdtree *kd, const Shape *d_shapes, int n, int light_cnt, const int **lights, int *nlights);
 // reference-layout trees -> traversal layout
 int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn,
                     const BVHNodeGPU *d_btl, DevBuf &nodes, DevBuf &prims, DevAccel *acc);

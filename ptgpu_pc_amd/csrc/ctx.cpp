@@ -115,6 +115,9 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     ctx->tmp_lights.release();
     ctx->tmp_scan.release();
     ctx->tmp_counter.release();
+    ctx->tmp_jobs.release();
+    if (ctx->job_pinned) (void)hipHostFree(ctx->job_pinned);
+    if (ctx->job_event) (void)hipEventDestroy(ctx->job_event);
     wf_release(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

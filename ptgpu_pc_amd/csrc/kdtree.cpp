@@ -90,6 +90,8 @@ struct ptgpu_kdtree {
     std::vector<int> kn;          // shape indices, leaves in depth-first order
     std::vector<Shape> shapes;    // host copy (for the heap export bounds and prims)
     DevBuf d_nodes, d_nodes4, d_prims;
+    DevBuf lights, scan;              // light list cache (index order), for lightCnt == lights_for
+    int lights_for = -1, nlights = 0;
     DevAccel acc;
     bool sah = false;
 };
@@ -398,6 +400,19 @@ int kd_common(ptgpu_ctx *ctx, const Shape *h_shapes, int n, bool sah, int max_de
 
 }  // namespace
 
+namespace ptgpu {
+int kdtree_lights(ptgpu_kdtree *kd, const Shape *d_shapes, int n, int light_cnt, const int **lights, int *nlights) {
+    PT_CHECK(kd->ctx != nullptr, PTGPU_ERR_INVALID, "KD tree was built without a context (host only)");
+    if (kd->lights_for != light_cnt) {
+        PT_TRY(build_light_list(kd->ctx, d_shapes, n, light_cnt, kd->lights, kd->scan, &kd->nlights));
+        kd->lights_for = light_cnt;
+    }
+    *lights = kd->lights.as<int>();
+    *nlights = kd->nlights;
+    return PTGPU_OK;
+}
+}  // namespace ptgpu
+
 extern "C" {
 
 int ptgpu_kdtree_build(ptgpu_ctx *ctx, const Shape *h_shapes, int n, ptgpu_kdtree **out) {
@@ -502,6 +517,8 @@ int ptgpu_kdtree_destroy(ptgpu_kdtree *kd) {
     if (kd->ctx && kd->ctx->stream) (void)hipStreamSynchronize(kd->ctx->stream);
     kd->d_nodes.release();
     kd->d_nodes4.release();
+    kd->lights.release();
+    kd->scan.release();
     kd->d_prims.release();
     delete kd;
     return PTGPU_OK;

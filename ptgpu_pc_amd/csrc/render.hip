@@ -17,11 +17,26 @@
 //     the next pixel from a global counter, wave-wide through one atomic (ballot + popcount).
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "lbvh.hpp"
 #include "render_device.hpp"
 
 namespace ptgpu {
+
+// One frame's work in a batch: a prepared tree, its scene and the frame's buffers (the reference's
+// RadianceGPU arguments).
+struct RenderJob {
+    DevAccel acc;
+    Vec *colors;
+    unsigned *seeds;
+    const Camera *camera;
+    const Shape *shapes;
+    const int *lights;
+    int nlights;
+    int width, height, currentSample, spp, max_depth;
+    unsigned *pixels;
+};
 
 struct RenderParams {
     DevAccel acc;
@@ -38,6 +53,9 @@ struct RenderParams {
     unsigned *work_counter;
     int total;
     int regroup;
+    const RenderJob *jobs;        // batch launches: job table, pixel prefix sums (njobs + 1)
+    const unsigned *job_begin;
+    int njobs;
 };
 
 // The path-tracing megakernel.  Lane state: one pixel (gid), its remaining samples, the RNG state
@@ -194,12 +212,19 @@ enum : int {
     kIdle = 7,    // no work left
 };
 
-template <int STACK, bool STATS, bool WIDE>
+template <int STACK, bool STATS, bool WIDE, bool MULTI>
 __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
-    const DevAccel &A = P.acc;
     const unsigned long long lt = lanemask_lt();
+    // frame fields: the launch's own, or (batch) the lane's job's
+#define JF(field) (MULTI ? P.jobs[jb].field : P.field)
+    int jb = 0;
+    // the tree of the lane's current query (uniform for a single frame)
+    const TNode *t_nodes = P.acc.nodes;
+    const TNode4 *t_nodes4 = P.acc.nodes4;
+    const Prim *t_prims = P.acc.prims;
+    int t_nnodes = P.acc.nnodes, t_nprims = P.acc.nprims, t_nshapes = P.acc.nshapes, t_root = P.acc.root;
 
     int st = kNeed;
     bool drained = false;
@@ -216,13 +241,13 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
 
     // the primitives of one leaf for the lane's current query; true when an any-hit query is decided
     auto leaf = [&](int first, int cnt) -> bool {
-        if (first < 0 || cnt > A.nprims - first) return false;   // malformed caller tree
+        if (first < 0 || cnt > t_nprims - first) return false;   // malformed caller tree
         for (int q = 0; q < cnt; q++) {
             int id;
-            const float t = prim_isect(A.prims, first + q, ro, rd, &id);
+            const float t = prim_isect(t_prims, first + q, ro, rd, &id);
             if (STATS) st_prims++;
             if (qany) {
-                if (t != 0.f && t < tb && id != A.nshapes - 1) { bid = 1; return true; }
+                if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; return true; }
             } else if (t != 0.f && (t < tb || (t == tb && id > bid))) {
                 tb = t;
                 bid = id;
@@ -236,7 +261,17 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         tb = any ? maxt : kInf;
         bid = -1;
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-        node = A.root;
+        if (MULTI) {
+            const DevAccel &A = P.jobs[jb].acc;
+            t_nodes = A.nodes;
+            t_nodes4 = A.nodes4;
+            t_prims = A.prims;
+            t_nnodes = A.nnodes;
+            t_nprims = A.nprims;
+            t_nshapes = A.nshapes;
+            t_root = A.root;
+        }
+        node = t_root;
         sp = 0;
         nv = 0;
         st = kTrav;
@@ -256,10 +291,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             if (!trav || (unsigned)__popcll(live & ~trav) >= regroup) break;
             if (!WIDE && st == kTrav) {
                 bool done = false;
-                if ((unsigned)node >= (unsigned)A.nnodes || ++nv > kMaxVisits) {
+                if ((unsigned)node >= (unsigned)t_nnodes || ++nv > kMaxVisits) {
                     done = true;   // malformed caller tree
                 } else {
-                    const TNode *np = A.nodes + node;
+                    const TNode *np = t_nodes + node;
                     const float4 a = np->a, b = np->b, c = np->c;
                     const int4 r = np->r;
                     if (STATS) st_nodes += 2;
@@ -294,10 +329,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             }
             if (WIDE && st == kTrav) {
                 bool done = false;
-                if ((unsigned)node >= (unsigned)A.nnodes || ++nv > kMaxVisits) {
+                if ((unsigned)node >= (unsigned)t_nnodes || ++nv > kMaxVisits) {
                     done = true;   // malformed caller tree
                 } else {
-                    const TNode4 *np = A.nodes4 + node;
+                    const TNode4 *np = t_nodes4 + node;
                     const float4 lx = np->lox, ly = np->loy, lz = np->loz;
                     const float4 hx = np->hix, hy = np->hiy, hz = np->hiz;
                     const int4 rf = np->ref, cn = np->cnt;
@@ -363,13 +398,24 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             if (st == kNeed) {
                 const unsigned my = base + (unsigned)__popcll(need & lt);
                 if (my < (unsigned)P.total) {
-                    gid = (int)my;
-                    const int x = gid % P.width, y = gid / P.width;
-                    s0 = P.seeds[2 * gid];
-                    s1 = P.seeds[2 * gid + 1];
+                    if (MULTI) {   // the job holding work item `my` (prefix sums of the jobs' pixel counts)
+                        int lo = 0, hi = P.njobs - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (P.job_begin[mid] <= my) lo = mid; else hi = mid - 1;
+                        }
+                        jb = lo;
+                        gid = (int)(my - P.job_begin[jb]);
+                    } else {
+                        gid = (int)my;
+                    }
+                    const int W = JF(width), H = JF(height);
+                    const int x = gid % W, y = gid / W;
+                    s0 = JF(seeds)[2 * gid];
+                    s1 = JF(seeds)[2 * gid + 1];
                     k = 0;
-                    if (P.currentSample > 0) col = ldvec(P.colors + (P.height - y - 1) * P.width + x);
-                    camera_ray(P.camera, s0, s1, P.width, P.height, x, y, ro, rd);
+                    if (JF(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
+                    camera_ray(JF(camera), s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
@@ -384,7 +430,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     st = kEnded;
                 } else {
                     const f3 hp = add3(ro, smul(tb, rd));
-                    const Surface S = surface_at(P.shapes + bid, hp, rd);
+                    const Surface S = surface_at(JF(shapes) + bid, hp, rd);
                     if (!iszero3(S.e)) {
                         if (spec) rad = add3(rad, mul3(thr, smul(fabsf(S.dp), S.e)));
                         st = kEnded;
@@ -414,8 +460,9 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 // lights whose sample needs no shadow ray only draw their random numbers
                 float len = 0.f;
                 bool shoot = false;
-                for (; li < P.nlights; li++) {
-                    if (sample_light(P.shapes + P.lights[li], s0, s1, ro, nl, &rd, &len, &contrib)) {
+                const int nlights = JF(nlights);
+                for (; li < nlights; li++) {
+                    if (sample_light(JF(shapes) + JF(lights)[li], s0, s1, ro, nl, &rd, &len, &contrib)) {
                         shoot = true;
                         break;
                     }
@@ -431,21 +478,24 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 depth++;
                 st = kSeg;
             }
-            if (st == kSeg && depth > P.max_depth) st = kEnded;
+            if (st == kSeg && depth > JF(max_depth)) st = kEnded;
             if (st == kEnded) {
-                col = running_mean(col, rad, P.currentSample + k);
+                col = running_mean(col, rad, JF(currentSample) + k);
                 k++;
-                const int x = gid % P.width, y = gid / P.width;
-                if (k < P.spp) {
-                    camera_ray(P.camera, s0, s1, P.width, P.height, x, y, ro, rd);
+                const int W = JF(width), H = JF(height);
+                const int x = gid % W, y = gid / W;
+                if (k < JF(spp)) {
+                    camera_ray(JF(camera), s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
-                    float *cp = reinterpret_cast<float *>(P.colors + (P.height - y - 1) * P.width + x);
+                    float *cp = reinterpret_cast<float *>(JF(colors) + (H - y - 1) * W + x);
                     cp[0] = col.x; cp[1] = col.y; cp[2] = col.z;
-                    if (P.pixels) P.pixels[y * P.width + x] = pack_rgba(col);
-                    P.seeds[2 * gid] = s0;
-                    P.seeds[2 * gid + 1] = s1;
+                    unsigned *px = JF(pixels);
+                    if (px) px[y * W + x] = pack_rgba(col);
+                    unsigned *sd = JF(seeds);
+                    sd[2 * gid] = s0;
+                    sd[2 * gid + 1] = s1;
                     st = drained ? kIdle : kNeed;
                 }
             }
@@ -477,6 +527,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             atomicAdd(P.trav_stats + 1, b);
         }
     }
+#undef JF
 }
 
 // Batch queries (Intersect / IntersectP on arbitrary rays).
@@ -818,6 +869,9 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.work_counter = ctx->tmp_counter.as<unsigned>();
     P.total = total;
     P.regroup = ctx->regroup;
+    P.jobs = nullptr;
+    P.job_begin = nullptr;
+    P.njobs = 0;
     // persistent waves: enough workgroups to fill every CU several times, never more than needed
     int blocks = ctx->cu_count * ctx->blocks_per_cu;
     const int need = (total + 255) / 256;
@@ -827,7 +881,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     // size tried); brute force and single-leaf scenes keep the lockstep megakernel
     if ((ctx->engine == 0 || ctx->engine == 3) && acc.mode != PTGPU_ACCEL_NONE && acc.root >= 0) {
         const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc) : -1;
-#define PT_LANES(ST, STATS_, WIDE_) hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_>), dim3(blocks), dim3(256), 0, \
+#define PT_LANES(ST, STATS_, WIDE_) hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_, false>), dim3(blocks), dim3(256), 0, \
                                                        ctx->stream, P)
         if (sw > 0) {
             if (L.trav_stats) {
@@ -999,6 +1053,100 @@ int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu
     L.ray_counter = a->ray_counter;
     L.trav_stats = a->trav_stats;
     return launch_render(ctx, acc, L);
+}
+
+int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter) {
+    PT_CHECK(ctx && (jobs || njobs == 0) && njobs >= 0, PTGPU_ERR_INVALID, "null argument");
+    if (njobs == 0) return PTGPU_OK;
+    PT_HIP(hipSetDevice(ctx->device));
+    std::vector<RenderJob> table((size_t)njobs);
+    std::vector<unsigned> begin((size_t)njobs + 1, 0u);
+    int st = 0;
+    unsigned long long total = 0;
+    for (int j = 0; j < njobs; j++) {
+        const ptgpu_render_job &J = jobs[j];
+        const ptgpu_render_args *a = &J.args;
+        PT_CHECK(!!J.bvh != !!J.kd, PTGPU_ERR_INVALID, "job %d: pass exactly one of bvh / kd", j);
+        PT_CHECK(a->colors && a->seeds && a->camera && a->shapes, PTGPU_ERR_INVALID, "job %d: null buffer", j);
+        PT_CHECK(a->shapeCnt >= 2, PTGPU_ERR_UNSUPPORTED, "job %d: a batch needs at least two shapes per scene", j);
+        PT_CHECK(a->width > 0 && a->height > 0, PTGPU_ERR_INVALID, "job %d: bad frame size %dx%d", j, a->width,
+                 a->height);
+        PT_CHECK(a->spp >= 1 && a->currentSample >= 0, PTGPU_ERR_INVALID, "job %d: bad spp/currentSample", j);
+        PT_CHECK(a->max_depth >= 0 && a->max_depth < 1024, PTGPU_ERR_INVALID, "job %d: bad max_depth", j);
+        RenderJob &R = table[(size_t)j];
+        PT_TRY(resolve_accel(ctx, J.bvh, J.kd, a->shapes, a->shapeCnt, &R.acc));
+        PT_CHECK(R.acc.root >= 0 && R.acc.nodes4, PTGPU_ERR_UNSUPPORTED, "job %d: the tree has no internal node", j);
+        const int sw = stack_for_wide(R.acc);
+        PT_CHECK(sw > 0, PTGPU_ERR_UNSUPPORTED, "job %d: tree height %d exceeds the traversal stack", j, R.acc.depth);
+        st = sw > st ? sw : st;
+        if (J.bvh) {
+            if (J.bvh->lights_for != (int)a->lightCnt) {
+                PT_TRY(build_light_list(ctx, a->shapes, (int)a->shapeCnt, (int)a->lightCnt, J.bvh->lights,
+                                        J.bvh->scan, &J.bvh->nlights));
+                J.bvh->lights_for = (int)a->lightCnt;
+            }
+            R.lights = J.bvh->lights.as<int>();
+            R.nlights = J.bvh->nlights;
+        } else {
+            PT_TRY(kdtree_lights(J.kd, a->shapes, (int)a->shapeCnt, (int)a->lightCnt, &R.lights, &R.nlights));
+        }
+        R.colors = a->colors;
+        R.seeds = a->seeds;
+        R.camera = a->camera;
+        R.shapes = a->shapes;
+        R.width = a->width;
+        R.height = a->height;
+        R.currentSample = a->currentSample;
+        R.spp = a->spp;
+        R.max_depth = a->max_depth;
+        R.pixels = a->pixels;
+        begin[(size_t)j] = (unsigned)total;
+        total += (unsigned long long)a->width * (unsigned long long)a->height;
+        PT_CHECK(total < (1ull << 31), PTGPU_ERR_UNSUPPORTED, "batch of more than 2^31 pixels");
+    }
+    begin[(size_t)njobs] = (unsigned)total;
+    // stage the table through pinned memory; the previous batch's copy must be done before reuse
+    const size_t tb = table.size() * sizeof(RenderJob), bb = begin.size() * sizeof(unsigned);
+    const size_t bytes = tb + bb + 64;
+    if (!ctx->job_event) PT_HIP(hipEventCreateWithFlags(&ctx->job_event, hipEventDisableTiming));
+    PT_HIP(hipEventSynchronize(ctx->job_event));
+    if (ctx->job_pinned_bytes < bytes) {
+        if (ctx->job_pinned) PT_HIP(hipHostFree(ctx->job_pinned));
+        ctx->job_pinned = nullptr;
+        ctx->job_pinned_bytes = 0;
+        PT_HIP(hipHostMalloc(&ctx->job_pinned, bytes * 2, hipHostMallocDefault));
+        ctx->job_pinned_bytes = bytes * 2;
+    }
+    char *hp = static_cast<char *>(ctx->job_pinned);
+    std::memcpy(hp, table.data(), tb);
+    const size_t boff = (tb + 63) & ~size_t(63);
+    std::memcpy(hp + boff, begin.data(), bb);
+    PT_TRY(ctx->tmp_jobs.ensure(boff + bb));
+    PT_HIP(hipMemcpyAsync(ctx->tmp_jobs.p, hp, boff + bb, hipMemcpyHostToDevice, ctx->stream));
+    PT_HIP(hipEventRecord(ctx->job_event, ctx->stream));
+    PT_TRY(ctx->tmp_counter.ensure(64));
+    PT_TRY(fill_u32(ctx->stream, ctx->tmp_counter.p, 0u, 1));
+    RenderParams P;
+    std::memset(static_cast<void *>(&P), 0, sizeof(P));
+    P.acc = table[0].acc;   // unused by the batch variant
+    P.ray_counter = d_ray_counter;
+    P.trav_stats = nullptr;
+    P.work_counter = ctx->tmp_counter.as<unsigned>();
+    P.total = (int)total;
+    P.regroup = ctx->regroup;
+    P.jobs = ctx->tmp_jobs.as<RenderJob>();
+    P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(ctx->tmp_jobs.p) + boff);
+    P.njobs = njobs;
+    long long blocks = (long long)ctx->cu_count * ctx->blocks_per_cu;
+    const long long need = ((long long)total + 255) / 256;
+    if (blocks > need) blocks = need;
+    if (blocks < 1) blocks = 1;
+    if (st == 32) hipLaunchKernelGGL((k_render_lanes<32, false, true, true>), dim3((unsigned)blocks), dim3(256), 0,
+                                     ctx->stream, P);
+    else hipLaunchKernelGGL((k_render_lanes<64, false, true, true>), dim3((unsigned)blocks), dim3(256), 0,
+                            ctx->stream, P);
+    PT_HIP(hipGetLastError());
+    return PTGPU_OK;
 }
 
 int ptgpu_trace_closest(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes,

@@ -47,6 +47,10 @@ class RenderArgs(C.Structure):
                 ("max_depth", i32), ("pixels", vp), ("ray_counter", vp), ("trav_stats", vp)]
 
 
+class RenderJob(C.Structure):
+    _fields_ = [("bvh", vp), ("kd", vp), ("args", RenderArgs)]
+
+
 class RadianceArgs(C.Structure):
     _fields_ = [("colors", vp), ("seeds", vp), ("camera", vp), ("shapes", vp), ("shapeCnt", u32),
                 ("lightCnt", u32), ("btn", vp), ("btl", vp), ("kng", vp), ("kngCnt", i32), ("kn", vp),
@@ -95,6 +99,7 @@ _SIGS = {
     "ptgpu_kdtree_destroy": [vp],
     "ptgpu_radiance_gpu": [vp, i32, C.POINTER(RadianceArgs)],
     "ptgpu_render": [vp, vp, vp, C.POINTER(RenderArgs)],
+    "ptgpu_render_batch": [vp, vp, i32, vp],
     "ptgpu_trace_closest": [vp, vp, vp, vp, u32, vp, i32, vp, vp],
     "ptgpu_trace_any": [vp, vp, vp, vp, u32, vp, vp, i32, vp],
     "ptgpu_update_camera": [vp],
@@ -443,6 +448,21 @@ def render(ctx: Context, bvh: CLBVH | None, kd: KDTree | None, *, colors: Device
                    spp, max_depth, pixels.ptr if pixels else None, ray_counter.ptr if ray_counter else None,
                    trav_stats.ptr if trav_stats else None)
     _check(ctx.lib.ptgpu_render(ctx.handle, _h(bvh), _h(kd), C.byref(a)))
+
+
+def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = None) -> None:
+    """Many frames in one launch.  `jobs`: dicts with the keyword arguments of `render` plus exactly
+    one of `bvh` / `kd`; each job's outputs equal its own `render` call."""
+    arr = (RenderJob * len(jobs))()
+    for k, j in enumerate(jobs):
+        px = j.get("pixels")
+        arr[k].bvh = _h(j.get("bvh"))
+        arr[k].kd = _h(j.get("kd"))
+        arr[k].args = RenderArgs(j["colors"].ptr, j["seeds"].ptr, j["camera"].ptr, j["shapes"].ptr, j["shapeCnt"],
+                                 j["lightCnt"], j["width"], j["height"], j.get("currentSample", 0), j.get("spp", 1),
+                                 j.get("max_depth", 6), px.ptr if px else None, None, None)
+    _check(ctx.lib.ptgpu_render_batch(ctx.handle, C.cast(arr, vp) if jobs else None, len(jobs),
+                                      ray_counter.ptr if ray_counter else None))
 
 
 def radiance_gpu(ctx: Context, accel: int, colors, seeds, camera, shapes, shapeCnt, lightCnt, btn=None, btl=None,
