@@ -165,6 +165,8 @@ int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl
 /* Zero-copy access to the device trees (N-1 internal nodes, N leaves). */
 int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl);
 /* Device pointer to sorted_geometry_indices (CLBVH.cpp:257), N ints. */
+/* device memory held by the handle (its own shapes copy included), in bytes */
+int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes);
 int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted);
 /* Tree used by the traversal layout (ptgpu_render / ptgpu_trace_*) after the next build_bvh_tree:
  * 0 = the exported index-keyed Karras tree (BVH.cl:127-134), 1 = the Morton-keyed LBVH over the same
@@ -269,6 +271,42 @@ typedef struct {
     ptgpu_render_args args;
 } ptgpu_render_job;
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter);
+
+/* ---- Serving: continuous batching of render requests (no reference counterpart) ----
+ * A request renders one frame of one scene: `spp` samples from sample 0.  Requests queue in
+ * submission order.  ptgpu_server_tick admits queued requests, head of line first, while their HBM
+ * footprint fits the budget (the scene's shapes + LBVH unless the scene key is resident, plus the
+ * frame's colors/seeds/pixels), builds the scenes they bring in, and renders min(spp_per_tick,
+ * remaining) samples of every admitted request in one batched launch; it returns when that launch
+ * is done.  A scene stays resident while requests use it and afterwards until its memory is needed
+ * (least recently used first).  A finished frame stays on the device until ptgpu_server_result
+ * copies it out.  Every result is bit-identical to one ptgpu_render of the request alone.
+ * h_shapes and h_seeds must stay valid until the tick that admits the request returns.  Requests
+ * on one scene key must use the same shapes and lightCnt. */
+typedef struct ptgpu_server ptgpu_server;
+typedef struct {
+    unsigned long long scene_key;
+    const Shape *h_shapes;
+    unsigned int shapeCnt;        /* >= 2 */
+    unsigned int lightCnt;
+    Camera camera;                /* UpdateCamera already applied */
+    int width;
+    int height;
+    int spp;
+    int max_depth;
+    const unsigned int *h_seeds;  /* 2 per pixel, or NULL: seeds[2g] = seeds[2g+1] = max(g, 2) */
+} ptgpu_request;
+int ptgpu_server_create(ptgpu_ctx *ctx, size_t hbm_budget, int spp_per_tick, ptgpu_server **out);
+int ptgpu_server_submit(ptgpu_server *s, const ptgpu_request *r, int *id);
+int ptgpu_server_tick(ptgpu_server *s, int *n_active, int *n_queued);
+/* ids of the requests finished since the last call (at most cap) */
+int ptgpu_server_finished(ptgpu_server *s, int *ids, int cap, int *n);
+/* copies a finished frame out (any pointer may be NULL) and releases its device memory */
+int ptgpu_server_result(ptgpu_server *s, int id, float *h_colors, unsigned int *h_pixels, unsigned int *h_seeds);
+/* {closest, any-hit} queries so far, HBM in use, resident scenes, LBVH builds so far */
+int ptgpu_server_stats(ptgpu_server *s, unsigned long long *rays2, size_t *hbm_used, int *scenes_resident,
+                       unsigned long long *builds);
+int ptgpu_server_destroy(ptgpu_server *s);
 
 /* Batch ray queries over the same traversal engine (the reference's Intersect and IntersectP,
  * rendering_kernel.cl:194-395).  d_rays: n Rays.  Closest: d_t[i], d_id[i] (id = original shape

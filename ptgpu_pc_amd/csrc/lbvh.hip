@@ -1133,6 +1133,19 @@ int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVH
     return PTGPU_OK;
 }
 
+int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes) {
+    PT_CHECK(h && bytes, PTGPU_ERR_INVALID, "null argument");
+    const DevBuf *bufs[] = {&h->own_shapes, &h->keysA, &h->keysB, &h->valsA, &h->valsB, &h->btn, &h->btl, &h->flags,
+                            &h->tnodes, &h->prims, &h->hist, &h->status, &h->misc, &h->lights, &h->scan,
+                            &h->opt_tmp, &h->mnode, &h->mpar, &h->gbi, &h->xlist, &h->tnodes4, &h->treelet};
+    size_t t = 0;
+    for (const DevBuf *b : bufs) t += b->bytes;
+    // the four-wide view is derived at the first traversal: count it before it exists
+    if (!h->tnodes4.p && h->n > 1) t += (size_t)(h->n - 1) * sizeof(TNode4);
+    *bytes = t;
+    return PTGPU_OK;
+}
+
 int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted) {
     PT_CHECK(h && d_sorted, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(h->state >= 3, PTGPU_ERR_INVALID, "build_bvh_tree has not run");

@@ -1067,6 +1067,10 @@ int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, 
         const ptgpu_render_job &J = jobs[j];
         const ptgpu_render_args *a = &J.args;
         PT_CHECK(!!J.bvh != !!J.kd, PTGPU_ERR_INVALID, "job %d: pass exactly one of bvh / kd", j);
+        for (int i = 0; i < j; i++)   // a handle holds one light list: jobs sharing it must agree on lightCnt
+            PT_CHECK(!((jobs[i].bvh && jobs[i].bvh == J.bvh) || (jobs[i].kd && jobs[i].kd == J.kd)) ||
+                         jobs[i].args.lightCnt == a->lightCnt,
+                     PTGPU_ERR_INVALID, "jobs %d and %d share a tree but differ in lightCnt", i, j);
         PT_CHECK(a->colors && a->seeds && a->camera && a->shapes, PTGPU_ERR_INVALID, "job %d: null buffer", j);
         PT_CHECK(a->shapeCnt >= 2, PTGPU_ERR_UNSUPPORTED, "job %d: a batch needs at least two shapes per scene", j);
         PT_CHECK(a->width > 0 && a->height > 0, PTGPU_ERR_INVALID, "job %d: bad frame size %dx%d", j, a->width,

@@ -1,0 +1,351 @@
+// serve.cpp -- continuous batching of render requests on one GPU (no reference counterpart: the
+// served form of the path).
+//
+// A request is one frame of one scene: `spp` samples from sample 0, the reference's RadianceGPU
+// loop.  The server keeps three sets on the device:
+//   * resident scenes, keyed by the caller's scene key: shapes + a built LBVH, shared by every request
+//     on that scene, kept after their last request until the memory is needed (LRU);
+//   * admitted requests: their frame buffers (colors, seeds, pixels, camera) stay resident while the
+//     frame accumulates over ticks and until the caller takes the result;
+//   * the queue, in arrival order.
+// One tick admits queued requests while their footprint fits the HBM budget (head of line first),
+// builds the scenes they bring in, then renders min(spp_per_tick, remaining) samples of every admitted
+// request in one ptgpu_render_batch launch.  Splitting a frame's samples over launches gives the same
+// bits as one launch (the running mean and the RNG state carry over in colors/seeds), so every
+// request's result equals rendering it alone.
+#include <cstddef>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "device_util.hpp"
+
+namespace {
+
+constexpr size_t kSceneBytesPerPrim = 600;   // estimate before a build: Shape 104 B + LBVH buffers ~454 B
+
+// the host-side mirror (ctypes) depends on this layout
+static_assert(sizeof(ptgpu_request) == 128, "ptgpu_request");
+static_assert(offsetof(ptgpu_request, camera) == 24 && offsetof(ptgpu_request, width) == 100 &&
+                  offsetof(ptgpu_request, h_seeds) == 120,
+              "ptgpu_request offsets");
+
+struct Scene {
+    unsigned long long key = 0;
+    int n = 0;
+    unsigned light_cnt = 0;
+    ptgpu::DevBuf shapes;
+    ptgpu_clbvh *bvh = nullptr;
+    int refs = 0;
+    size_t bytes = 0;
+    unsigned long long last_use = 0;
+};
+
+struct Request {
+    int id = 0;
+    ptgpu_request r{};
+};
+
+struct Slot {
+    int id = 0;
+    ptgpu_request r{};
+    Scene *scene = nullptr;
+    ptgpu::DevBuf frame;      // colors | seeds | pixels | camera
+    size_t bytes = 0;
+    int done = 0;             // samples accumulated
+    int step = 0;             // samples in the launch in flight
+    bool finished = false;
+};
+
+size_t frame_bytes(int w, int h) {
+    const size_t px = (size_t)w * (size_t)h;
+    return ((px * 12 + 255) & ~size_t(255)) + ((px * 8 + 255) & ~size_t(255)) + ((px * 4 + 255) & ~size_t(255)) +
+           256;
+}
+
+}  // namespace
+
+struct ptgpu_server {
+    ptgpu_ctx *ctx = nullptr;
+    size_t budget = 0, used = 0;
+    int spp_per_tick = 1;
+    int next_id = 1;
+    unsigned long long tick_no = 0, builds = 0;
+    std::deque<Request> queue;
+    std::vector<Slot *> active;                  // admitted, frame accumulating or waiting to be taken
+    std::map<int, Slot *> by_id;
+    std::map<unsigned long long, Scene *> scenes;
+    std::vector<int> finished;                   // finished since the last ptgpu_server_finished call
+    ptgpu::DevBuf rays;                          // u64[2] query counters
+    std::vector<ptgpu_render_job> jobs;
+};
+
+namespace ptgpu {
+
+__global__ void k_default_seeds(unsigned *seeds, unsigned n) {
+    const unsigned g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const unsigned s = g > 2u ? g : 2u;
+    seeds[2 * g] = s;
+    seeds[2 * g + 1] = s;
+}
+
+}  // namespace ptgpu
+
+using namespace ptgpu;
+
+static void release_scene(Scene *sc) {
+    if (sc->bvh) ptgpu_clbvh_destroy(sc->bvh);
+    sc->bvh = nullptr;
+    sc->shapes.release();
+}
+
+// frees resident scenes nobody uses (least recently used first) until `need` more bytes fit
+static void evict_for(ptgpu_server *s, size_t need) {
+    while (s->used + need > s->budget) {
+        Scene *victim = nullptr;
+        for (auto &kv : s->scenes)
+            if (kv.second->refs == 0 && (!victim || kv.second->last_use < victim->last_use)) victim = kv.second;
+        if (!victim) return;
+        s->used -= victim->bytes;
+        release_scene(victim);
+        s->scenes.erase(victim->key);
+        delete victim;
+    }
+}
+
+static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
+    *fitted = false;
+    const ptgpu_request &r = q.r;
+    auto it = s->scenes.find(r.scene_key);
+    Scene *sc = it == s->scenes.end() ? nullptr : it->second;
+    PT_CHECK(!sc || (sc->light_cnt == r.lightCnt && sc->n == (int)r.shapeCnt), PTGPU_ERR_INVALID,
+             "request %d: scene key %llu is resident with other shapes or lightCnt", q.id, r.scene_key);
+    const size_t fb = frame_bytes(r.width, r.height);
+    const size_t sb = sc ? 0 : (size_t)r.shapeCnt * kSceneBytesPerPrim;
+    evict_for(s, fb + sb);
+    if (s->used + fb + sb > s->budget) return PTGPU_OK;   // wait for memory
+    hipStream_t st = s->ctx->stream;
+    if (!sc) {
+        sc = new Scene();
+        sc->key = r.scene_key;
+        sc->n = (int)r.shapeCnt;
+        sc->light_cnt = r.lightCnt;
+        int e = sc->shapes.ensure((size_t)r.shapeCnt * sizeof(Shape));
+        if (e == PTGPU_OK) {
+            hipError_t he = hipMemcpyAsync(sc->shapes.p, r.h_shapes, (size_t)r.shapeCnt * sizeof(Shape),
+                                           hipMemcpyHostToDevice, st);
+            if (he != hipSuccess) e = hip_fail(he, "hipMemcpyAsync", __FILE__, __LINE__);
+        }
+        if (e == PTGPU_OK) e = ptgpu_clbvh_create_device(s->ctx, sc->shapes.as<Shape>(), sc->n, &sc->bvh);
+        if (e == PTGPU_OK) e = ptgpu_clbvh_build_radix_tree(sc->bvh);
+        if (e == PTGPU_OK) e = ptgpu_clbvh_build_bvh_tree(sc->bvh);
+        size_t hb = 0;
+        if (e == PTGPU_OK) e = ptgpu_clbvh_device_bytes(sc->bvh, &hb);
+        if (e != PTGPU_OK) {
+            release_scene(sc);
+            delete sc;
+            return e;
+        }
+        sc->bytes = sc->shapes.bytes + hb;
+        s->used += sc->bytes;
+        s->scenes[sc->key] = sc;
+        s->builds++;
+    }
+    Slot *sl = new Slot();
+    sl->id = q.id;
+    sl->r = r;
+    sl->scene = sc;
+    int e = sl->frame.ensure(fb);
+    if (e != PTGPU_OK) {
+        delete sl;
+        return e;
+    }
+    sl->bytes = fb;
+    const size_t px = (size_t)r.width * (size_t)r.height;
+    char *base = static_cast<char *>(sl->frame.p);
+    const size_t oc = 0, os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
+    const size_t ocam = op + ((px * 4 + 255) & ~size_t(255));
+    (void)oc;
+    if (r.h_seeds) {
+        PT_HIP(hipMemcpyAsync(base + os, r.h_seeds, px * 8, hipMemcpyHostToDevice, st));
+    } else {
+        hipLaunchKernelGGL(k_default_seeds, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st,
+                           reinterpret_cast<unsigned *>(base + os), (unsigned)px);
+        PT_HIP(hipGetLastError());
+    }
+    PT_HIP(hipMemcpyAsync(base + ocam, &r.camera, sizeof(Camera), hipMemcpyHostToDevice, st));
+    PT_HIP(hipStreamSynchronize(st));   // the host request data may be released once admitted
+    sc->refs++;
+    sc->last_use = s->tick_no;
+    s->used += fb;
+    s->active.push_back(sl);
+    s->by_id[sl->id] = sl;
+    *fitted = true;
+    return PTGPU_OK;
+}
+
+extern "C" {
+
+int ptgpu_server_create(ptgpu_ctx *ctx, size_t hbm_budget, int spp_per_tick, ptgpu_server **out) {
+    PT_CHECK(ctx && out, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(spp_per_tick >= 1, PTGPU_ERR_INVALID, "spp_per_tick %d < 1", spp_per_tick);
+    PT_CHECK(hbm_budget >= (1u << 20), PTGPU_ERR_INVALID, "HBM budget below 1 MiB");
+    ptgpu_server *s = new ptgpu_server();
+    s->ctx = ctx;
+    s->budget = hbm_budget;
+    s->spp_per_tick = spp_per_tick;
+    int e = s->rays.ensure(64);
+    if (e == PTGPU_OK) e = fill_u32(ctx->stream, s->rays.p, 0u, 4);
+    if (e != PTGPU_OK) {
+        delete s;
+        return e;
+    }
+    *out = s;
+    return PTGPU_OK;
+}
+
+int ptgpu_server_submit(ptgpu_server *s, const ptgpu_request *r, int *id) {
+    PT_CHECK(s && r && id, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(r->h_shapes && r->shapeCnt >= 2, PTGPU_ERR_INVALID, "a request needs at least two shapes");
+    PT_CHECK(r->width > 0 && r->height > 0 && r->spp >= 1, PTGPU_ERR_INVALID, "bad frame %dx%d x %d spp", r->width,
+             r->height, r->spp);
+    PT_CHECK(r->max_depth >= 0 && r->max_depth < 1024, PTGPU_ERR_INVALID, "bad max_depth");
+    const size_t need = frame_bytes(r->width, r->height) + (size_t)r->shapeCnt * kSceneBytesPerPrim;
+    PT_CHECK(need <= s->budget, PTGPU_ERR_UNSUPPORTED, "request needs %zu bytes, above the %zu-byte budget", need,
+             s->budget);
+    Request q;
+    q.id = s->next_id++;
+    q.r = *r;
+    s->queue.push_back(q);
+    *id = q.id;
+    return PTGPU_OK;
+}
+
+int ptgpu_server_tick(ptgpu_server *s, int *n_active, int *n_queued) {
+    PT_CHECK(s, PTGPU_ERR_INVALID, "null server");
+    PT_HIP(hipSetDevice(s->ctx->device));
+    s->tick_no++;
+    // admission, head of line first
+    while (!s->queue.empty()) {
+        bool fitted = false;
+        PT_TRY(admit(s, s->queue.front(), &fitted));
+        if (!fitted) break;
+        s->queue.pop_front();
+    }
+    // one batched launch: every unfinished request advances by up to spp_per_tick samples
+    s->jobs.clear();
+    for (Slot *sl : s->active) {
+        if (sl->finished) continue;
+        const ptgpu_request &r = sl->r;
+        const size_t px = (size_t)r.width * (size_t)r.height;
+        char *base = static_cast<char *>(sl->frame.p);
+        const size_t os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
+        const size_t ocam = op + ((px * 4 + 255) & ~size_t(255));
+        sl->step = r.spp - sl->done < s->spp_per_tick ? r.spp - sl->done : s->spp_per_tick;
+        ptgpu_render_job j{};
+        j.bvh = sl->scene->bvh;
+        j.kd = nullptr;
+        j.args.colors = reinterpret_cast<Vec *>(base);
+        j.args.seeds = reinterpret_cast<unsigned *>(base + os);
+        j.args.pixels = reinterpret_cast<unsigned *>(base + op);
+        j.args.camera = reinterpret_cast<const Camera *>(base + ocam);
+        j.args.shapes = sl->scene->shapes.as<Shape>();
+        j.args.shapeCnt = (unsigned)sl->scene->n;
+        j.args.lightCnt = r.lightCnt;
+        j.args.width = r.width;
+        j.args.height = r.height;
+        j.args.currentSample = sl->done;
+        j.args.spp = sl->step;
+        j.args.max_depth = r.max_depth;
+        s->jobs.push_back(j);
+        sl->scene->last_use = s->tick_no;
+    }
+    if (!s->jobs.empty()) {
+        PT_TRY(ptgpu_render_batch(s->ctx, s->jobs.data(), (int)s->jobs.size(),
+                                  s->rays.as<unsigned long long>()));
+        PT_HIP(hipStreamSynchronize(s->ctx->stream));
+        for (Slot *sl : s->active) {
+            if (sl->finished) continue;
+            sl->done += sl->step;
+            sl->step = 0;
+            if (sl->done >= sl->r.spp) {
+                sl->finished = true;
+                s->finished.push_back(sl->id);
+                if (--sl->scene->refs < 0) sl->scene->refs = 0;
+            }
+        }
+    }
+    int act = 0;
+    for (Slot *sl : s->active) act += sl->finished ? 0 : 1;
+    if (n_active) *n_active = act;
+    if (n_queued) *n_queued = (int)s->queue.size();
+    return PTGPU_OK;
+}
+
+int ptgpu_server_finished(ptgpu_server *s, int *ids, int cap, int *n) {
+    PT_CHECK(s && n && (ids || cap == 0), PTGPU_ERR_INVALID, "null argument");
+    const int k = (int)s->finished.size() < cap ? (int)s->finished.size() : cap;
+    for (int i = 0; i < k; i++) ids[i] = s->finished[(size_t)i];
+    s->finished.erase(s->finished.begin(), s->finished.begin() + k);
+    *n = k;
+    return PTGPU_OK;
+}
+
+int ptgpu_server_result(ptgpu_server *s, int id, float *h_colors, unsigned int *h_pixels, unsigned int *h_seeds) {
+    PT_CHECK(s, PTGPU_ERR_INVALID, "null server");
+    auto it = s->by_id.find(id);
+    PT_CHECK(it != s->by_id.end(), PTGPU_ERR_INVALID, "request %d is not admitted (or already taken)", id);
+    Slot *sl = it->second;
+    PT_CHECK(sl->finished, PTGPU_ERR_INVALID, "request %d has not finished", id);
+    const size_t px = (size_t)sl->r.width * (size_t)sl->r.height;
+    const char *base = static_cast<const char *>(sl->frame.p);
+    const size_t os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
+    hipStream_t st = s->ctx->stream;
+    if (h_colors) PT_HIP(hipMemcpyAsync(h_colors, base, px * 12, hipMemcpyDeviceToHost, st));
+    if (h_seeds) PT_HIP(hipMemcpyAsync(h_seeds, base + os, px * 8, hipMemcpyDeviceToHost, st));
+    if (h_pixels) PT_HIP(hipMemcpyAsync(h_pixels, base + op, px * 4, hipMemcpyDeviceToHost, st));
+    PT_HIP(hipStreamSynchronize(st));
+    s->used -= sl->bytes;
+    sl->frame.release();
+    for (size_t i = 0; i < s->active.size(); i++)
+        if (s->active[i] == sl) {
+            s->active.erase(s->active.begin() + (long)i);
+            break;
+        }
+    s->by_id.erase(it);
+    delete sl;
+    return PTGPU_OK;
+}
+
+int ptgpu_server_stats(ptgpu_server *s, unsigned long long *rays2, size_t *hbm_used, int *scenes_resident,
+                       unsigned long long *builds) {
+    PT_CHECK(s, PTGPU_ERR_INVALID, "null server");
+    if (rays2) {
+        PT_HIP(hipMemcpyAsync(rays2, s->rays.p, 16, hipMemcpyDeviceToHost, s->ctx->stream));
+        PT_HIP(hipStreamSynchronize(s->ctx->stream));
+    }
+    if (hbm_used) *hbm_used = s->used;
+    if (scenes_resident) *scenes_resident = (int)s->scenes.size();
+    if (builds) *builds = s->builds;
+    return PTGPU_OK;
+}
+
+int ptgpu_server_destroy(ptgpu_server *s) {
+    if (!s) return PTGPU_OK;
+    if (s->ctx && s->ctx->stream) (void)hipStreamSynchronize(s->ctx->stream);
+    for (Slot *sl : s->active) {
+        sl->frame.release();
+        delete sl;
+    }
+    for (auto &kv : s->scenes) {
+        release_scene(kv.second);
+        delete kv.second;
+    }
+    s->rays.release();
+    delete s;
+    return PTGPU_OK;
+}
+
+}  // extern "C"

@@ -47,6 +47,12 @@ class RenderArgs(C.Structure):
                 ("max_depth", i32), ("pixels", vp), ("ray_counter", vp), ("trav_stats", vp)]
 
 
+class Request(C.Structure):
+    _fields_ = [("scene_key", C.c_ulonglong), ("h_shapes", vp), ("shapeCnt", u32), ("lightCnt", u32),
+                ("camera", C.c_ubyte * 76), ("width", i32), ("height", i32), ("spp", i32), ("max_depth", i32),
+                ("h_seeds", vp)]
+
+
 class RenderJob(C.Structure):
     _fields_ = [("bvh", vp), ("kd", vp), ("args", RenderArgs)]
 
@@ -100,6 +106,14 @@ _SIGS = {
     "ptgpu_radiance_gpu": [vp, i32, C.POINTER(RadianceArgs)],
     "ptgpu_render": [vp, vp, vp, C.POINTER(RenderArgs)],
     "ptgpu_render_batch": [vp, vp, i32, vp],
+    "ptgpu_server_create": [vp, C.c_size_t, i32, vp],
+    "ptgpu_server_submit": [vp, C.POINTER(Request), vp],
+    "ptgpu_server_tick": [vp, vp, vp],
+    "ptgpu_server_finished": [vp, vp, i32, vp],
+    "ptgpu_server_result": [vp, i32, vp, vp, vp],
+    "ptgpu_server_stats": [vp, vp, vp, vp, vp],
+    "ptgpu_server_destroy": [vp],
+    "ptgpu_clbvh_device_bytes": [vp, vp],
     "ptgpu_trace_closest": [vp, vp, vp, vp, u32, vp, i32, vp, vp],
     "ptgpu_trace_any": [vp, vp, vp, vp, u32, vp, vp, i32, vp],
     "ptgpu_update_camera": [vp],
@@ -448,6 +462,83 @@ def render(ctx: Context, bvh: CLBVH | None, kd: KDTree | None, *, colors: Device
                    spp, max_depth, pixels.ptr if pixels else None, ray_counter.ptr if ray_counter else None,
                    trav_stats.ptr if trav_stats else None)
     _check(ctx.lib.ptgpu_render(ctx.handle, _h(bvh), _h(kd), C.byref(a)))
+
+
+class Server:
+    """Continuous batching of render requests (ptgpu_server_*): submit frames as they arrive, call
+    tick() to admit and advance every admitted frame by spp_per_tick samples in one launch."""
+
+    def __init__(self, ctx: Context, hbm_budget: int, spp_per_tick: int = 1):
+        self.ctx = ctx
+        h = vp()
+        _check(ctx.lib.ptgpu_server_create(ctx.handle, int(hbm_budget), int(spp_per_tick), C.byref(h)))
+        self.handle = h.value
+        self._keep = {}     # host arrays a queued request points at
+        self._size = {}
+        ctx._children.add(self)
+
+    def submit(self, scene_key: int, shapes: np.ndarray, light_cnt: int, camera: np.ndarray, width: int, height: int,
+               spp: int, max_depth: int = 5, seeds: np.ndarray | None = None) -> int:
+        if shapes.dtype.itemsize != SHAPE_DTYPE.itemsize or camera.dtype.itemsize != CAMERA_DTYPE.itemsize:
+            raise ValueError("shapes / camera must use the reference layouts")
+        shapes = np.ascontiguousarray(shapes)
+        r = Request()
+        r.scene_key = int(scene_key)
+        r.h_shapes = _ptr(shapes)
+        r.shapeCnt = len(shapes)
+        r.lightCnt = int(light_cnt)
+        C.memmove(r.camera, np.ascontiguousarray(camera).ctypes.data, 76)
+        r.width, r.height, r.spp, r.max_depth = int(width), int(height), int(spp), int(max_depth)
+        if seeds is not None:
+            seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+            if seeds.size != 2 * width * height:
+                raise ValueError("seeds: 2 per pixel")
+        r.h_seeds = _ptr(seeds) if seeds is not None else None
+        rid = i32()
+        _check(self.ctx.lib.ptgpu_server_submit(self.handle, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = (shapes, seeds)
+        self._size[rid.value] = (int(width), int(height))
+        return rid.value
+
+    def tick(self) -> tuple:
+        a, q = i32(), i32()
+        _check(self.ctx.lib.ptgpu_server_tick(self.handle, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def finished(self) -> list:
+        out = []
+        ids = (i32 * 256)()
+        while True:
+            n = i32()
+            _check(self.ctx.lib.ptgpu_server_finished(self.handle, ids, 256, C.byref(n)))
+            out.extend(ids[:n.value])
+            if n.value < 256:
+                return out
+
+    def result(self, rid: int, colors: bool = True, pixels: bool = True, seeds: bool = False):
+        w, h = self._size[rid]
+        c = np.empty(w * h * 3, np.float32) if colors else None
+        p = np.empty(w * h, np.uint32) if pixels else None
+        s = np.empty(w * h * 2, np.uint32) if seeds else None
+        _check(self.ctx.lib.ptgpu_server_result(self.handle, rid, _ptr(c) if c is not None else None,
+                                                _ptr(p) if p is not None else None,
+                                                _ptr(s) if s is not None else None))
+        self._keep.pop(rid, None)
+        self._size.pop(rid, None)
+        return c, p, s
+
+    def stats(self) -> dict:
+        rays = (C.c_ulonglong * 2)()
+        used, scenes, builds = C.c_size_t(), i32(), C.c_ulonglong()
+        _check(self.ctx.lib.ptgpu_server_stats(self.handle, rays, C.byref(used), C.byref(scenes), C.byref(builds)))
+        return {"closest": rays[0], "anyhit": rays[1], "hbm_used": used.value, "scenes": scenes.value,
+                "builds": builds.value}
+
+    def close(self) -> None:
+        if self.handle and self.ctx.handle:
+            self.ctx.lib.ptgpu_server_destroy(self.handle)
+        self.handle = None
+        self._keep.clear()
 
 
 def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = None) -> None:
