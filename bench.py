@@ -209,6 +209,139 @@ def build_sweep(sizes, reps=5):
     ctx.close()
 
 
+def serve_trace(n_req: int, rate: float, pool: int, seed: int = 2024):
+    """The served workload: Poisson arrivals at `rate`/s; each request picks one of `pool` scenes
+    (triangle count log-uniform in [1e3, 2e6] and mesh type from the scene seed), a camera at a
+    random orbit angle, a resolution in {256^2, 512^2, 1024^2} and spp in {1, 4, 16}; depth 5."""
+    from ptgpu_pc_amd import scenes
+    rng = np.random.default_rng(seed)
+    pool_scenes = []
+    for k in range(pool):
+        ss = 1000 + k
+        n = int(np.exp(np.random.default_rng(ss).uniform(np.log(1e3), np.log(2e6))))
+        shapes, lc, _, kind = scenes.mixed_scene(ss, n, 64, 64)
+        pool_scenes.append((ss, shapes, lc, kind))
+    t, reqs = 0.0, []
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        sc = int(rng.integers(pool))
+        res = int(rng.choice([256, 512, 1024]))
+        spp = int(rng.choice([1, 4, 16]))
+        yaw = float(rng.uniform(0, 2 * np.pi))
+        reqs.append({"arrival": t, "scene": sc, "w": res, "h": res, "spp": spp, "yaw": yaw})
+    return pool_scenes, reqs
+
+
+def run_plain(ctx, pool_scenes, reqs):
+    """One scene per launch, in arrival order: upload + LBVH build + one fused spp render + pixel readback."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import default_seeds
+    d_rays = ctx.alloc(16, np.uint64).zero()
+    lat, pix_out = [], []
+    t0 = time.perf_counter()
+    for r in reqs:
+        now = time.perf_counter() - t0
+        if now < r["arrival"]:
+            time.sleep(r["arrival"] - now)
+        _, shapes, lc, _ = pool_scenes[r["scene"]]
+        w, h = r["w"], r["h"]
+        cam = scenes.orbit_camera(r["yaw"], w, h)
+        pt.update_camera(cam)
+        s = shapes.copy()
+        d_sh = ctx.to_device(s)
+        bvh = pt.CLBVH(ctx, d_sh, len(s))
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        d_col = ctx.alloc(w * h * 12, np.float32)
+        d_seed = ctx.to_device(default_seeds(w, h))
+        d_pix = ctx.alloc(w * h * 4, np.uint32)
+        pt.render(ctx, bvh, None, colors=d_col, seeds=d_seed, camera=ctx.to_device(cam), shapes=d_sh, shapeCnt=len(s),
+                  lightCnt=lc, width=w, height=h, currentSample=0, spp=r["spp"], max_depth=5, pixels=d_pix,
+                  ray_counter=d_rays)
+        px = d_pix.download(np.uint32)
+        lat.append(time.perf_counter() - t0 - r["arrival"])
+        pix_out.append(px)
+        bvh.close()
+        for a in (d_sh, d_col, d_seed, d_pix):
+            a.free()
+    ctx.synchronize()
+    end = time.perf_counter() - t0
+    rays = int(d_rays.download(np.uint64)[:2].sum())
+    d_rays.free()
+    return lat, end, rays, pix_out
+
+
+def run_served(ctx, pool_scenes, reqs, budget, spp_per_tick):
+    """Continuous batching: submit on arrival, tick, take finished frames (pixel readback)."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    srv = pt.Server(ctx, budget, spp_per_tick)
+    cams = []
+    for r in reqs:
+        cam = scenes.orbit_camera(r["yaw"], r["w"], r["h"])
+        pt.update_camera(cam)
+        cams.append(cam)
+    lat = [None] * len(reqs)
+    pix_out = [None] * len(reqs)
+    ids = {}
+    nxt, done = 0, 0
+    t0 = time.perf_counter()
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt]["arrival"] <= now:
+            r = reqs[nxt]
+            ss, shapes, lc, _ = pool_scenes[r["scene"]]
+            ids[srv.submit(ss, shapes, lc, cams[nxt], r["w"], r["h"], r["spp"], 5)] = nxt
+            nxt += 1
+        active, queued = srv.tick()
+        for rid in srv.finished():
+            k = ids[rid]
+            _, px, _ = srv.result(rid, colors=False, pixels=True)
+            lat[k] = time.perf_counter() - t0 - reqs[k]["arrival"]
+            pix_out[k] = px
+            done += 1
+        if active == 0 and queued == 0 and nxt < len(reqs):
+            wait = reqs[nxt]["arrival"] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+    end = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return lat, end, st["closest"] + st["anyhit"], pix_out, st
+
+
+def serve_compare(n_req, rate, pool, spp_per_tick, budget_gb, blocks_per_cu=None, regroup=None):
+    import ptgpu_pc_amd as pt
+    pool_scenes, reqs = serve_trace(n_req, rate, pool)
+    ctx = pt.Context(0)
+    if blocks_per_cu:
+        ctx.set_tuning("batch_blocks_per_cu", blocks_per_cu)
+    if regroup:
+        ctx.set_tuning("batch_regroup", regroup)
+    # warm the code paths once (first-launch costs are not part of either path)
+    warm = [dict(reqs[0], arrival=0.0)]
+    run_plain(ctx, pool_scenes, warm)
+    run_served(ctx, pool_scenes, warm, budget_gb << 30, spp_per_tick)
+    lat_p, end_p, rays_p, pix_p = run_plain(ctx, pool_scenes, reqs)
+    lat_s, end_s, rays_s, pix_s, st = run_served(ctx, pool_scenes, reqs, budget_gb << 30, spp_per_tick)
+    ctx.close()
+    first = reqs[0]["arrival"]
+    same = all(a.tobytes() == b.tobytes() for a, b in zip(pix_p, pix_s))
+
+    def summary(lat, end, rays):
+        lat = np.array(lat)
+        return {"frames_s": round(len(lat) / (end - first), 2), "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 1), "mrays_s": round(rays / (end - first) / 1e6, 1),
+                "makespan_s": round(end - first, 3)}
+    plain, served = summary(lat_p, end_p, rays_p), summary(lat_s, end_s, rays_s)
+    return {"requests": n_req, "rate_per_s": rate, "scene_pool": pool, "spp_per_tick": spp_per_tick,
+            "hbm_budget_gb": budget_gb, "plain": plain, "served": served,
+            "frames_speedup": round(served["frames_s"] / plain["frames_s"], 2), "pixels_identical": same,
+            "served_builds": st["builds"], "workload": "Poisson trace; scenes log-uniform [1e3, 2e6] triangles "
+            "(soup or displaced icosphere by seed), 256^2/512^2/1024^2, spp 1/4/16, depth 5"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,11 +354,21 @@ def main():
     ap.add_argument("--wide", type=int, default=None)
     ap.add_argument("--treelet", type=int, default=0, help="treelet size for CLBVH::optimize (0: no optimize)")
     ap.add_argument("--treelet-passes", type=int, default=1)
+    ap.add_argument("--serve", action="store_true", help="served vs plain GPU path on a Poisson request trace")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=500.0)
+    ap.add_argument("--pool", type=int, default=16)
+    ap.add_argument("--spp-per-tick", type=int, default=4)
+    ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--profile-summary", type=str, default=None,
                     help="directory of rocprofv3 results (kt/, fetch/, write/, sq*/, commands.json) to summarise")
     ap.add_argument("--build-sweep", type=str, default=None,
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
+    if args.serve:
+        print(json.dumps(serve_compare(args.requests, args.rate, args.pool, args.spp_per_tick, args.budget_gb,
+                                       args.blocks_per_cu, args.regroup)), flush=True)
+        return
     if args.profile_summary:
         profile_summary(Path(args.profile_summary), ROOT / "profiles" / "k_render_lanes_c2_soup65538_512x512x8spp.md")
         return

@@ -118,8 +118,9 @@ int ptgpu_ctx_set_max_depth(ptgpu_ctx *ctx, int max_depth);
  * nodes).  All give the same bits. */
 int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine);
 /* Launch tuning, no effect on results: "regroup" (1..64, ready lanes that start a shading pass in
- * engine 3, default 8), "blocks_per_cu" (1..64, persistent workgroups per CU, default 2), "wide"
- * (0/1, engine 3 walks four-wide nodes, default 1). */
+ * engine 3, default 16), "blocks_per_cu" (1..64, persistent workgroups per CU, default 2), "wide"
+ * (0/1, engine 3 walks four-wide nodes, default 1); "batch_regroup" and "batch_blocks_per_cu" (the
+ * same for ptgpu_render_batch, defaults 32 and 3). */
 int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value);
 int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);
 int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr);

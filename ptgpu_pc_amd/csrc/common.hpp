@@ -117,6 +117,20 @@ struct DevBuf {
 
 }  // namespace ptgpu
 
+namespace ptgpu {
+// One stream's worth of batched-render launches: the device job table, its pinned staging copy (reused
+// once the previous copy is done), and the launch's work counter.
+struct BatchChannel {
+    hipStream_t stream = nullptr;
+    DevBuf table, counter;
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    hipEvent_t copied = nullptr;   // the last table upload from `pinned` has finished
+    hipEvent_t ready = nullptr;    // scene preparation on the context's stream, waited on by `stream`
+    void release();
+};
+}  // namespace ptgpu
+
 struct ptgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -125,17 +139,15 @@ struct ptgpu_ctx {
     int cu_count = 0;
     int max_depth = 6;   // MAX_DEPTH (geom.h:32) for ptgpu_radiance_gpu
     int engine = 0;      // 0 auto, 1 megakernel (lockstep queries), 2 wavefront, 3 megakernel (per-lane states)
-    int regroup = 8;    // per-lane-state megakernel: ready lanes that trigger a shading pass
-    int blocks_per_cu = 2;
+    int regroup = 16;   // per-lane-state megakernel: ready lanes that trigger a shading pass
+    int batch_regroup = 32;
+    int blocks_per_cu = 2;        // single-frame renders: few waves shorten each pixel's chain
+    int batch_blocks_per_cu = 3;  // batched renders: throughput-bound, more waves per SIMD
     int wide = 1;        // per-lane-state megakernel: walk the four-wide nodes when they exist
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
-    // batch launches: device job table, pinned staging copy, and the event that fences its reuse
-    ptgpu::DevBuf tmp_jobs;
-    void *job_pinned = nullptr;
-    size_t job_pinned_bytes = 0;
-    hipEvent_t job_event = nullptr;
+    ptgpu::BatchChannel batch;    // ptgpu_render_batch launches (on the context's stream)
 };
 
 namespace ptgpu {
@@ -147,6 +159,9 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
 int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_prims);
 // four-wide nodes for a prepared binary structure (no-op without internal nodes); sets acc->nodes4
 int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4);
+// batched render on a channel: jobs resolved on the context's stream, launched on the channel's
+int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs, int njobs,
+                 unsigned long long *d_ray_counter);
 // light list cached in a KD handle (as the LBVH handle caches its own)
 int kdtree_lights(ptgpu_kdtree *kd, const Shape *d_shapes, int n, int light_cnt, const int **lights, int *nlights);
 // reference-layout trees -> traversal layout

@@ -42,6 +42,17 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+void BatchChannel::release() {
+    table.release();
+    counter.release();
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    pinned_bytes = 0;
+    if (copied) (void)hipEventDestroy(copied);
+    if (ready) (void)hipEventDestroy(ready);
+    copied = ready = nullptr;
+}
+
 }  // namespace ptgpu
 
 using namespace ptgpu;
@@ -115,9 +126,7 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     ctx->tmp_lights.release();
     ctx->tmp_scan.release();
     ctx->tmp_counter.release();
-    ctx->tmp_jobs.release();
-    if (ctx->job_pinned) (void)hipHostFree(ctx->job_pinned);
-    if (ctx->job_event) (void)hipEventDestroy(ctx->job_event);
+    ctx->batch.release();
     wf_release(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -148,6 +157,12 @@ int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value) {
     } else if (!strcmp(name, "blocks_per_cu")) {
         PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "blocks_per_cu %d not in [1, 64]", value);
         ctx->blocks_per_cu = value;
+    } else if (!strcmp(name, "batch_blocks_per_cu")) {
+        PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "batch_blocks_per_cu %d not in [1, 64]", value);
+        ctx->batch_blocks_per_cu = value;
+    } else if (!strcmp(name, "batch_regroup")) {
+        PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "batch_regroup %d not in [1, 64]", value);
+        ctx->batch_regroup = value;
     } else if (!strcmp(name, "wide")) {
         PT_CHECK(value == 0 || value == 1, PTGPU_ERR_INVALID, "wide %d not in {0, 1}", value);
         ctx->wide = value;

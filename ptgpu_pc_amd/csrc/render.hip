@@ -827,6 +827,7 @@ int convert_kd_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU
 static int stack_for_wide(const DevAccel &acc) {
     const int need = 3 * ((acc.depth + 2) / 2) + 1;
     if (need <= 32) return 32;
+    if (need <= 48) return 48;   // 48 KB of LDS per workgroup: three workgroups per CU
     if (need <= 64) return 64;
     return -1;
 }
@@ -886,9 +887,11 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
         if (sw > 0) {
             if (L.trav_stats) {
                 if (sw == 32) PT_LANES(32, true, true);
+                else if (sw == 48) PT_LANES(48, true, true);
                 else PT_LANES(64, true, true);
             } else {
                 if (sw == 32) PT_LANES(32, false, true);
+                else if (sw == 48) PT_LANES(48, false, true);
                 else PT_LANES(64, false, true);
             }
         } else if (L.trav_stats) {
@@ -1059,6 +1062,15 @@ int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, 
     PT_CHECK(ctx && (jobs || njobs == 0) && njobs >= 0, PTGPU_ERR_INVALID, "null argument");
     if (njobs == 0) return PTGPU_OK;
     PT_HIP(hipSetDevice(ctx->device));
+    ctx->batch.stream = ctx->stream;
+    return launch_batch(ctx, ctx->batch, jobs, njobs, d_ray_counter);
+}
+
+}  // extern "C"
+
+namespace ptgpu {
+int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs, int njobs,
+                 unsigned long long *d_ray_counter) {
     std::vector<RenderJob> table((size_t)njobs);
     std::vector<unsigned> begin((size_t)njobs + 1, 0u);
     int st = 0;
@@ -1109,49 +1121,60 @@ int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, 
         PT_CHECK(total < (1ull << 31), PTGPU_ERR_UNSUPPORTED, "batch of more than 2^31 pixels");
     }
     begin[(size_t)njobs] = (unsigned)total;
-    // stage the table through pinned memory; the previous batch's copy must be done before reuse
+    hipStream_t stream = ch.stream;
+    if (!ch.copied) PT_HIP(hipEventCreateWithFlags(&ch.copied, hipEventDisableTiming));
+    if (!ch.ready) PT_HIP(hipEventCreateWithFlags(&ch.ready, hipEventDisableTiming));
+    if (stream != ctx->stream) {   // builds and layout conversion ran on the context's stream
+        PT_HIP(hipEventRecord(ch.ready, ctx->stream));
+        PT_HIP(hipStreamWaitEvent(stream, ch.ready, 0));
+    }
+    // stage the table through pinned memory; the channel's previous copy must be done before reuse
     const size_t tb = table.size() * sizeof(RenderJob), bb = begin.size() * sizeof(unsigned);
     const size_t bytes = tb + bb + 64;
-    if (!ctx->job_event) PT_HIP(hipEventCreateWithFlags(&ctx->job_event, hipEventDisableTiming));
-    PT_HIP(hipEventSynchronize(ctx->job_event));
-    if (ctx->job_pinned_bytes < bytes) {
-        if (ctx->job_pinned) PT_HIP(hipHostFree(ctx->job_pinned));
-        ctx->job_pinned = nullptr;
-        ctx->job_pinned_bytes = 0;
-        PT_HIP(hipHostMalloc(&ctx->job_pinned, bytes * 2, hipHostMallocDefault));
-        ctx->job_pinned_bytes = bytes * 2;
+    PT_HIP(hipEventSynchronize(ch.copied));
+    if (ch.pinned_bytes < bytes) {
+        if (ch.pinned) PT_HIP(hipHostFree(ch.pinned));
+        ch.pinned = nullptr;
+        ch.pinned_bytes = 0;
+        PT_HIP(hipHostMalloc(&ch.pinned, bytes * 2, hipHostMallocDefault));
+        ch.pinned_bytes = bytes * 2;
     }
-    char *hp = static_cast<char *>(ctx->job_pinned);
+    char *hp = static_cast<char *>(ch.pinned);
     std::memcpy(hp, table.data(), tb);
     const size_t boff = (tb + 63) & ~size_t(63);
     std::memcpy(hp + boff, begin.data(), bb);
-    PT_TRY(ctx->tmp_jobs.ensure(boff + bb));
-    PT_HIP(hipMemcpyAsync(ctx->tmp_jobs.p, hp, boff + bb, hipMemcpyHostToDevice, ctx->stream));
-    PT_HIP(hipEventRecord(ctx->job_event, ctx->stream));
-    PT_TRY(ctx->tmp_counter.ensure(64));
-    PT_TRY(fill_u32(ctx->stream, ctx->tmp_counter.p, 0u, 1));
+    PT_TRY(ch.table.ensure(boff + bb));
+    PT_HIP(hipMemcpyAsync(ch.table.p, hp, boff + bb, hipMemcpyHostToDevice, stream));
+    PT_HIP(hipEventRecord(ch.copied, stream));
+    PT_TRY(ch.counter.ensure(64));
+    PT_TRY(fill_u32(stream, ch.counter.p, 0u, 1));
     RenderParams P;
     std::memset(static_cast<void *>(&P), 0, sizeof(P));
     P.acc = table[0].acc;   // unused by the batch variant
     P.ray_counter = d_ray_counter;
     P.trav_stats = nullptr;
-    P.work_counter = ctx->tmp_counter.as<unsigned>();
+    P.work_counter = ch.counter.as<unsigned>();
     P.total = (int)total;
-    P.regroup = ctx->regroup;
-    P.jobs = ctx->tmp_jobs.as<RenderJob>();
-    P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(ctx->tmp_jobs.p) + boff);
+    P.regroup = ctx->batch_regroup;
+    P.jobs = ch.table.as<RenderJob>();
+    P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(ch.table.p) + boff);
     P.njobs = njobs;
-    long long blocks = (long long)ctx->cu_count * ctx->blocks_per_cu;
+    long long blocks = (long long)ctx->cu_count * ctx->batch_blocks_per_cu;
     const long long need = ((long long)total + 255) / 256;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
     if (st == 32) hipLaunchKernelGGL((k_render_lanes<32, false, true, true>), dim3((unsigned)blocks), dim3(256), 0,
-                                     ctx->stream, P);
-    else hipLaunchKernelGGL((k_render_lanes<64, false, true, true>), dim3((unsigned)blocks), dim3(256), 0,
-                            ctx->stream, P);
+                                     stream, P);
+    else if (st == 48) hipLaunchKernelGGL((k_render_lanes<48, false, true, true>), dim3((unsigned)blocks), dim3(256),
+                                          0, stream, P);
+    else hipLaunchKernelGGL((k_render_lanes<64, false, true, true>), dim3((unsigned)blocks), dim3(256), 0, stream,
+                            P);
     PT_HIP(hipGetLastError());
     return PTGPU_OK;
 }
+}  // namespace ptgpu
+
+extern "C" {
 
 int ptgpu_trace_closest(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes,
                         unsigned int shapeCnt, const Ray *d_rays, int n, float *d_t, int *d_id) {

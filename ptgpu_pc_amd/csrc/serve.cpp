@@ -8,11 +8,15 @@
 //   * admitted requests: their frame buffers (colors, seeds, pixels, camera) stay resident while the
 //     frame accumulates over ticks and until the caller takes the result;
 //   * the queue, in arrival order.
-// One tick admits queued requests while their footprint fits the HBM budget (head of line first),
-// builds the scenes they bring in, then renders min(spp_per_tick, remaining) samples of every admitted
-// request in one ptgpu_render_batch launch.  Splitting a frame's samples over launches gives the same
-// bits as one launch (the running mean and the RNG state carry over in colors/seeds), so every
-// request's result equals rendering it alone.
+// One tick retires the batches that have finished, admits queued requests while their footprint fits
+// the HBM budget (head of line first), builds the scenes they bring in, and launches one batched render
+// in which every admitted request that is not already in flight advances by min(spp_per_tick,
+// remaining) samples.  Two batches may be in flight on two streams, so a new batch fills the GPU while
+// the previous one drains; a request is in at most one of them.  Splitting a frame's samples over
+// launches gives the same bits as one launch (the running mean and the RNG state carry over in
+// colors/seeds), so every request's result equals rendering it alone.  Frame buffers of taken results
+// go to a pool for later requests (freeing device memory would wait for the batches in flight).
+#include <algorithm>
 #include <cstddef>
 #include <cstring>
 #include <deque>
@@ -55,8 +59,11 @@ struct Slot {
     size_t bytes = 0;
     int done = 0;             // samples accumulated
     int step = 0;             // samples in the launch in flight
+    bool in_flight = false;
     bool finished = false;
 };
+
+constexpr int kChannels = 2;
 
 size_t frame_bytes(int w, int h) {
     const size_t px = (size_t)w * (size_t)h;
@@ -79,6 +86,13 @@ struct ptgpu_server {
     std::vector<int> finished;                   // finished since the last ptgpu_server_finished call
     ptgpu::DevBuf rays;                          // u64[2] query counters
     std::vector<ptgpu_render_job> jobs;
+    std::vector<Slot *> job_slots;
+    ptgpu::BatchChannel ch[kChannels];
+    hipEvent_t done[kChannels] = {};
+    std::vector<Slot *> flight[kChannels];
+    unsigned long long launched_at[kChannels] = {};   // launch sequence number, 0 = idle
+    unsigned long long launches = 0;
+    std::vector<ptgpu::DevBuf> pool;              // frame buffers of taken results
 };
 
 namespace ptgpu {
@@ -103,6 +117,11 @@ static void release_scene(Scene *sc) {
 
 // frees resident scenes nobody uses (least recently used first) until `need` more bytes fit
 static void evict_for(ptgpu_server *s, size_t need) {
+    while (s->used + need > s->budget && !s->pool.empty()) {   // idle frame buffers first
+        s->used -= s->pool.back().bytes;
+        s->pool.back().release();
+        s->pool.pop_back();
+    }
     while (s->used + need > s->budget) {
         Scene *victim = nullptr;
         for (auto &kv : s->scenes)
@@ -115,6 +134,23 @@ static void evict_for(ptgpu_server *s, size_t need) {
     }
 }
 
+// a pooled frame buffer of at least `bytes` (the smallest such), moved into *out
+static bool take_pooled(ptgpu_server *s, size_t bytes, ptgpu::DevBuf *out) {
+    int best = -1;
+    for (size_t i = 0; i < s->pool.size(); i++)
+        if (s->pool[i].bytes >= bytes && (best < 0 || s->pool[i].bytes < s->pool[(size_t)best].bytes)) best = (int)i;
+    if (best < 0) return false;
+    *out = s->pool[(size_t)best];
+    s->pool.erase(s->pool.begin() + best);
+    return true;
+}
+
+static bool pool_has(const ptgpu_server *s, size_t bytes) {
+    for (const auto &b : s->pool)
+        if (b.bytes >= bytes) return true;
+    return false;
+}
+
 static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
     *fitted = false;
     const ptgpu_request &r = q.r;
@@ -124,8 +160,9 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
              "request %d: scene key %llu is resident with other shapes or lightCnt", q.id, r.scene_key);
     const size_t fb = frame_bytes(r.width, r.height);
     const size_t sb = sc ? 0 : (size_t)r.shapeCnt * kSceneBytesPerPrim;
-    evict_for(s, fb + sb);
-    if (s->used + fb + sb > s->budget) return PTGPU_OK;   // wait for memory
+    const size_t fresh = pool_has(s, fb) ? 0 : fb;   // a pooled frame buffer needs no new memory
+    evict_for(s, fresh + sb);
+    if (s->used + fresh + sb > s->budget) return PTGPU_OK;   // wait for memory
     hipStream_t st = s->ctx->stream;
     if (!sc) {
         sc = new Scene();
@@ -157,12 +194,15 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
     sl->id = q.id;
     sl->r = r;
     sl->scene = sc;
-    int e = sl->frame.ensure(fb);
-    if (e != PTGPU_OK) {
-        delete sl;
-        return e;
+    if (!take_pooled(s, fb, &sl->frame)) {
+        int e = sl->frame.ensure(fb);
+        if (e != PTGPU_OK) {
+            delete sl;
+            return e;
+        }
+        s->used += sl->frame.bytes;
     }
-    sl->bytes = fb;
+    sl->bytes = sl->frame.bytes;
     const size_t px = (size_t)r.width * (size_t)r.height;
     char *base = static_cast<char *>(sl->frame.p);
     const size_t oc = 0, os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
@@ -175,11 +215,9 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
                            reinterpret_cast<unsigned *>(base + os), (unsigned)px);
         PT_HIP(hipGetLastError());
     }
-    PT_HIP(hipMemcpyAsync(base + ocam, &r.camera, sizeof(Camera), hipMemcpyHostToDevice, st));
-    PT_HIP(hipStreamSynchronize(st));   // the host request data may be released once admitted
+    PT_HIP(hipMemcpyAsync(base + ocam, &sl->r.camera, sizeof(Camera), hipMemcpyHostToDevice, st));
     sc->refs++;
     sc->last_use = s->tick_no;
-    s->used += fb;
     s->active.push_back(sl);
     s->by_id[sl->id] = sl;
     *fitted = true;
@@ -223,59 +261,118 @@ int ptgpu_server_submit(ptgpu_server *s, const ptgpu_request *r, int *id) {
     return PTGPU_OK;
 }
 
+static void retire(ptgpu_server *s, int c) {
+    for (Slot *sl : s->flight[c]) {
+        sl->done += sl->step;
+        sl->step = 0;
+        sl->in_flight = false;
+        if (sl->done >= sl->r.spp) {
+            sl->finished = true;
+            s->finished.push_back(sl->id);
+            if (--sl->scene->refs < 0) sl->scene->refs = 0;
+        }
+    }
+    s->flight[c].clear();
+    s->launched_at[c] = 0;
+}
+
+// waits for the oldest batch in flight and retires it; false if none is in flight
+static int retire_oldest(ptgpu_server *s, bool *any) {
+    int c = -1;
+    for (int k = 0; k < kChannels; k++)
+        if (s->launched_at[k] && (c < 0 || s->launched_at[k] < s->launched_at[c])) c = k;
+    *any = c >= 0;
+    if (c < 0) return PTGPU_OK;
+    PT_HIP(hipEventSynchronize(s->done[c]));
+    retire(s, c);
+    return PTGPU_OK;
+}
+
 int ptgpu_server_tick(ptgpu_server *s, int *n_active, int *n_queued) {
     PT_CHECK(s, PTGPU_ERR_INVALID, "null server");
     PT_HIP(hipSetDevice(s->ctx->device));
     s->tick_no++;
-    // admission, head of line first
+    // 1. retire the batches that are done
+    for (int c = 0; c < kChannels; c++) {
+        if (!s->launched_at[c]) continue;
+        const hipError_t q = hipEventQuery(s->done[c]);
+        if (q == hipSuccess) retire(s, c);
+        else if (q != hipErrorNotReady) return hip_fail(q, "hipEventQuery", __FILE__, __LINE__);
+    }
+    // 2. admission, head of line first; one wait at the end releases the admitted requests' host data
+    bool admitted = false;
     while (!s->queue.empty()) {
         bool fitted = false;
         PT_TRY(admit(s, s->queue.front(), &fitted));
         if (!fitted) break;
+        admitted = true;
         s->queue.pop_front();
     }
-    // one batched launch: every unfinished request advances by up to spp_per_tick samples
+    if (admitted) PT_HIP(hipStreamSynchronize(s->ctx->stream));
+    // 3. one batch of every admitted request not in flight, on a free channel
+    int c = -1;
+    for (int k = 0; k < kChannels; k++)
+        if (!s->launched_at[k]) {
+            c = k;
+            break;
+        }
     s->jobs.clear();
-    for (Slot *sl : s->active) {
-        if (sl->finished) continue;
-        const ptgpu_request &r = sl->r;
-        const size_t px = (size_t)r.width * (size_t)r.height;
-        char *base = static_cast<char *>(sl->frame.p);
-        const size_t os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
-        const size_t ocam = op + ((px * 4 + 255) & ~size_t(255));
-        sl->step = r.spp - sl->done < s->spp_per_tick ? r.spp - sl->done : s->spp_per_tick;
-        ptgpu_render_job j{};
-        j.bvh = sl->scene->bvh;
-        j.kd = nullptr;
-        j.args.colors = reinterpret_cast<Vec *>(base);
-        j.args.seeds = reinterpret_cast<unsigned *>(base + os);
-        j.args.pixels = reinterpret_cast<unsigned *>(base + op);
-        j.args.camera = reinterpret_cast<const Camera *>(base + ocam);
-        j.args.shapes = sl->scene->shapes.as<Shape>();
-        j.args.shapeCnt = (unsigned)sl->scene->n;
-        j.args.lightCnt = r.lightCnt;
-        j.args.width = r.width;
-        j.args.height = r.height;
-        j.args.currentSample = sl->done;
-        j.args.spp = sl->step;
-        j.args.max_depth = r.max_depth;
-        s->jobs.push_back(j);
-        sl->scene->last_use = s->tick_no;
+    s->job_slots.clear();
+    if (c >= 0) {
+        for (Slot *sl : s->active) {
+            if (sl->finished || sl->in_flight) continue;
+            const ptgpu_request &r = sl->r;
+            const size_t px = (size_t)r.width * (size_t)r.height;
+            char *base = static_cast<char *>(sl->frame.p);
+            const size_t os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
+            const size_t ocam = op + ((px * 4 + 255) & ~size_t(255));
+            sl->step = r.spp - sl->done < s->spp_per_tick ? r.spp - sl->done : s->spp_per_tick;
+            ptgpu_render_job j{};
+            j.bvh = sl->scene->bvh;
+            j.kd = nullptr;
+            j.args.colors = reinterpret_cast<Vec *>(base);
+            j.args.seeds = reinterpret_cast<unsigned *>(base + os);
+            j.args.pixels = reinterpret_cast<unsigned *>(base + op);
+            j.args.camera = reinterpret_cast<const Camera *>(base + ocam);
+            j.args.shapes = sl->scene->shapes.as<Shape>();
+            j.args.shapeCnt = (unsigned)sl->scene->n;
+            j.args.lightCnt = r.lightCnt;
+            j.args.width = r.width;
+            j.args.height = r.height;
+            j.args.currentSample = sl->done;
+            j.args.spp = sl->step;
+            j.args.max_depth = r.max_depth;
+            s->jobs.push_back(j);
+            s->job_slots.push_back(sl);
+            sl->scene->last_use = s->tick_no;
+        }
     }
     if (!s->jobs.empty()) {
-        PT_TRY(ptgpu_render_batch(s->ctx, s->jobs.data(), (int)s->jobs.size(),
-                                  s->rays.as<unsigned long long>()));
-        PT_HIP(hipStreamSynchronize(s->ctx->stream));
-        for (Slot *sl : s->active) {
-            if (sl->finished) continue;
-            sl->done += sl->step;
-            sl->step = 0;
-            if (sl->done >= sl->r.spp) {
-                sl->finished = true;
-                s->finished.push_back(sl->id);
-                if (--sl->scene->refs < 0) sl->scene->refs = 0;
-            }
+        // lanes take pixels in job order: the longest per-pixel chains (most samples this launch, then the
+        // largest frames) go first, so the launch does not end waiting on a long chain started late
+        std::vector<size_t> order(s->jobs.size());
+        for (size_t i = 0; i < order.size(); i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            const ptgpu_render_args &x = s->jobs[a].args, &y = s->jobs[b].args;
+            if (x.spp != y.spp) return x.spp > y.spp;
+            return (long long)x.width * x.height > (long long)y.width * y.height;
+        });
+        std::vector<ptgpu_render_job> sorted;
+        sorted.reserve(order.size());
+        for (size_t i : order) {
+            sorted.push_back(s->jobs[i]);
+            s->flight[c].push_back(s->job_slots[i]);
+            s->job_slots[i]->in_flight = true;
         }
+        if (!s->ch[c].stream) PT_HIP(hipStreamCreateWithFlags(&s->ch[c].stream, hipStreamNonBlocking));
+        if (!s->done[c]) PT_HIP(hipEventCreateWithFlags(&s->done[c], hipEventDisableTiming));
+        PT_TRY(launch_batch(s->ctx, s->ch[c], sorted.data(), (int)sorted.size(), s->rays.as<unsigned long long>()));
+        PT_HIP(hipEventRecord(s->done[c], s->ch[c].stream));
+        s->launched_at[c] = ++s->launches;
+    } else {
+        // 4. nothing new to launch: wait for the oldest batch so the caller's loop makes progress
+        bool any = false;
+        PT_TRY(retire_oldest(s, &any));
     }
     int act = 0;
     for (Slot *sl : s->active) act += sl->finished ? 0 : 1;
@@ -307,8 +404,8 @@ int ptgpu_server_result(ptgpu_server *s, int id, float *h_colors, unsigned int *
     if (h_seeds) PT_HIP(hipMemcpyAsync(h_seeds, base + os, px * 8, hipMemcpyDeviceToHost, st));
     if (h_pixels) PT_HIP(hipMemcpyAsync(h_pixels, base + op, px * 4, hipMemcpyDeviceToHost, st));
     PT_HIP(hipStreamSynchronize(st));
-    s->used -= sl->bytes;
-    sl->frame.release();
+    s->pool.push_back(sl->frame);   // still counted in `used`; reused or freed by admission
+    sl->frame = ptgpu::DevBuf();
     for (size_t i = 0; i < s->active.size(); i++)
         if (s->active[i] == sl) {
             s->active.erase(s->active.begin() + (long)i);
@@ -338,6 +435,13 @@ int ptgpu_server_destroy(ptgpu_server *s) {
     for (Slot *sl : s->active) {
         sl->frame.release();
         delete sl;
+    }
+    for (auto &b : s->pool) b.release();
+    for (int c = 0; c < kChannels; c++) {
+        if (s->ch[c].stream) (void)hipStreamSynchronize(s->ch[c].stream);
+        s->ch[c].release();
+        if (s->ch[c].stream) (void)hipStreamDestroy(s->ch[c].stream);
+        if (s->done[c]) (void)hipEventDestroy(s->done[c]);
     }
     for (auto &kv : s->scenes) {
         release_scene(kv.second);

@@ -197,6 +197,12 @@ def icosphere_in_box(level: int = 7, seed: int = 3, width: int = 1024, height: i
     return shapes, 2, cam
 
 
+def orbit_camera(yaw: float, width: int, height: int) -> np.ndarray:
+    """Camera of the batched / served workloads: on a radius-2 orbit around the unit cube's centre."""
+    orig = (0.5 + 2.0 * math.sin(yaw), 0.5, 0.5 + 2.0 * math.cos(yaw))
+    return make_camera(orig, (0.5, 0.5, 0.5), yaw, 0.0, width, height)
+
+
 def mixed_scene(seed: int, n: int, width: int = 256, height: int = 256):
     """One scene of the batched / served workloads: n triangles of soup or displaced icosphere
     (truncated to n faces) in the unit cube, an emissive quad (stored last), camera on an orbit."""
@@ -217,6 +223,4 @@ def mixed_scene(seed: int, n: int, width: int = 256, height: int = 256):
                      [(0.25, 1.25, 0.25), (0.75, 1.25, 0.75), (0.25, 1.25, 0.75)]], dtype=F32)
     light = triangles_to_shapes(quad, e=(12.0, 12.0, 12.0), c=(0.0, 0.0, 0.0))
     shapes = concat_shapes([body, light])
-    orig = (0.5 + 2.0 * math.sin(yaw), 0.5, 0.5 + 2.0 * math.cos(yaw))
-    cam = make_camera(orig, (0.5, 0.5, 0.5), yaw, 0.0, width, height)
-    return shapes, 2, cam, kind
+    return shapes, 2, orbit_camera(yaw, width, height), kind

@@ -84,14 +84,14 @@ def test_batch_equals_each_job_alone(ctx):
     assert all(a[0].mean() > 0 for a in alone)
     # a sparse grid (several pixels per lane, lanes crossing jobs) and extreme shading thresholds
     for regroup, bpc in ((1, 1), (64, 1)):
-        ctx.set_tuning("regroup", regroup)
-        ctx.set_tuning("blocks_per_cu", bpc)
+        ctx.set_tuning("batch_regroup", regroup)
+        ctx.set_tuning("batch_blocks_per_cu", bpc)
         again = _run(ctx, jobs, batch=True)
         for a, b in zip(alone, again):
             for x, y in zip(a, b):
                 assert x.tobytes() == y.tobytes()
-    ctx.set_tuning("regroup", 8)
-    ctx.set_tuning("blocks_per_cu", 2)
+    ctx.set_tuning("batch_regroup", 32)
+    ctx.set_tuning("batch_blocks_per_cu", 3)
     for _, bvh, kd, _ in cache.values():
         (bvh if bvh is not None else kd).close()
 

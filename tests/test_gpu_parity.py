@@ -334,7 +334,7 @@ def test_wavefront_equals_megakernel(ctx, scene):
         ctx.set_tuning("wide", wide)
         out[f"lanes{regroup}/{bpc}/{wide}"] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd,
                                                        d_shapes=d_sh)[:3]
-    ctx.set_tuning("regroup", 8)
+    ctx.set_tuning("regroup", 16)
     ctx.set_tuning("blocks_per_cu", 2)
     ctx.set_tuning("wide", 1)
     ctx.set_engine("auto")
