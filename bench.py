@@ -4,14 +4,19 @@ Workload (one step): the 65,536-triangle soup + emissive quad (65,538 shapes, se
 LBVH from device-resident shapes (bounds + Morton, onesweep sort, Karras topology, refit) and
 rendered at 512x512, 8 spp, MAX_DEPTH 5 (6 path segments) with BVH closest-hit and any-hit shadow
 queries.  `value` is Mrays/s = (closest-hit + any-hit queries) / wall time of the timed steps,
-build included.  Prints one JSON line.
+build included.  The line also carries `served`: a Poisson request trace served with continuous
+batching against the plain one-scene-per-launch path (frames/s, p50/p99 latency, Mrays/s).
+Prints one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-serve]
+  python bench.py --serve            # only the served comparison
+  python bench.py --build-sweep 16,20,23
 """
 from __future__ import annotations
 
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -75,6 +80,8 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
 
 
 RENDER_KERNEL = "k_render_lanes"
+# the timed instance: single frame, four-wide nodes, no statistics (<stack, STATS, WIDE, MULTI>)
+RENDER_INSTANCE = "k_render_lanes<%, false, true, false>"
 WORKLOAD = ("LBVH build + BVH traversal: 65,536-triangle soup (seed 1) + emissive quad, 512x512, 8 spp, depth 5, "
             "30-bit Morton, one leaf per primitive")
 TRAFFIC_FILE = ROOT / "profiles" / "k_render_lanes_c2_traffic.json"
@@ -95,7 +102,7 @@ def _counter(db, name, kernel_like):
     con = sqlite3.connect(str(db))
     try:
         r = con.execute("select avg(value), count(*) from counters_collection where counter_name = ? and kernel_name "
-                        "like ? and kernel_name not like '%, true, %'", (name, f"%{kernel_like}%")).fetchone()
+                        "like ?", (name, f"%{kernel_like}%")).fetchone()
     finally:
         con.close()
     return r
@@ -109,8 +116,8 @@ def profile_summary(d: Path, out_md: Path) -> None:
     rows = _db_rows(kt, "select name, total_calls, average, percentage from top_kernels order by total_duration desc "
                         "limit 12")
     fetch_db, write_db = next(d.glob("fetch/**/*results.db")), next(d.glob("write/**/*results.db"))
-    f_k, f_n = _counter(fetch_db, "FETCH_SIZE", RENDER_KERNEL)
-    w_k, w_n = _counter(write_db, "WRITE_SIZE", RENDER_KERNEL)
+    f_k, f_n = _counter(fetch_db, "FETCH_SIZE", RENDER_INSTANCE)
+    w_k, w_n = _counter(write_db, "WRITE_SIZE", RENDER_INSTANCE)
     f_c = _counter(fetch_db, "FETCH_SIZE", "k_copy_calib")[0]
     w_c = _counter(write_db, "WRITE_SIZE", "k_copy_calib")[0]
     f_scale = CALIB_BYTES / (f_c * 1024.0)     # bytes per reported KiB, from the known copy
@@ -119,10 +126,10 @@ def profile_summary(d: Path, out_md: Path) -> None:
     sq = {}
     for dbp in d.glob("sq*/**/*results.db"):
         for name, val in _db_rows(dbp, "select counter_name, avg(value) from counters_collection where kernel_name "
-                                       f"like '%{RENDER_KERNEL}%' and kernel_name not like '%, true, %' group by "
-                                       "counter_name"):
+                                       f"like '%{RENDER_INSTANCE}%' group by counter_name"):
             sq[name] = val
-    render_avg = next((r[2] / 1e3 for r in rows if RENDER_KERNEL in r[0] and ", true," not in r[0]), None)
+    timed = re.compile(r"k_render_lanes<\d+, false, true, false>")
+    render_avg = next((r[2] / 1e3 for r in rows if timed.search(r[0])), None)
     cmds = json.loads((d / "commands.json").read_text())
     lines = [f"# {RENDER_KERNEL} — C2 (65,538-shape soup, 512x512, 8 spp, depth 5), MI355X", "",
              "Commands (run from the repository root; rocprofv3 output directories as named):", ""]
@@ -354,11 +361,12 @@ def main():
     ap.add_argument("--wide", type=int, default=None)
     ap.add_argument("--treelet", type=int, default=0, help="treelet size for CLBVH::optimize (0: no optimize)")
     ap.add_argument("--treelet-passes", type=int, default=1)
-    ap.add_argument("--serve", action="store_true", help="served vs plain GPU path on a Poisson request trace")
+    ap.add_argument("--serve", action="store_true", help="only the served vs plain comparison (a JSON line)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served comparison in the default line")
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--rate", type=float, default=500.0)
     ap.add_argument("--pool", type=int, default=16)
-    ap.add_argument("--spp-per-tick", type=int, default=4)
+    ap.add_argument("--spp-per-tick", type=int, default=16)
     ap.add_argument("--budget-gb", type=int, default=64)
     ap.add_argument("--profile-summary", type=str, default=None,
                     help="directory of rocprofv3 results (kt/, fetch/, write/, sq*/, commands.json) to summarise")
@@ -464,6 +472,11 @@ def main():
                 "per_launch": {"box_tests": box_tests, "prim_tests": prim_tests, "alg_bytes": alg_bytes,
                                "avg_ms": round(render_avg, 4)}}
 
+    served = None
+    if not args.no_serve:
+        served = serve_compare(args.requests, args.rate, args.pool, args.spp_per_tick, args.budget_gb,
+                               args.blocks_per_cu, None)
+
     cpu = None
     if not args.no_cpu:
         btn, btl = bvh.getTrees()
@@ -494,6 +507,7 @@ def main():
         "build_stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "msamples_s": round(W * H * SPP * args.steps / wall / 1e6, 2),
         "roofline": roofline,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
