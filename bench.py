@@ -5,11 +5,14 @@ LBVH from device-resident shapes (bounds + Morton, onesweep sort, Karras topolog
 rendered at 512x512, 8 spp, MAX_DEPTH 5 (6 path segments) with BVH closest-hit and any-hit shadow
 queries.  `value` is Mrays/s = (closest-hit + any-hit queries) / wall time of the timed steps,
 build included.  The line also carries `served`: a Poisson request trace served with continuous
-batching against the plain one-scene-per-launch path (frames/s, p50/p99 latency, Mrays/s).
+batching against the plain one-scene-per-launch path (frames/s, p50/p99 latency, Mrays/s), and
+`batched`: 64 scenes built in one batched launch sequence and rendered in one launch against the
+plain per-scene build + render.
 Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-serve]
   python bench.py --serve            # only the served comparison
+  python bench.py --batch            # only the batched (64-scene) comparison
   python bench.py --build-sweep 16,20,23
 """
 from __future__ import annotations
@@ -177,6 +180,7 @@ def measured_traffic():
 # algorithmic bytes per primitive of each build stage (each logical load/store counted once at its
 # natural width, atomics 8 B): Morton 36+36+4, 8-bit onesweep 4 + 16*4, topology 28, refit 156
 STAGE_BYTES = {"morton": 76, "sort": 68, "topology": 28, "refit": 156}
+BUILD_BYTES_PER_PRIM = sum(STAGE_BYTES.values())   # 328 B per primitive with 8-bit digits
 
 
 def build_sweep(sizes, reps=5):
@@ -349,6 +353,125 @@ def serve_compare(n_req, rate, pool, spp_per_tick, budget_gb, blocks_per_cu=None
             "(soup or displaced icosphere by seed), 256^2/512^2/1024^2, spp 1/4/16, depth 5"}
 
 
+def batch_workload(n_scenes: int = 64, seed: int = 4000):
+    """The many-scene workload: independent scenes with triangle counts log-uniform in [1e3, 1e6],
+    soup or displaced icosphere by scene seed, a camera at a random orbit angle, 256^2 / 512^2 /
+    1024^2 by seed; rendered at 4 spp, depth 5."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    out = []
+    for k in range(n_scenes):
+        ss = seed + k
+        r = np.random.default_rng([ss, 1])
+        n = int(np.exp(r.uniform(np.log(1e3), np.log(1e6))))
+        res = int(r.choice([256, 512, 1024]))
+        shapes, lc, cam, kind = scenes.mixed_scene(ss, n, res, res)
+        pt.update_camera(cam)
+        out.append({"shapes": shapes, "lc": lc, "cam": cam, "w": res, "h": res, "kind": kind})
+    return out
+
+
+def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64):
+    """Plain path (per scene: LBVH build, then one fused 4-spp render launch) against the batched
+    path (every scene's build in one batched sequence of launches, every frame in one render
+    launch) on the same resident scenes.  Both rebuild every tree and re-render every frame per step."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd.layout import default_seeds
+    W = batch_workload(n_scenes)
+    ctx = pt.Context(0)
+    total_n = sum(len(w["shapes"]) for w in W)
+    d_sh = [ctx.to_device(w["shapes"]) for w in W]
+    d_cam = [ctx.to_device(w["cam"]) for w in W]
+    seeds0 = [ctx.to_device(default_seeds(w["w"], w["h"])) for w in W]
+    frames = [{"colors": ctx.alloc(w["w"] * w["h"] * 12, np.float32), "seeds": ctx.alloc(w["w"] * w["h"] * 8, np.uint32),
+               "pixels": ctx.alloc(w["w"] * w["h"] * 4, np.uint32)} for w in W]
+    d_rays = ctx.alloc(16, np.uint64).zero()
+    plain_h = [pt.CLBVH(ctx, d, len(w["shapes"])) for d, w in zip(d_sh, W)]
+    batch_h = pt.CLBVH.create_batch(ctx, d_sh)
+
+    def job(k, h):
+        w = W[k]
+        return dict(bvh=h, colors=frames[k]["colors"], seeds=frames[k]["seeds"], pixels=frames[k]["pixels"],
+                    camera=d_cam[k], shapes=d_sh[k], shapeCnt=len(w["shapes"]), lightCnt=w["lc"], width=w["w"],
+                    height=w["h"], currentSample=0, spp=4, max_depth=5)
+
+    def reseed():
+        for k in range(len(W)):
+            frames[k]["seeds"].copy_from(seeds0[k])
+
+    def plain_step(counter=None):
+        for k, h in enumerate(plain_h):
+            h.reset()
+            h.buildRadixTree()
+            h.buildBVHTree()
+            pt.render(ctx, h, None, ray_counter=counter, **{a: b for a, b in job(k, h).items() if a != "bvh"})
+
+    def batch_step(counter=None):
+        ctx.record(0)
+        pt.CLBVH.build_batch(ctx, batch_h)
+        ctx.record(1)
+        pt.render_batch(ctx, [job(k, h) for k, h in enumerate(batch_h)], ray_counter=counter)
+        ctx.record(2)
+
+    # same frames from both paths (from the same seeds)
+    reseed()
+    plain_step()
+    ctx.synchronize()
+    pix_plain = [f["pixels"].download(np.uint32) for f in frames]
+    reseed()
+    batch_step()
+    ctx.synchronize()
+    same = all(a.tobytes() == f["pixels"].download(np.uint32).tobytes() for a, f in zip(pix_plain, frames))
+
+    def timed(step_fn, events=False):
+        for _ in range(warmup):
+            step_fn()
+        ctx.synchronize()
+        d_rays.zero()
+        ctx.synchronize()
+        b_ms, r_ms = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step_fn(d_rays)
+            if events:
+                b_ms.append(ctx.elapsed_ms(0, 1))
+                r_ms.append(ctx.elapsed_ms(1, 2))
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        rays = int(d_rays.download(np.uint64)[:2].sum())
+        return wall / steps, rays / steps, (float(np.mean(b_ms)) if b_ms else None), (float(np.mean(r_ms)) if r_ms else None)
+
+    t_plain, rays_plain, _, _ = timed(plain_step)
+    t_batch, rays_batch, b_ms, r_ms = timed(batch_step, events=True)
+    # the plain path's builds alone
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for h in plain_h:
+        h.reset()
+        h.buildRadixTree()
+        h.buildBVHTree()
+    ctx.synchronize()
+    plain_build_ms = (time.perf_counter() - t0) * 1e3
+    for h in plain_h + batch_h:
+        h.close()
+    ctx.close()
+    alg = BUILD_BYTES_PER_PRIM * total_n
+    nfr = len(W)
+    return {"scenes": nfr, "shapes": total_n, "soup_scenes": sum(1 for w in W if w["kind"] == 0),
+            "pixels": sum(w["w"] * w["h"] for w in W), "spp": 4, "max_depth": 5,
+            "plain": {"frames_s": round(nfr / t_plain, 2), "ms_per_step": round(t_plain * 1e3, 2),
+                      "mrays_s": round(rays_plain / t_plain / 1e6, 1), "build_ms": round(plain_build_ms, 2)},
+            "batched": {"frames_s": round(nfr / t_batch, 2), "ms_per_step": round(t_batch * 1e3, 2),
+                        "mrays_s": round(rays_batch / t_batch / 1e6, 1), "build_ms": round(b_ms, 2),
+                        "render_ms": round(r_ms, 2), "build_mtri_s": round(total_n / b_ms / 1e3, 1),
+                        "build_alg_gbs": round(alg / (b_ms * 1e-3) / 1e9, 1)},
+            "frames_speedup": round(t_plain / t_batch, 2), "build_speedup": round(plain_build_ms / b_ms, 2),
+            "pixels_identical": same,
+            "workload": f"{nfr} scenes, triangles log-uniform [1e3, 1e6] (soup or displaced icosphere by seed), "
+                        "orbit cameras, 256^2/512^2/1024^2 by seed, 4 spp, depth 5; every tree rebuilt and every "
+                        "frame rendered per step"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -363,6 +486,9 @@ def main():
     ap.add_argument("--treelet-passes", type=int, default=1)
     ap.add_argument("--serve", action="store_true", help="only the served vs plain comparison (a JSON line)")
     ap.add_argument("--no-serve", action="store_true", help="skip the served comparison in the default line")
+    ap.add_argument("--batch", action="store_true", help="only the 64-scene batched vs plain comparison (a JSON line)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison in the default line")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--rate", type=float, default=500.0)
     ap.add_argument("--pool", type=int, default=16)
@@ -373,6 +499,9 @@ def main():
     ap.add_argument("--build-sweep", type=str, default=None,
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
+    if args.batch:
+        print(json.dumps(batch_compare(n_scenes=args.batch_scenes)), flush=True)
+        return
     if args.serve:
         print(json.dumps(serve_compare(args.requests, args.rate, args.pool, args.spp_per_tick, args.budget_gb,
                                        args.blocks_per_cu, args.regroup)), flush=True)
@@ -477,6 +606,10 @@ def main():
         served = serve_compare(args.requests, args.rate, args.pool, args.spp_per_tick, args.budget_gb,
                                args.blocks_per_cu, None)
 
+    batched = None
+    if not args.no_batch:
+        batched = batch_compare(n_scenes=args.batch_scenes)
+
     cpu = None
     if not args.no_cpu:
         btn, btl = bvh.getTrees()
@@ -508,6 +641,7 @@ def main():
         "msamples_s": round(W * H * SPP * args.steps / wall / 1e6, 2),
         "roofline": roofline,
         "served": served,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -157,6 +157,18 @@ int ptgpu_clbvh_build_radix_tree(ptgpu_clbvh *h);
 /* CLBVH::buildBVHTree (CLBVH.cpp:245-311; kernel BVH.cl:214-258): stable descending sort by Morton
  * key, leaf fill and bottom-up bound refit. */
 int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h);
+/* Batched builds (no single reference counterpart: the reference builds one CLBVH at a time,
+ * smallptGPU.cpp:1462-1470).  For every handle, the same work and results as
+ * ptgpu_clbvh_reset(h, n) + ptgpu_clbvh_build_radix_tree + ptgpu_clbvh_build_bvh_tree on the
+ * handle's device shapes, with each stage run as one launch over all scenes; bit-identical to
+ * building each scene alone.  Handles must be distinct, belong to ctx and share one sort digit
+ * width.  Returns after the builds have finished (one synchronisation for the batch). */
+int ptgpu_clbvh_build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *handles, int count);
+/* ptgpu_clbvh_create_device for `count` scenes followed by ptgpu_clbvh_build_batch: out[s] is a
+ * handle over d_shapes[s] (counts[s] shapes, caller-owned, updated in place) with its trees built.
+ * On failure every out[s] is NULL. */
+int ptgpu_clbvh_create_batch(ptgpu_ctx *ctx, Shape *const *d_shapes, const int *counts, int count,
+                             ptgpu_clbvh **out);
 /* CLBVH::optimize (CLBVH.cpp:318-355; kernel BVH.cl:514-543): treelet restructuring with up to
  * `treelet_size` (2..7) leaves per treelet, SAH constants Ci=1.2, Ct=1.0 (clheader.h:14-15). */
 int ptgpu_clbvh_optimize(ptgpu_clbvh *h, int treelet_size);
@@ -165,9 +177,9 @@ int ptgpu_clbvh_optimize(ptgpu_clbvh *h, int treelet_size);
 int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl);
 /* Zero-copy access to the device trees (N-1 internal nodes, N leaves). */
 int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl);
-/* Device pointer to sorted_geometry_indices (CLBVH.cpp:257), N ints. */
-/* device memory held by the handle (its own shapes copy included), in bytes */
+/* Device memory held by the handle (its own shapes copy included), in bytes. */
 int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes);
+/* Device pointer to sorted_geometry_indices (CLBVH.cpp:257), N ints. */
 int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted);
 /* Tree used by the traversal layout (ptgpu_render / ptgpu_trace_*) after the next build_bvh_tree:
  * 0 = the exported index-keyed Karras tree (BVH.cl:127-134), 1 = the Morton-keyed LBVH over the same

@@ -131,6 +131,17 @@ struct BatchChannel {
 };
 }  // namespace ptgpu
 
+namespace ptgpu {
+// Tables of a batched LBVH build (per-scene buffers, block offsets, clears) and their pinned
+// upload copy; reused by the next batch once the previous one has been waited for.
+struct BuildStaging {
+    DevBuf table;
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    void release();
+};
+}  // namespace ptgpu
+
 struct ptgpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -147,7 +158,8 @@ struct ptgpu_ctx {
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
-    ptgpu::BatchChannel batch;    // ptgpu_render_batch launches (on the context's stream)
+    ptgpu::BatchChannel batch;
+    ptgpu::BuildStaging build_staging;    // ptgpu_render_batch launches (on the context's stream)
 };
 
 namespace ptgpu {

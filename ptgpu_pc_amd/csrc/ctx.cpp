@@ -127,6 +127,7 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     ctx->tmp_scan.release();
     ctx->tmp_counter.release();
     ctx->batch.release();
+    ctx->build_staging.release();
     wf_release(ctx);
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);

@@ -1,6 +1,7 @@
 // lbvh.hip -- LBVH build for gfx950: scene bounds + 30-bit Morton codes, onesweep LSD radix sort
 // (stable, descending by Morton code), Karras radix-tree topology over leaf indices, bottom-up
-// bound refit with atomic visit flags, and the C ABI that replaces class CLBVH (CLBVH.h:39-47).
+// bound refit, the Morton-keyed traversal tree, and the C ABI that replaces class CLBVH
+// (CLBVH.h:39-47).  The stage bodies live in lbvh_device.hpp; the kernels here run one scene.
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -8,6 +9,7 @@
 
 #include "device_util.hpp"
 #include "lbvh.hpp"
+#include "lbvh_device.hpp"
 
 namespace ptgpu {
 
@@ -37,221 +39,23 @@ int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words) {
     return PTGPU_OK;
 }
 
-__device__ __forceinline__ float shape_f(const Shape *s, int word) {
-    return reinterpret_cast<const float *>(s)[word];
-}
-__device__ __forceinline__ int shape_i(const Shape *s, int word) {
-    return reinterpret_cast<const int *>(s)[word];
-}
-
-// CLBVH::getBound CLBVH.cpp:49-75 from the Shape words (type @0, p1..p3 / rad,p @1..9).
-__device__ __forceinline__ Bound shape_bound(const Shape *s, int *type_out) {
-    const float *w = reinterpret_cast<const float *>(s);
-    int type = reinterpret_cast<const int *>(s)[0];
-    *type_out = type;
-    Bound b;
-    if (type == SPHERE) {
-        float rad = w[1], px = w[2], py = w[3], pz = w[4];
-        b.min_x = px - rad; b.max_x = px + rad;
-        b.min_y = py - rad; b.max_y = py + rad;
-        b.min_z = pz - rad; b.max_z = pz + rad;
-    } else {
-        float x1 = w[1], y1 = w[2], z1 = w[3], x2 = w[4], y2 = w[5], z2 = w[6], x3 = w[7], y3 = w[8], z3 = w[9];
-        b.min_x = sel_min(sel_min(x1, x2), x3); b.max_x = sel_max(sel_max(x1, x2), x3);
-        b.min_y = sel_min(sel_min(y1, y2), y3); b.max_y = sel_max(sel_max(y1, y2), y3);
-        b.min_z = sel_min(sel_min(z1, z2), z3); b.max_z = sel_max(sel_max(z1, z2), z3);
-    }
-    return b;
-}
-
-// -------------------------------------------------------------------------------------------
-// Stage 1: AABBs, scene centroid bounds, Morton codes (CLBVH.cpp:90-164)
-// -------------------------------------------------------------------------------------------
-constexpr int kBoundsThreads = 256;
-
-__device__ __forceinline__ void reduce6(float *v, float *sh /*6*kBoundsThreads*/) {
-    const int tid = threadIdx.x;
-    for (int k = 0; k < 6; k++) sh[k * kBoundsThreads + tid] = v[k];
-    __syncthreads();
-    for (int s = kBoundsThreads / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            for (int k = 0; k < 6; k++) {
-                float a = sh[k * kBoundsThreads + tid], b = sh[k * kBoundsThreads + tid + s];
-                sh[k * kBoundsThreads + tid] = (k & 1) ? sel_max(a, b) : sel_min(a, b);
-            }
-        }
-        __syncthreads();
-    }
-    for (int k = 0; k < 6; k++) v[k] = sh[k * kBoundsThreads];
-}
-
-// Per shape: bound -> Shape.b, index -> Shape.index, centroid; block partial of the centroid
-// min/max (max starts at FLT_MIN as CLBVH.cpp:92-96 does); the last block to finish folds the
-// partials into scene6 = {min_x, max_x, min_y, max_y, min_z, max_z}.
 __global__ __launch_bounds__(kBoundsThreads) void k_bounds(Shape *shapes, int n, float *partials,
                                                            unsigned *done_counter, float *scene6) {
-    __shared__ float sh[6 * kBoundsThreads];
-    __shared__ bool s_last;
-    float acc[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        Shape *s = shapes + i;
-        int type;
-        Bound b = shape_bound(s, &type);
-        int *wi = reinterpret_cast<int *>(s);
-        float *wf = reinterpret_cast<float *>(s);
-        wi[17] = i;                                    // index @68
-        wf[19] = b.min_x; wf[20] = b.max_x; wf[21] = b.min_y;   // b @76
-        wf[22] = b.max_y; wf[23] = b.min_z; wf[24] = b.max_z;
-        float px, py, pz;
-        if (type == SPHERE) {
-            px = wf[2]; py = wf[3]; pz = wf[4];
-        } else {
-            px = (b.min_x + b.max_x) / 2.0f;
-            py = (b.min_y + b.max_y) / 2.0f;
-            pz = (b.min_z + b.max_z) / 2.0f;
-        }
-        if (px < acc[0]) acc[0] = px;
-        if (px > acc[1]) acc[1] = px;
-        if (py < acc[2]) acc[2] = py;
-        if (py > acc[3]) acc[3] = py;
-        if (pz < acc[4]) acc[4] = pz;
-        if (pz > acc[5]) acc[5] = pz;
-    }
-    reduce6(acc, sh);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) partials[blockIdx.x * 6 + k] = acc[k];
-        unsigned prev = add_acq_rel(done_counter, 1u);
-        s_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    float fin[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        for (int k = 0; k < 6; k++) {
-            float v = __hip_atomic_load(partials + b * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            fin[k] = (k & 1) ? sel_max(fin[k], v) : sel_min(fin[k], v);
-        }
-    }
-    reduce6(fin, sh);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) scene6[k] = fin[k];
-        *done_counter = 0;   // ready for the next build
-    }
+    bounds_body(shapes, n, partials, done_counter, scene6, blockIdx.x, gridDim.x);
 }
 
-__device__ __forceinline__ unsigned f2u_code(float v) { return v >= 0.f ? (unsigned)v : 0u; }
-
-__device__ __forceinline__ unsigned expand10(unsigned x) {
-    x &= 0x3FFu;
-    x = (x | (x << 16)) & 0x030000FFu;
-    x = (x | (x << 8)) & 0x0300F00Fu;
-    x = (x | (x << 4)) & 0x030C30C3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-
-// CLBVH.cpp:125-164: quantise each centroid to 10 bits per axis (1024 wraps to 0 through the
-// low-10-bit interleave) and interleave x above y above z.  Also emits the sort key ~code (30 bits)
-// so that an ascending stable sort yields the reference's stable descending order.
 __global__ __launch_bounds__(256) void k_morton(Shape *shapes, int n, const float *scene6, unsigned *keys) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float mnx = scene6[0], mxx = scene6[1], mny = scene6[2], mxy = scene6[3], mnz = scene6[4],
-                mxz = scene6[5];
-    Shape *s = shapes + i;
-    float *wf = reinterpret_cast<float *>(s);
-    int type = reinterpret_cast<int *>(s)[0];
-    float px, py, pz;
-    if (type == SPHERE) {
-        px = wf[2]; py = wf[3]; pz = wf[4];
-    } else {
-        px = (wf[19] + wf[20]) / 2.0f;
-        py = (wf[21] + wf[22]) / 2.0f;
-        pz = (wf[23] + wf[24]) / 2.0f;
-    }
-    unsigned a = f2u_code(((px - mnx) / (mxx - mnx)) * 1024.f);
-    unsigned b = f2u_code(((py - mny) / (mxy - mny)) * 1024.f);
-    unsigned c = f2u_code(((pz - mnz) / (mxz - mnz)) * 1024.f);
-    unsigned code = (expand10(a) << 2) | (expand10(b) << 1) | expand10(c);
-    reinterpret_cast<unsigned *>(s)[18] = code;   // morton_code @72
-    keys[i] = (~code) & 0x3FFFFFFFu;
+    morton_body(shapes, n, scene6, keys, blockIdx.x);
 }
-
-// -------------------------------------------------------------------------------------------
-// Stage 2: onesweep LSD radix sort (Adinets & Merrill 2022) for wave64.
-//   k_sort_hist:   one read of the keys builds the digit histograms of every pass.
-//   k_sort_scan:   exclusive scan of each pass histogram (global digit bases).
-//   k_sort_pass:   per 4096-key tile: wave-level match ranking in input order (stable), tile
-//                  histogram, decoupled look-back over earlier tiles per digit, LDS shuffle into
-//                  digit order, coalesced scatter.  Tile ids come from an atomic counter so every
-//                  tile a block waits on has already started.
-// -------------------------------------------------------------------------------------------
-constexpr int kSortThreads = 256;
-constexpr int kSortWaves = kSortThreads / kWave;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortThreads * kSortItems;
-constexpr unsigned kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kValMask = (1u << 30) - 1;
-constexpr unsigned kSpinLimit = 1u << 26;
-
-template <int BITS> struct SortCfg {
-    static constexpr int kRadix = 1 << BITS;
-    static constexpr unsigned kMask = (unsigned)kRadix - 1;
-    static constexpr int kPasses = (30 + BITS - 1) / BITS;
-};
 
 template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned *keys, int n, unsigned *hist) {
-    using C = SortCfg<BITS>;
-    __shared__ unsigned sh[C::kPasses * C::kRadix];
-    for (int j = threadIdx.x; j < C::kPasses * C::kRadix; j += kSortThreads) sh[j] = 0;
-    __syncthreads();
-    for (int i = blockIdx.x * kSortThreads + threadIdx.x; i < n; i += gridDim.x * kSortThreads) {
-        unsigned k = keys[i];
-#pragma unroll
-        for (int p = 0; p < C::kPasses; p++) atomicAdd(&sh[p * C::kRadix + ((k >> (p * BITS)) & C::kMask)], 1u);
-    }
-    __syncthreads();
-    for (int j = threadIdx.x; j < C::kPasses * C::kRadix; j += kSortThreads)
-        if (sh[j]) atomicAdd(&hist[j], sh[j]);
-}
-
-// exclusive scan of `v` across the 256 threads of a block; returns the exclusive prefix
-__device__ __forceinline__ unsigned block_exscan256(unsigned v, unsigned *s_w, unsigned *total) {
-    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    unsigned x = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        unsigned y = __shfl_up(x, o, kWave);
-        if (lane >= o) x += y;
-    }
-    if (lane == kWave - 1) s_w[w] = x;
-    __syncthreads();
-    unsigned pre = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < kSortWaves; k++) {
-        unsigned t = s_w[k];
-        if (k < w) pre += t;
-        tot += t;
-    }
-    __syncthreads();
-    *total = tot;
-    return pre + x - v;
+    sort_hist_body<BITS>(keys, n, hist, blockIdx.x, gridDim.x);
 }
 
 template <int BITS>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scan(unsigned *hist) {
-    using C = SortCfg<BITS>;
-    __shared__ unsigned s_w[kSortWaves];
-    for (int p = 0; p < C::kPasses; p++) {
-        for (int base = 0; base < C::kRadix; base += kSortThreads) {
-            int d = base + threadIdx.x;
-            unsigned v = d < C::kRadix ? hist[p * C::kRadix + d] : 0u;
-            unsigned tot;
-            unsigned ex = block_exscan256(v, s_w, &tot);
-            if (d < C::kRadix) hist[p * C::kRadix + d] = ex;   // kRadix <= 256: one round
-        }
-    }
+    sort_scan_body<BITS>(hist);
 }
 
 template <int BITS>
@@ -262,463 +66,41 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const unsigned *__re
                                                             const unsigned *__restrict__ digit_base,
                                                             unsigned *status, unsigned *tile_counter,
                                                             unsigned *err) {
-    using C = SortCfg<BITS>;
-    __shared__ unsigned s_keys[kSortTile];
-    __shared__ unsigned s_vals[kSortTile];
-    __shared__ unsigned s_wcnt[kSortWaves][C::kRadix];
-    __shared__ unsigned s_lofs[C::kRadix];
-    __shared__ unsigned s_gofs[C::kRadix];
-    __shared__ unsigned s_w[kSortWaves];
     __shared__ unsigned s_tile;
-
-    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
-    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-    for (int j = tid; j < kSortWaves * C::kRadix; j += kSortThreads) (&s_wcnt[0][0])[j] = 0;
+    if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
     __syncthreads();
-    const unsigned tile = s_tile;
-    const long long base = (long long)tile * kSortTile;
-    const long long wbase = base + (long long)w * kWave * kSortItems;
-
-    unsigned k[kSortItems], v[kSortItems], rank[kSortItems];
-#pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
-        long long idx = wbase + j * kWave + lane;
-        bool valid = idx < n;
-        k[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
-        v[j] = valid ? (vals_in ? vals_in[idx] : (unsigned)idx) : 0u;
-    }
-    const unsigned long long lt = lanemask_lt();
-#pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
-        long long idx = wbase + j * kWave + lane;
-        bool valid = idx < n;
-        unsigned digit = (k[j] >> shift) & C::kMask;
-        unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < BITS; b++) {
-            bool bit = (digit >> b) & 1u;
-            unsigned long long m = __ballot(bit);
-            peers &= bit ? m : ~m;
-        }
-        unsigned pre = 0;
-        if (valid) pre = s_wcnt[w][digit];
-        __builtin_amdgcn_wave_barrier();
-        if (valid) {
-            rank[j] = pre + (unsigned)__popcll(peers & lt);
-            if ((peers & lt) == 0) s_wcnt[w][digit] = pre + (unsigned)__popcll(peers);
-        }
-        __builtin_amdgcn_wave_barrier();
-    }
-    __syncthreads();
-
-    // per digit: wave-exclusive offsets, tile count, tile-local offsets, global base (look-back)
-    for (int d0 = 0; d0 < C::kRadix; d0 += kSortThreads) {
-        const int d = d0 + tid;
-        unsigned cnt = 0;
-        if (d < C::kRadix) {
-            unsigned run = 0;
-#pragma unroll
-            for (int q = 0; q < kSortWaves; q++) {
-                unsigned c = s_wcnt[q][d];
-                s_wcnt[q][d] = run;
-                run += c;
-            }
-            cnt = run;
-        }
-        unsigned tot;
-        unsigned lofs = block_exscan256(cnt, s_w, &tot);
-        if (d < C::kRadix) {
-            s_lofs[d] = lofs;
-            unsigned *my = status + (size_t)tile * C::kRadix + d;
-            unsigned prefix = 0;
-            if (tile == 0) {
-                st_relaxed(my, kFlagPrefix | cnt);
-            } else {
-                st_relaxed(my, kFlagAgg | cnt);
-                long long t = (long long)tile - 1;
-                unsigned spins = 0;
-                while (t >= 0) {
-                    unsigned s = ld_relaxed(status + (size_t)t * C::kRadix + d);
-                    unsigned flag = s & ~kValMask;
-                    if (flag == 0) {
-                        if (++spins > kSpinLimit) { atomicOr(err, 1u); break; }
-                        continue;
-                    }
-                    prefix += s & kValMask;
-                    if (flag == kFlagPrefix) break;
-                    t--;
-                }
-                st_relaxed(my, kFlagPrefix | ((prefix + cnt) & kValMask));
-            }
-            s_gofs[d] = digit_base[d] + prefix;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kSortItems; j++) {
-        long long idx = wbase + j * kWave + lane;
-        if (idx < n) {
-            unsigned digit = (k[j] >> shift) & C::kMask;
-            unsigned lpos = s_lofs[digit] + s_wcnt[w][digit] + rank[j];
-            s_keys[lpos] = k[j];
-            s_vals[lpos] = v[j];
-        }
-    }
-    __syncthreads();
-    long long rem = (long long)n - base;
-    const int count = rem < kSortTile ? (int)rem : kSortTile;
-    for (int i = tid; i < count; i += kSortThreads) {
-        unsigned key = s_keys[i];
-        unsigned digit = (key >> shift) & C::kMask;
-        unsigned dest = s_gofs[digit] + (unsigned)i - s_lofs[digit];
-        keys_out[dest] = key;
-        vals_out[dest] = s_vals[i];
-    }
-}
-
-// -------------------------------------------------------------------------------------------
-// Stage 3: Karras radix tree over leaf indices (BVH.cl:127-208).  The prefix length of two leaves
-// is clz(i ^ j); the tree is therefore the binary radix tree of 0..N-1.
-// -------------------------------------------------------------------------------------------
-__device__ __forceinline__ int delta_idx(int i, int j, int n) {
-    return (j >= 0 && j < n) ? __clz(i ^ j) : -1;
+    sort_pass_body<BITS>(keys_in, vals_in, keys_out, vals_out, n, shift, digit_base, status,
+                         __builtin_amdgcn_readfirstlane(s_tile), err);
 }
 
 __global__ __launch_bounds__(256) void k_karras(int n, BVHNodeGPU *btn, BVHNodeGPU *btl) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    const int d = (delta_idx(i, i + 1, n) - delta_idx(i, i - 1, n)) > 0 ? 1 : -1;
-    const int dmin = delta_idx(i, i - d, n);
-    int lmax = 2;
-    while (delta_idx(i, i + lmax * d, n) > dmin) lmax <<= 1;
-    int l = 0;
-    for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta_idx(i, i + (l + t) * d, n) > dmin) l += t;
-    const int j = i + l * d;
-    const int dnode = delta_idx(i, j, n);
-    int s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta_idx(i, i + (s + t) * d, n) > dnode) s += t;
-    } while (t > 1);
-    const int gamma = i + s * d + (d < 0 ? d : 0);
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    BVHNodeGPU *cur = btn + i;
-    int leafflag = 0;
-    cur->nLeft = gamma;
-    if (lo == gamma) { leafflag = 2; btl[gamma].nParent = i; }
-    else btn[gamma].nParent = i;
-    cur->nRight = gamma + 1;
-    if (hi == gamma + 1) { leafflag = 2; btl[gamma + 1].nParent = i; }
-    else btn[gamma + 1].nParent = i;
-    cur->leaf = leafflag;
-    cur->min = lo;
-    cur->max = hi;
-}
-
-// -------------------------------------------------------------------------------------------
-// Stage 4: leaves + bottom-up refit (BVH.cl:214-258 with a parent visit flag).  Leaf k takes the
-// AABB and index of shape sorted[k]; the second child to arrive at a parent merges the two child
-// bounds (left, right), where a child is a leaf iff nLeft == min / nRight == max.  The flag
-// hand-off is an acquire-release device-scope atomic, so the second arriver sees the first
-// child's bound.  Also writes the traversal layout (TNode per internal node, Prim per leaf).
-// -------------------------------------------------------------------------------------------
-__device__ __forceinline__ Prim make_prim(const Shape *s, int id) {
-    const float *w = reinterpret_cast<const float *>(s);
-    const int type = reinterpret_cast<const int *>(s)[0] | kLastInLeaf;   // one primitive per leaf
-    Prim p;
-    if ((type & 0xff) == SPHERE) {
-        p.p0 = make_float4(w[2], w[3], w[4], __int_as_float(id));
-        p.p1 = make_float4(w[1], 0.f, 0.f, __int_as_float(type));
-        p.p2 = make_float4(0.f, 0.f, 0.f, __int_as_float(type));
-    } else {
-        float x1 = w[1], y1 = w[2], z1 = w[3];
-        p.p0 = make_float4(x1, y1, z1, __int_as_float(id));
-        p.p1 = make_float4(w[4] - x1, w[5] - y1, w[6] - z1, __int_as_float(type));
-        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, __int_as_float(type));
-    }
-    return p;
-}
-
-__device__ __forceinline__ void store_bound(BVHNodeGPU *node, const Bound &b) {
-    float *f = reinterpret_cast<float *>(&node->bound);
-    f[0] = b.min_x; f[1] = b.max_x; f[2] = b.min_y; f[3] = b.max_y; f[4] = b.min_z; f[5] = b.max_z;
-}
-
-__device__ __forceinline__ TNode make_tnode(const Bound &L, const Bound &R, int rl, int rr, int cl, int cr) {
-    TNode t;
-    t.a = make_float4(L.min_x, L.min_y, L.min_z, L.max_x);
-    t.b = make_float4(L.max_y, L.max_z, R.min_x, R.min_y);
-    t.c = make_float4(R.min_z, R.max_x, R.max_y, R.max_z);
-    t.r = make_int4(rl, rr, cl, cr);
-    return t;
-}
-
-__global__ __launch_bounds__(256) void k_refit(const Shape *shapes, const unsigned *sorted, int n,
-                                               BVHNodeGPU *btn, BVHNodeGPU *btl, unsigned *flags,
-                                               TNode *tnodes, Prim *prims) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int sid = (int)sorted[k];
-    const Shape *s = shapes + sid;
-    Bound b = load_bound(&s->b);
-    store_bound(btl + k, b);
-    btl[k].nShape = sid;
-    prims[k] = make_prim(s, sid);
-    if (n < 2) return;
-    int p = btl[k].nParent;
-    for (;;) {
-        unsigned old = add_acq_rel(flags + p, 1u);
-        if (old == 0) return;
-        const int nl = btn[p].nLeft, nr = btn[p].nRight, mn = btn[p].min, mx = btn[p].max;
-        const bool lleaf = nl == mn, rleaf = nr == mx;
-        Bound L = load_bound(lleaf ? &btl[nl].bound : &btn[nl].bound);
-        Bound R = load_bound(rleaf ? &btl[nr].bound : &btn[nr].bound);
-        store_bound(btn + p, merge_bounds(L, R));
-        tnodes[p] = make_tnode(L, R, lleaf ? ~nl : nl, rleaf ? ~nr : nr, lleaf ? 1 : 0, rleaf ? 1 : 0);
-        if (p == 0) return;
-        p = btn[p].nParent;
-    }
-}
-
-// -------------------------------------------------------------------------------------------
-// Stage 4 (index-keyed trees): fence-free refit by level passes.
-// The Karras tree over leaf indices is the binary radix tree of 0..N-1, so every internal node
-// covers an aligned block [j*2^s, min((j+1)*2^s, N) - 1] of leaves, with s = ceil(log2(len)),
-// and its Karras index is a closed form of (lo, hi, N): the node is the LEFT child of its parent
-// (index hi) iff bit s of lo is 0 and leaves remain after the block, otherwise it is the right
-// child or the root (index lo).  One workgroup reduces 1024 consecutive items (leaves, or subtree
-// roots of the previous pass) bottom-up in LDS, merging (left, right) exactly as BVH.cl:112-122,
-// writes every internal node's bound and TNode on the way, and hands its block root to the next
-// pass.  No atomics, no cross-workgroup visibility, deterministic; identical results to the
-// visit-flag refit (k_refit), which remains for trees whose topology is not index-keyed.
-// -------------------------------------------------------------------------------------------
-constexpr int kRefitLog = 10;
-constexpr int kRefitItems = 1 << kRefitLog;
-
-__device__ __forceinline__ int karras_index(long long lo, long long hi, int n) {
-    const int len = (int)(hi - lo + 1);
-    const int s = 32 - __clz(len - 1);   // ceil(log2(len)), len >= 2
-    return (lo + (1ll << s) <= n - 1 && !((lo >> s) & 1)) ? (int)hi : (int)lo;
-}
-__device__ __forceinline__ int child_ref(long long lo, long long hi, int n) {
-    return lo == hi ? ~(int)lo : karras_index(lo, hi, n);
+    karras_body(n, btn, btl, blockIdx.x);
 }
 
 template <bool LEAF, bool TNODES>
 __global__ __launch_bounds__(256) void k_refit_levels(const Shape *shapes, const unsigned *sorted, int n, int s0,
                                                       int levels, const float *in_bounds, float *out_bounds,
                                                       BVHNodeGPU *btn, BVHNodeGPU *btl, TNode *tnodes, Prim *prims) {
-    __shared__ float sb[6][kRefitItems];
-    const long long item_base = (long long)blockIdx.x * kRefitItems;
-    const long long m = ((long long)n + (1ll << s0) - 1) >> s0;   // items at level s0
-    for (int q = 0; q < kRefitItems / 256; q++) {
-        const int j = threadIdx.x + q * 256;
-        const long long it = item_base + j;
-        if (it >= m) continue;
-        Bound B;
-        if (LEAF) {
-            const int sid = (int)sorted[it];
-            const Shape *sh = shapes + sid;
-            B = load_bound(&sh->b);
-            store_bound(btl + it, B);
-            btl[it].nShape = sid;
-            prims[it] = make_prim(sh, sid);
-        } else {
-            const float *f = in_bounds + it * 6;
-            B.min_x = f[0]; B.max_x = f[1]; B.min_y = f[2]; B.max_y = f[3]; B.min_z = f[4]; B.max_z = f[5];
-        }
-        sb[0][j] = B.min_x; sb[1][j] = B.max_x; sb[2][j] = B.min_y;
-        sb[3][j] = B.max_y; sb[4][j] = B.min_z; sb[5][j] = B.max_z;
-    }
-    __syncthreads();
-    for (int lev = 1; lev <= levels; lev++) {
-        const int s = s0 + lev;
-        const int nodes_here = kRefitItems >> lev;
-        for (int j = threadIdx.x; j < nodes_here; j += 256) {
-            const long long lo = (item_base + ((long long)j << lev)) << s0;
-            const long long mid = lo + (1ll << (s - 1));
-            if (mid > (long long)n - 1) continue;   // right half empty (or block past the end): collapsed
-            const long long hi_full = lo + (1ll << s) - 1;
-            const long long hi = hi_full < n - 1 ? hi_full : (long long)n - 1;
-            const int ls = j << lev, rs = ls + (1 << (lev - 1));
-            Bound L, R;
-            L.min_x = sb[0][ls]; L.max_x = sb[1][ls]; L.min_y = sb[2][ls];
-            L.max_y = sb[3][ls]; L.min_z = sb[4][ls]; L.max_z = sb[5][ls];
-            R.min_x = sb[0][rs]; R.max_x = sb[1][rs]; R.min_y = sb[2][rs];
-            R.max_y = sb[3][rs]; R.min_z = sb[4][rs]; R.max_z = sb[5][rs];
-            const Bound M = merge_bounds(L, R);
-            sb[0][ls] = M.min_x; sb[1][ls] = M.max_x; sb[2][ls] = M.min_y;
-            sb[3][ls] = M.max_y; sb[4][ls] = M.min_z; sb[5][ls] = M.max_z;
-            const int idx = karras_index(lo, hi, n);
-            store_bound(btn + idx, M);
-            if (TNODES) {
-                const int rl = child_ref(lo, mid - 1, n), rr = child_ref(mid, hi, n);
-                tnodes[idx] = make_tnode(L, R, rl, rr, rl < 0 ? 1 : 0, rr < 0 ? 1 : 0);
-            }
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < 6 && out_bounds) out_bounds[(long long)blockIdx.x * 6 + threadIdx.x] = sb[threadIdx.x][0];
-}
-
-// -------------------------------------------------------------------------------------------
-// Traversal tree: Karras radix tree over the sorted Morton keys (ties broken by leaf index), the
-// standard LBVH.  The reference exports the index-keyed tree (kept bit-exact in btn/btl); the
-// traversal layout may use any tree over the same leaves, and the Morton-keyed one halves the SAH
-// cost of the index-keyed one on the C2 soup (159 vs 318).  Closest-hit and any-hit results do
-// not depend on the tree.
-// -------------------------------------------------------------------------------------------
-__device__ __forceinline__ int delta_key(const unsigned *keys, int i, int j, int n) {
-    if (j < 0 || j >= n) return -1;
-    const unsigned x = keys[i] ^ keys[j];
-    return x ? __clz(x) : 32 + __clz(i ^ j);
+    refit_levels_body<LEAF>(shapes, sorted, n, s0, levels, in_bounds, out_bounds, btn, btl, tnodes, prims, TNODES,
+                            blockIdx.x);
 }
 
 __global__ __launch_bounds__(256) void k_karras_morton(const unsigned *keys, int n, int4 *mnode, int *mpar_int,
                                                        int *mpar_leaf) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    const int d = (delta_key(keys, i, i + 1, n) - delta_key(keys, i, i - 1, n)) > 0 ? 1 : -1;
-    const int dmin = delta_key(keys, i, i - d, n);
-    long long lmax = 2;
-    while (delta_key(keys, i, (int)(i + lmax * d), n) > dmin) lmax <<= 1;
-    long long l = 0;
-    for (long long t = lmax >> 1; t >= 1; t >>= 1)
-        if (delta_key(keys, i, (int)(i + (l + t) * d), n) > dmin) l += t;
-    const int j = (int)(i + l * d);
-    const int dnode = delta_key(keys, i, j, n);
-    long long s = 0, t = l;
-    do {
-        t = (t + 1) >> 1;
-        if (delta_key(keys, i, (int)(i + (s + t) * d), n) > dnode) s += t;
-    } while (t > 1);
-    const int gamma = (int)(i + s * d + (d < 0 ? d : 0));
-    const int lo = i < j ? i : j, hi = i < j ? j : i;
-    const int lref = (lo == gamma) ? ~gamma : gamma;
-    const int rref = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
-    mnode[i] = make_int4(lref, rref, lo, hi);
-    if (lref < 0) mpar_leaf[gamma] = i; else mpar_int[gamma] = i;
-    if (rref < 0) mpar_leaf[gamma + 1] = i; else mpar_int[gamma + 1] = i;
-    if (i == 0) mpar_int[0] = -1;
+    karras_morton_body(keys, n, mnode, mpar_int, mpar_leaf, blockIdx.x);
 }
-
-// Pass 1: one workgroup per 1024 consecutive leaves.  Every node whose leaf range lies inside the
-// chunk has its Karras index inside the chunk too, so visit counters and bounds live in LDS and the
-// bottom-up climb synchronises through workgroup-scope LDS atomics only.  A climb that reaches a
-// node spanning chunks publishes its child's bound and height and queues the (node, child) arrival.
-constexpr int kMChunk = 1024;
 
 __global__ __launch_bounds__(256) void k_mrefit_chunk(int n, const int4 *mnode, const int *mpar_int,
                                                       const int *mpar_leaf, const BVHNodeGPU *btl, TNode *tnodes,
                                                       float *gbi, int *gh, int2 *xlist, unsigned *xcount,
                                                       unsigned *root_height) {
-    __shared__ float lb[6][kMChunk];
-    __shared__ float nb[6][kMChunk];
-    __shared__ int nh[kMChunk];
-    __shared__ unsigned cnt[kMChunk];
-    const int base = blockIdx.x * kMChunk;
-    const int cend = (n - base) < kMChunk ? n : base + kMChunk;
-    for (int q = threadIdx.x; q < kMChunk; q += 256) {
-        cnt[q] = 0;
-        const int k = base + q;
-        if (k < cend) {
-            const Bound b = load_bound(&btl[k].bound);
-            lb[0][q] = b.min_x; lb[1][q] = b.max_x; lb[2][q] = b.min_y;
-            lb[3][q] = b.max_y; lb[4][q] = b.min_z; lb[5][q] = b.max_z;
-        }
-    }
-    __syncthreads();
-    for (int q = threadIdx.x; q < kMChunk; q += 256) {
-        const int k = base + q;
-        if (k >= cend) break;
-        int child = ~k, ch = 0;
-        int p = n > 1 ? mpar_leaf[k] : -1;
-        while (p >= 0) {
-            const int4 P = mnode[p];
-            if (P.z < base || P.w >= base + kMChunk) {
-                if (child >= 0) {
-                    const int c = child - base;
-                    for (int f = 0; f < 6; f++) gbi[(size_t)child * 6 + f] = nb[f][c];
-                    gh[child] = ch;
-                }
-                const unsigned slot = atomicAdd(xcount, 1u);
-                xlist[slot] = make_int2(p, child);
-                break;
-            }
-            const unsigned old = __hip_atomic_fetch_add(&cnt[p - base], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            Bound L, R;
-            int hl, hr;
-            {
-                const int a = P.x < 0 ? ~P.x - base : P.x - base;
-                float (*src)[kMChunk] = P.x < 0 ? lb : nb;
-                L.min_x = src[0][a]; L.max_x = src[1][a]; L.min_y = src[2][a];
-                L.max_y = src[3][a]; L.min_z = src[4][a]; L.max_z = src[5][a];
-                hl = P.x < 0 ? 0 : nh[a];
-            }
-            {
-                const int a = P.y < 0 ? ~P.y - base : P.y - base;
-                float (*src)[kMChunk] = P.y < 0 ? lb : nb;
-                R.min_x = src[0][a]; R.max_x = src[1][a]; R.min_y = src[2][a];
-                R.max_y = src[3][a]; R.min_z = src[4][a]; R.max_z = src[5][a];
-                hr = P.y < 0 ? 0 : nh[a];
-            }
-            const Bound M = merge_bounds(L, R);
-            const int c = p - base;
-            nb[0][c] = M.min_x; nb[1][c] = M.max_x; nb[2][c] = M.min_y;
-            nb[3][c] = M.max_y; nb[4][c] = M.min_z; nb[5][c] = M.max_z;
-            ch = (hl > hr ? hl : hr) + 1;
-            nh[c] = ch;
-            tnodes[p] = make_tnode(L, R, P.x, P.y, P.x < 0 ? 1 : 0, P.y < 0 ? 1 : 0);
-            child = p;
-            p = mpar_int[p];
-        }
-        if (p < 0 && child >= 0) *root_height = (unsigned)ch;   // the root closed inside this chunk
-    }
+    mrefit_chunk_body(n, mnode, mpar_int, mpar_leaf, btl, tnodes, gbi, gh, xlist, xcount, root_height, blockIdx.x);
 }
 
-// Pass 2: the nodes spanning chunks, one workgroup, arrivals from pass 1's queue.  Child bounds
-// written by pass 1 are visible (earlier launch); those written here by another thread of the
-// same workgroup are ordered by the workgroup-scope acquire/release counter.
 __global__ __launch_bounds__(1024) void k_mrefit_top(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
                                                      TNode *tnodes, float *gbi, int *gh, const int2 *xlist,
                                                      const unsigned *xcount, unsigned *gcnt, unsigned *root_height) {
-    const unsigned count = *xcount;
-    for (unsigned e = threadIdx.x; e < count; e += blockDim.x) {
-        int p = xlist[e].x;
-        for (;;) {
-            const unsigned old = __hip_atomic_fetch_add(gcnt + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-            if (old == 0) break;
-            const int4 P = mnode[p];
-            Bound L, R;
-            int hl = 0, hr = 0;
-            if (P.x < 0) L = load_bound(&btl[~P.x].bound);
-            else {
-                const float *f = gbi + (size_t)P.x * 6;
-                L.min_x = f[0]; L.max_x = f[1]; L.min_y = f[2]; L.max_y = f[3]; L.min_z = f[4]; L.max_z = f[5];
-                hl = gh[P.x];
-            }
-            if (P.y < 0) R = load_bound(&btl[~P.y].bound);
-            else {
-                const float *f = gbi + (size_t)P.y * 6;
-                R.min_x = f[0]; R.max_x = f[1]; R.min_y = f[2]; R.max_y = f[3]; R.min_z = f[4]; R.max_z = f[5];
-                hr = gh[P.y];
-            }
-            const Bound M = merge_bounds(L, R);
-            float *o = gbi + (size_t)p * 6;
-            o[0] = M.min_x; o[1] = M.max_x; o[2] = M.min_y; o[3] = M.max_y; o[4] = M.min_z; o[5] = M.max_z;
-            const int hp = (hl > hr ? hl : hr) + 1;
-            gh[p] = hp;
-            tnodes[p] = make_tnode(L, R, P.x, P.y, P.x < 0 ? 1 : 0, P.y < 0 ? 1 : 0);
-            const int pp = mpar_int[p];
-            if (pp < 0) {
-                *root_height = (unsigned)hp;
-                break;
-            }
-            p = pp;
-        }
-    }
+    mrefit_top_body(mnode, mpar_int, btl, tnodes, gbi, gh, xlist, xcount, gcnt, root_height);
 }
 
 }  // namespace ptgpu
@@ -737,7 +119,6 @@ static int clbvh_alloc(ptgpu_clbvh *h, int n) {
     PT_TRY(h->valsB.ensure(N * 4));
     PT_TRY(h->btn.ensure((N > 1 ? N - 1 : 1) * sizeof(BVHNodeGPU)));
     PT_TRY(h->btl.ensure(N * sizeof(BVHNodeGPU)));
-    PT_TRY(h->flags.ensure(N * 4));
     PT_TRY(h->tnodes.ensure((N > 1 ? N - 1 : 1) * sizeof(TNode)));
     PT_TRY(h->prims.ensure(N * sizeof(Prim)));
     // sort scratch: histograms (<= 8 passes x 256), status words for every pass, counters
@@ -747,9 +128,35 @@ static int clbvh_alloc(ptgpu_clbvh *h, int n) {
     return PTGPU_OK;
 }
 
-static inline float *misc_scene6(ptgpu_clbvh *h) { return h->misc.as<float>(); }
-static inline unsigned *misc_counters(ptgpu_clbvh *h) { return h->misc.as<unsigned>() + 8; }  // [0]=bounds done, [1..8] tile counters, [9] err
-static inline float *misc_partials(ptgpu_clbvh *h) { return h->misc.as<float>() + 64; }
+static inline float *misc_scene6(ptgpu_clbvh *h) { return h->scene6(); }
+static inline unsigned *misc_counters(ptgpu_clbvh *h) { return h->counters(); }
+static inline float *misc_partials(ptgpu_clbvh *h) { return h->partials(); }
+
+namespace ptgpu {
+int clbvh_level_scratch(ptgpu_clbvh *h, float **halves, size_t *per) {
+    *per = (((size_t)h->n + kRefitItems - 1) / kRefitItems + 1) * 6;
+    PT_TRY(h->opt_tmp.ensure(2 * *per * sizeof(float)));
+    halves[0] = h->opt_tmp.as<float>();
+    halves[1] = h->opt_tmp.as<float>() + *per;
+    return PTGPU_OK;
+}
+
+int clbvh_morton_scratch(ptgpu_clbvh *h, MortonScratch *m) {
+    const size_t N = (size_t)h->n;
+    PT_TRY(h->mnode.ensure((N > 1 ? N - 1 : 1) * sizeof(int4)));
+    PT_TRY(h->mpar.ensure((2 * N + 2) * sizeof(int)));
+    PT_TRY(h->gbi.ensure((N > 1 ? N - 1 : 1) * 6 * sizeof(float) + (N + 2) * sizeof(int)));
+    PT_TRY(h->xlist.ensure((2 * N + 2) * sizeof(int2) + (N + 2) * sizeof(unsigned)));
+    m->mnode = h->mnode.as<int4>();
+    m->mpar_int = h->mpar.as<int>();
+    m->mpar_leaf = h->mpar.as<int>() + N;
+    m->gbi = h->gbi.as<float>();
+    m->gh = reinterpret_cast<int *>(m->gbi + (N > 1 ? N - 1 : 1) * 6);
+    m->xlist = h->xlist.as<int2>();
+    m->gcnt = reinterpret_cast<unsigned *>(m->xlist + 2 * N + 2);
+    return PTGPU_OK;
+}
+}  // namespace ptgpu
 
 static int clbvh_construct(ptgpu_clbvh *h) {
     hipStream_t st = h->ctx->stream;
@@ -776,9 +183,9 @@ static int clbvh_refit_levels(ptgpu_clbvh *h) {
     const int n = h->n;
     int L = 0;
     while ((1ll << L) < n) L++;
-    const size_t per = (((size_t)n + kRefitItems - 1) / kRefitItems + 1) * 6;
-    PT_TRY(h->opt_tmp.ensure(2 * per * sizeof(float)));
-    float *buf[2] = {h->opt_tmp.as<float>(), h->opt_tmp.as<float>() + per};
+    size_t per = 0;
+    float *buf[2];
+    PT_TRY(clbvh_level_scratch(h, buf, &per));
     int s0 = 0, pass = 0;
     const float *in = nullptr;
     do {
@@ -819,15 +226,13 @@ static int clbvh_traversal_morton(ptgpu_clbvh *h) {
     hipStream_t st = h->ctx->stream;
     const int n = h->n;
     const size_t N = (size_t)n;
-    PT_TRY(h->mnode.ensure((N > 1 ? N - 1 : 1) * sizeof(int4)));
-    PT_TRY(h->mpar.ensure((2 * N + 2) * sizeof(int)));
-    PT_TRY(h->gbi.ensure((N > 1 ? N - 1 : 1) * 6 * sizeof(float) + (N + 2) * sizeof(int)));
-    PT_TRY(h->xlist.ensure((2 * N + 2) * sizeof(int2) + (N + 2) * sizeof(unsigned)));
-    int *mpar_int = h->mpar.as<int>(), *mpar_leaf = h->mpar.as<int>() + N;
-    float *gbi = h->gbi.as<float>();
-    int *gh = reinterpret_cast<int *>(gbi + (N > 1 ? N - 1 : 1) * 6);
-    int2 *xlist = h->xlist.as<int2>();
-    unsigned *gcnt = reinterpret_cast<unsigned *>(xlist + 2 * N + 2);
+    MortonScratch ms;
+    PT_TRY(clbvh_morton_scratch(h, &ms));
+    int *mpar_int = ms.mpar_int, *mpar_leaf = ms.mpar_leaf;
+    float *gbi = ms.gbi;
+    int *gh = ms.gh;
+    int2 *xlist = ms.xlist;
+    unsigned *gcnt = ms.gcnt;
     unsigned *ctr = misc_counters(h);   // [10] crossing count, [11] root height
     PT_TRY(fill_u32(st, ctr + 10, 0u, 2));
     if (n > 1) {
@@ -846,7 +251,6 @@ static int clbvh_traversal_morton(ptgpu_clbvh *h) {
             PT_HIP(hipGetLastError());
         }
     }
-    h->height_pending = true;
     return PTGPU_OK;
 }
 
@@ -903,6 +307,56 @@ int clbvh_prepare_traversal(ptgpu_clbvh *h) {
 }
 }  // namespace ptgpu
 
+int ptgpu::clbvh_new(ptgpu_ctx *ctx, int n, ptgpu_clbvh **out) {
+    ptgpu_clbvh *h = new ptgpu_clbvh();
+    h->ctx = ctx;
+    h->n = n;
+    h->cap = n;
+    for (auto &e : h->ev) {
+        hipError_t r = hipEventCreate(&e);
+        if (r != hipSuccess) {
+            ptgpu_clbvh_destroy(h);
+            return hip_fail(r, "hipEventCreate", __FILE__, __LINE__);
+        }
+    }
+    int s = clbvh_alloc(h, n);
+    if (s == PTGPU_OK) s = fill_u32(ctx->stream, h->misc.p, 0u, h->misc.bytes / 4);
+    if (s != PTGPU_OK) {
+        ptgpu_clbvh_destroy(h);
+        return s;
+    }
+    *out = h;
+    return PTGPU_OK;
+}
+
+void ptgpu::clbvh_mark_built(ptgpu_clbvh *h) {
+    const int n = h->n;
+    h->state = 3;
+    h->optimized = false;
+    h->export_pending = false;
+    h->acc.nodes = h->tnodes.as<TNode>();
+    h->acc.prims = h->prims.as<Prim>();
+    h->acc.nprims = n;
+    h->acc.nnodes = n - 1;
+    h->acc.nshapes = n;
+    h->acc.mode = PTGPU_ACCEL_BVH;
+    h->acc.leaf_runs = 1;
+    h->acc.nodes4 = nullptr;
+    h->wide_pending = n >= 2;   // four-wide nodes are derived at the first traversal after a build
+    if (n >= 2) {
+        h->acc.root = 0;
+        h->acc.root_cnt = 0;
+        int d = 0;
+        while ((1ll << d) < n) d++;
+        h->acc.depth = d;                 // index-keyed: ceil(log2 N); Morton-keyed: read back lazily
+    } else {
+        h->acc.root = ~0;
+        h->acc.root_cnt = 1;
+        h->acc.depth = 0;
+    }
+    h->height_pending = h->trav_topology == 1 && n >= 2;
+}
+
 extern "C" {
 
 int ptgpu_copy_calibration(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
@@ -955,28 +409,6 @@ int ptgpu_clbvh_set_traversal_topology(ptgpu_clbvh *h, int mode) {
     PT_CHECK(h, PTGPU_ERR_INVALID, "null handle");
     PT_CHECK(mode == 0 || mode == 1, PTGPU_ERR_INVALID, "traversal topology %d (0 reference, 1 Morton)", mode);
     h->trav_topology = mode;
-    return PTGPU_OK;
-}
-
-static int clbvh_new(ptgpu_ctx *ctx, int n, ptgpu_clbvh **out) {
-    ptgpu_clbvh *h = new ptgpu_clbvh();
-    h->ctx = ctx;
-    h->n = n;
-    h->cap = n;
-    for (auto &e : h->ev) {
-        hipError_t r = hipEventCreate(&e);
-        if (r != hipSuccess) {
-            ptgpu_clbvh_destroy(h);
-            return hip_fail(r, "hipEventCreate", __FILE__, __LINE__);
-        }
-    }
-    int s = clbvh_alloc(h, n);
-    if (s == PTGPU_OK) s = fill_u32(ctx->stream, h->misc.p, 0u, h->misc.bytes / 4);
-    if (s != PTGPU_OK) {
-        ptgpu_clbvh_destroy(h);
-        return s;
-    }
-    *out = h;
     return PTGPU_OK;
 }
 
@@ -1070,29 +502,7 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     PT_HIP(hipEventRecord(h->ev[6], st));
     h->timed[1] = true;
     h->timed[3] = true;
-    h->state = 3;
-    h->optimized = false;
-    h->export_pending = false;
-    h->acc.nodes = h->tnodes.as<TNode>();
-    h->acc.prims = h->prims.as<Prim>();
-    h->acc.nprims = n;
-    h->acc.nnodes = n - 1;
-    h->acc.nshapes = n;
-    h->acc.mode = PTGPU_ACCEL_BVH;
-    h->acc.leaf_runs = 1;
-    h->acc.nodes4 = nullptr;
-    h->wide_pending = n >= 2;   // four-wide nodes are derived at the first traversal after a build
-    if (n >= 2) {
-        h->acc.root = 0;
-        h->acc.root_cnt = 0;
-        int d = 0;
-        while ((1ll << d) < n) d++;
-        h->acc.depth = d;                 // index-keyed: ceil(log2 N); Morton-keyed: read back lazily
-    } else {
-        h->acc.root = ~0;
-        h->acc.root_cnt = 1;
-        h->acc.depth = 0;
-    }
+    clbvh_mark_built(h);
     return PTGPU_OK;
 }
 
@@ -1135,7 +545,7 @@ int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVH
 
 int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes) {
     PT_CHECK(h && bytes, PTGPU_ERR_INVALID, "null argument");
-    const DevBuf *bufs[] = {&h->own_shapes, &h->keysA, &h->keysB, &h->valsA, &h->valsB, &h->btn, &h->btl, &h->flags,
+    const DevBuf *bufs[] = {&h->own_shapes, &h->keysA, &h->keysB, &h->valsA, &h->valsB, &h->btn, &h->btl,
                             &h->tnodes, &h->prims, &h->hist, &h->status, &h->misc, &h->lights, &h->scan,
                             &h->opt_tmp, &h->mnode, &h->mpar, &h->gbi, &h->xlist, &h->tnodes4, &h->treelet};
     size_t t = 0;
@@ -1169,7 +579,7 @@ int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
     if (!h) return PTGPU_OK;
     if (h->ctx && h->ctx->stream) (void)hipStreamSynchronize(h->ctx->stream);
     h->keysA.release(); h->keysB.release(); h->valsA.release(); h->valsB.release();
-    h->btn.release(); h->btl.release(); h->flags.release(); h->tnodes.release(); h->prims.release();
+    h->btn.release(); h->btl.release(); h->tnodes.release(); h->prims.release();
     h->hist.release(); h->status.release(); h->misc.release(); h->own_shapes.release();
     h->lights.release(); h->scan.release(); h->opt_tmp.release();
     h->mnode.release(); h->mpar.release(); h->gbi.release(); h->xlist.release(); h->tnodes4.release();

@@ -8,6 +8,22 @@
 namespace ptgpu {
 int clbvh_prepare_traversal(ptgpu_clbvh *h);
 int clbvh_export_optimized(ptgpu_clbvh *h);
+
+// Scratch of the Morton-keyed traversal tree: topology, parents, and the refit's crossing queue.
+struct MortonScratch {
+    int4 *mnode;
+    int *mpar_int, *mpar_leaf;
+    float *gbi;      // bounds of internal nodes whose subtree spans 1024-leaf chunks
+    int *gh;         // their heights
+    int2 *xlist;     // (node, child) arrivals at chunk-spanning nodes
+    unsigned *gcnt;  // visit counters of the top pass
+};
+int clbvh_morton_scratch(ptgpu_clbvh *h, MortonScratch *m);
+// Bound buffers handed between the level passes of the index-keyed refit (two halves of `per` floats).
+int clbvh_level_scratch(ptgpu_clbvh *h, float **halves, size_t *per);
+int clbvh_new(ptgpu_ctx *ctx, int n, ptgpu_clbvh **out);
+// Handle state after a full build (bounds, Morton, topology, sort, refit) has been enqueued.
+void clbvh_mark_built(ptgpu_clbvh *h);
 }  // namespace ptgpu
 
 struct ptgpu_clbvh {
@@ -19,7 +35,7 @@ struct ptgpu_clbvh {
     Shape *d_shapes = nullptr;
     ptgpu::DevBuf own_shapes;
     ptgpu::DevBuf keysA, keysB, valsA, valsB;
-    ptgpu::DevBuf btn, btl, flags, tnodes, prims;
+    ptgpu::DevBuf btn, btl, tnodes, prims;
     ptgpu::DevBuf hist, status, misc;
     ptgpu::DevBuf lights, scan, opt_tmp;
     ptgpu::DevBuf mnode, mpar, gbi, xlist;   // Morton-keyed traversal tree: topology, parents, refit scratch
@@ -33,8 +49,14 @@ struct ptgpu_clbvh {
     int lights_for = -1, nlights = 0;
     ptgpu::DevAccel acc;
     std::vector<BVHNodeGPU> h_btn, h_btl;
+    // misc: [0..5] scene centroid bounds, words 8.. counters ([0] bounds blocks done, [1..8] sort
+    // tile counters, [9] sort look-back error, [10] chunk crossings, [11] traversal-tree height),
+    // floats 64.. per-block bound partials
+    float *scene6() const { return misc.as<float>(); }
+    unsigned *counters() const { return misc.as<unsigned>() + 8; }
+    float *partials() const { return misc.as<float>() + 64; }
     // device word holding the traversal tree's height (read back lazily by clbvh_prepare_traversal)
-    unsigned *height_word() const { return misc.as<unsigned>() + 8 + 11; }
+    unsigned *height_word() const { return counters() + 11; }
     // events: 0/1 morton, 2/3 topology, 4/5 sort, 5/6 refit, 7/8 optimize
     hipEvent_t ev[10] = {};
     bool timed[5] = {false, false, false, false, false};

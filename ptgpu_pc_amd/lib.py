@@ -97,6 +97,8 @@ _SIGS = {
     "ptgpu_clbvh_device_sorted": [vp, C.POINTER(vp)],
     "ptgpu_clbvh_stage_times": [vp, C.POINTER(f32)],
     "ptgpu_clbvh_destroy": [vp],
+    "ptgpu_clbvh_build_batch": [vp, vp, i32],
+    "ptgpu_clbvh_create_batch": [vp, vp, vp, i32, vp],
     "ptgpu_kdtree_build": [vp, vp, i32, C.POINTER(vp)],
     "ptgpu_kdtree_build_sah": [vp, vp, i32, i32, i32, C.POINTER(vp)],
     "ptgpu_kdtree_get_trees": [vp, C.POINTER(vp), C.POINTER(i32), C.POINTER(vp), C.POINTER(i32)],
@@ -318,6 +320,36 @@ class CLBVH:
             _check(ctx.lib.ptgpu_clbvh_set_digit_bits(self.handle, digit_bits))
         if traversal != "morton":
             self.set_traversal(traversal)
+
+    @classmethod
+    def create_batch(cls, ctx: Context, shapes: list, digit_bits: int = 8) -> list["CLBVH"]:
+        """Handles over many device shape arrays (DeviceArray each), all built in one batched sequence of
+        launches (ptgpu_clbvh_create_batch); each equals CLBVH(ctx, shapes[s]) + buildRadixTree + buildBVHTree."""
+        k = len(shapes)
+        ptrs = (vp * max(k, 1))(*[a.ptr for a in shapes])
+        counts = (i32 * max(k, 1))(*[a.nbytes // SHAPE_DTYPE.itemsize for a in shapes])
+        out = (vp * max(k, 1))()
+        if digit_bits != 8:   # the digit width is a per-handle setting: create, set, then build
+            hs = [cls(ctx, a) for a in shapes]
+            for h in hs:
+                h.set_digit_bits(digit_bits)
+            cls.build_batch(ctx, hs)
+            return hs
+        _check(ctx.lib.ptgpu_clbvh_create_batch(ctx.handle, ptrs, counts, k, out))
+        hs = []
+        for s in range(k):
+            h = cls.__new__(cls)
+            h.ctx, h.n, h.d_shapes, h.handle = ctx, int(counts[s]), shapes[s], out[s]
+            ctx._adopt(h)
+            hs.append(h)
+        return hs
+
+    @staticmethod
+    def build_batch(ctx: Context, handles: list["CLBVH"]) -> None:
+        """Rebuild every handle from its current device shapes in one batched sequence of launches
+        (ptgpu_clbvh_build_batch): reset + buildRadixTree + buildBVHTree per handle."""
+        arr = (vp * max(len(handles), 1))(*[h.handle for h in handles])
+        _check(ctx.lib.ptgpu_clbvh_build_batch(ctx.handle, arr, len(handles)))
 
     def set_traversal(self, topology: str) -> None:
         """'morton' (Morton-keyed LBVH, default) or 'reference' (the exported index-keyed tree)."""
