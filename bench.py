@@ -83,8 +83,8 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
 
 
 RENDER_KERNEL = "k_render_lanes"
-# the timed instance: single frame, four-wide nodes, no statistics (<stack, STATS, WIDE, MULTI>)
-RENDER_INSTANCE = "k_render_lanes<%, false, true, false>"
+# the timed instance: single frame, no statistics (<stack, STATS, WIDE, MULTI>)
+RENDER_INSTANCE = "k_render_lanes<%, false, %, false>"
 WORKLOAD = ("LBVH build + BVH traversal: 65,536-triangle soup (seed 1) + emissive quad, 512x512, 8 spp, depth 5, "
             "30-bit Morton, one leaf per primitive")
 TRAFFIC_FILE = ROOT / "profiles" / "k_render_lanes_c2_traffic.json"
@@ -131,7 +131,7 @@ def profile_summary(d: Path, out_md: Path) -> None:
         for name, val in _db_rows(dbp, "select counter_name, avg(value) from counters_collection where kernel_name "
                                        f"like '%{RENDER_INSTANCE}%' group by counter_name"):
             sq[name] = val
-    timed = re.compile(r"k_render_lanes<\d+, false, true, false>")
+    timed = re.compile(r"k_render_lanes<\d+, false, (true|false), false>")
     render_avg = next((r[2] / 1e3 for r in rows if timed.search(r[0])), None)
     cmds = json.loads((d / "commands.json").read_text())
     lines = [f"# {RENDER_KERNEL} — C2 (65,538-shape soup, 512x512, 8 spp, depth 5), MI355X", "",
@@ -229,7 +229,7 @@ def serve_trace(n_req: int, rate: float, pool: int, seed: int = 2024):
     pool_scenes = []
     for k in range(pool):
         ss = 1000 + k
-        n = int(np.exp(np.random.default_rng(ss).uniform(np.log(1e3), np.log(2e6))))
+        n = int(np.exp(np.random.default_rng([ss, 1]).uniform(np.log(1e3), np.log(2e6))))
         shapes, lc, _, kind = scenes.mixed_scene(ss, n, 64, 64)
         pool_scenes.append((ss, shapes, lc, kind))
     t, reqs = 0.0, []
@@ -371,7 +371,7 @@ def batch_workload(n_scenes: int = 64, seed: int = 4000):
     return out
 
 
-def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64):
+def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: bool = True):
     """Plain path (per scene: LBVH build, then one fused 4-spp render launch) against the batched
     path (every scene's build in one batched sequence of launches, every frame in one render
     launch) on the same resident scenes.  Both rebuild every tree and re-render every frame per step."""
@@ -422,6 +422,7 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64):
     batch_step()
     ctx.synchronize()
     same = all(a.tobytes() == f["pixels"].download(np.uint32).tobytes() for a, f in zip(pix_plain, frames))
+    tuning = pt.autotune_batch(ctx, [job(k, h) for k, h in enumerate(batch_h)], trials=1) if tune else None
 
     def timed(step_fn, events=False):
         for _ in range(warmup):
@@ -466,7 +467,7 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64):
                         "render_ms": round(r_ms, 2), "build_mtri_s": round(total_n / b_ms / 1e3, 1),
                         "build_alg_gbs": round(alg / (b_ms * 1e-3) / 1e9, 1)},
             "frames_speedup": round(t_plain / t_batch, 2), "build_speedup": round(plain_build_ms / b_ms, 2),
-            "pixels_identical": same,
+            "pixels_identical": same, "tuning": tuning,
             "workload": f"{nfr} scenes, triangles log-uniform [1e3, 1e6] (soup or displaced icosphere by seed), "
                         "orbit cameras, 256^2/512^2/1024^2 by seed, 4 spp, depth 5; every tree rebuilt and every "
                         "frame rendered per step"}
@@ -489,7 +490,9 @@ def main():
     ap.add_argument("--batch", action="store_true", help="only the 64-scene batched vs plain comparison (a JSON line)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison in the default line")
     ap.add_argument("--batch-scenes", type=int, default=64)
-    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in defaults instead of autotuning")
+    ap.add_argument("--tuning", default=None, help="JSON file with the tuning object of an earlier run (no autotune)")
+    ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=500.0)
     ap.add_argument("--pool", type=int, default=16)
     ap.add_argument("--spp-per-tick", type=int, default=16)
@@ -500,7 +503,7 @@ def main():
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
     if args.batch:
-        print(json.dumps(batch_compare(n_scenes=args.batch_scenes)), flush=True)
+        print(json.dumps(batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)), flush=True)
         return
     if args.serve:
         print(json.dumps(serve_compare(args.requests, args.rate, args.pool, args.spp_per_tick, args.budget_gb,
@@ -541,14 +544,34 @@ def main():
     d_rays = ctx.alloc(16, np.uint64).zero()
     d_stats = ctx.alloc(16, np.uint64).zero()
     bvh = pt.CLBVH(ctx, d_shapes, n)
+    treelet, passes = args.treelet, args.treelet_passes
+    tuning = None
+    if args.tuning:
+        # settings from an earlier autotuned run (its line's "tuning" object), e.g. for profiled runs
+        tuning = json.loads(Path(args.tuning).read_text())
+        for k in ("blocks_per_cu", "regroup", "wide"):
+            ctx.set_tuning(k, int(tuning[k]))
+        bvh.set_digit_bits(int(tuning["digit_bits"]))
+        if not args.treelet and tuning["treelet"]:
+            treelet, passes = int(tuning["treelet"]), 1
+    elif not args.no_tune:
+        # autotuner: digit width, launch shape, treelet size for this scene and frame (one frame per build)
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        d_seeds.copy_from(seeds0)
+        tuning = pt.autotune(ctx, bvh, frames_per_build=1, trials=2, colors=d_col, seeds=d_seeds, camera=d_cam,
+                             shapes=d_shapes, shapeCnt=n, lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP,
+                             max_depth=DEPTH, pixels=d_pix)
+        if not args.treelet and tuning["treelet"]:
+            treelet, passes = tuning["treelet"], 1
 
     def step(ray_counter=None, trav_stats=None):
         ctx.record(0)
         bvh.reset()
         bvh.buildRadixTree()
         bvh.buildBVHTree()
-        for _ in range(args.treelet_passes if args.treelet else 0):
-            bvh.optimize(args.treelet)
+        for _ in range(passes if treelet else 0):
+            bvh.optimize(treelet)
         d_seeds.copy_from(seeds0)
         ctx.record(1)
         pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_shapes, shapeCnt=n,
@@ -608,7 +631,7 @@ def main():
 
     batched = None
     if not args.no_batch:
-        batched = batch_compare(n_scenes=args.batch_scenes)
+        batched = batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)
 
     cpu = None
     if not args.no_cpu:
@@ -640,6 +663,7 @@ def main():
         "build_stage_ms": {k: round(v, 4) for k, v in stage.items()},
         "msamples_s": round(W * H * SPP * args.steps / wall / 1e6, 2),
         "roofline": roofline,
+        "tuning": tuning,
         "served": served,
         "batched": batched,
         "cpu_baseline": cpu,

@@ -285,13 +285,41 @@ typedef struct {
 } ptgpu_render_job;
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter);
 
+/* ---- Autotuning (no reference counterpart) ----
+ * Chooses the settings that change speed only -- every value gives bit-identical images, hits and
+ * trees -- by timing real work on the caller's scene and frame (HIP events, fastest of `trials`).
+ * Trial renders use scratch copies of the frame buffers: the caller's colors, seeds and pixels are
+ * left as they were. */
+typedef struct {
+    int blocks_per_cu;        /* ptgpu_ctx_set_tuning names */
+    int regroup;
+    int wide;
+    int batch_blocks_per_cu;
+    int batch_regroup;
+    int digit_bits;           /* LBVH sort digit width (4 or 8) */
+    int treelet;              /* treelet restructuring size, 0 = none */
+    float render_ms;          /* fastest trial render with the chosen settings */
+    float build_ms;           /* fastest rebuild (+ optimise) with the chosen settings */
+} ptgpu_tuning;
+/* Tunes the digit width, the single-frame launch shape (blocks per CU 1..4, regroup 8/16/32, four-
+ * wide or binary nodes) and the treelet size (none, 5, 7; minimising rebuild + optimise +
+ * frames_per_build renders).  Applies the result to ctx and h and leaves h holding a tree built
+ * (and optimised) with it.  args: one ptgpu_render call's arguments on h (h: >= 2 shapes). */
+int ptgpu_autotune(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *args, int frames_per_build,
+                   int trials, ptgpu_tuning *out);
+/* Tunes batch_blocks_per_cu (2..4) and batch_regroup (16/32/64) on a ptgpu_render_batch job list
+ * and applies them to ctx. */
+int ptgpu_autotune_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, int trials,
+                         ptgpu_tuning *out);
+
 /* ---- Serving: continuous batching of render requests (no reference counterpart) ----
  * A request renders one frame of one scene: `spp` samples from sample 0.  Requests queue in
  * submission order.  ptgpu_server_tick admits queued requests, head of line first, while their HBM
  * footprint fits the budget (the scene's shapes + LBVH unless the scene key is resident, plus the
  * frame's colors/seeds/pixels), builds the scenes they bring in, and renders min(spp_per_tick,
- * remaining) samples of every admitted request in one batched launch; it returns when that launch
- * is done.  A scene stays resident while requests use it and afterwards until its memory is needed
+ * remaining) samples of every admitted request in one batched launch.  Up to two launches are in
+ * flight (on two streams): a tick retires the launches that have finished, and waits for the oldest
+ * only when it has nothing new to launch.  A scene stays resident while requests use it and afterwards until its memory is needed
  * (least recently used first).  A finished frame stays on the device until ptgpu_server_result
  * copies it out.  Every result is bit-identical to one ptgpu_render of the request alone.
  * h_shapes and h_seeds must stay valid until the tick that admits the request returns.  Requests

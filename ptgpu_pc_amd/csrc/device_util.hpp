@@ -32,21 +32,14 @@ __device__ __forceinline__ unsigned long long lanemask_lt() {
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
-// Device-scope atomics used for cross-workgroup hand-off (decoupled look-back, refit flags).
-__device__ __forceinline__ unsigned ld_acquire(const unsigned *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Device-scope relaxed atomics for the sort's decoupled look-back: each status word carries its own
+// value, so no ordering with other memory is needed (an acquire / release at device scope would
+// write back or invalidate the L2 on gfx950).
 __device__ __forceinline__ unsigned ld_relaxed(const unsigned *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void st_release(unsigned *p, unsigned v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ void st_relaxed(unsigned *p, unsigned v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned add_acq_rel(unsigned *p, unsigned v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Fill kernel (used instead of hipMemsetAsync so that sequences stay graph-capture safe).

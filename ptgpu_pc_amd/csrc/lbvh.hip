@@ -39,9 +39,12 @@ int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words) {
     return PTGPU_OK;
 }
 
-__global__ __launch_bounds__(kBoundsThreads) void k_bounds(Shape *shapes, int n, float *partials,
-                                                           unsigned *done_counter, float *scene6) {
-    bounds_body(shapes, n, partials, done_counter, scene6, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kBoundsThreads) void k_bounds(Shape *shapes, int n, float *partials) {
+    bounds_body(shapes, n, partials, blockIdx.x, gridDim.x);
+}
+
+__global__ __launch_bounds__(kBoundsThreads) void k_bounds_final(const float *partials, int nblocks, float *scene6) {
+    bounds_final_body(partials, nblocks, scene6);
 }
 
 __global__ __launch_bounds__(256) void k_morton(Shape *shapes, int n, const float *scene6, unsigned *keys) {
@@ -165,8 +168,10 @@ static int clbvh_construct(ptgpu_clbvh *h) {
     int blocks = (n + kBoundsThreads - 1) / kBoundsThreads;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBoundsThreads), 0, st, h->d_shapes, n, misc_partials(h),
-                       misc_counters(h), misc_scene6(h));
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBoundsThreads), 0, st, h->d_shapes, n, misc_partials(h));
+    PT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBoundsThreads), 0, st, (const float *)misc_partials(h), blocks,
+                       misc_scene6(h));
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, h->d_shapes, n, misc_scene6(h),
                        h->keysA.as<unsigned>());
@@ -332,6 +337,7 @@ int ptgpu::clbvh_new(ptgpu_ctx *ctx, int n, ptgpu_clbvh **out) {
 void ptgpu::clbvh_mark_built(ptgpu_clbvh *h) {
     const int n = h->n;
     h->state = 3;
+    h->timed[4] = false;   // an optimisation timed before this build no longer describes the tree
     h->optimized = false;
     h->export_pending = false;
     h->acc.nodes = h->tnodes.as<TNode>();

@@ -49,7 +49,7 @@ struct ptgpu_clbvh {
     int lights_for = -1, nlights = 0;
     ptgpu::DevAccel acc;
     std::vector<BVHNodeGPU> h_btn, h_btl;
-    // misc: [0..5] scene centroid bounds, words 8.. counters ([0] bounds blocks done, [1..8] sort
+    // misc: [0..5] scene centroid bounds, words 8.. counters ([0] unused, [1..8] sort
     // tile counters, [9] sort look-back error, [10] chunk crossings, [11] traversal-tree height),
     // floats 64.. per-block bound partials
     float *scene6() const { return misc.as<float>(); }

@@ -77,7 +77,13 @@ __global__ __launch_bounds__(256) void kb_fill(const FillSeg *f, const int *star
 __global__ __launch_bounds__(kBoundsThreads) void kb_bounds(const BuildSeg *g, const int *starts, int ns) {
     const int s = seg_find(starts, ns, blockIdx.x);
     const BuildSeg &S = g[s];
-    bounds_body(S.shapes, S.n, S.partials, S.ctr, S.scene6, blockIdx.x - starts[s], starts[s + 1] - starts[s]);
+    bounds_body(S.shapes, S.n, S.partials, blockIdx.x - starts[s], starts[s + 1] - starts[s]);
+}
+
+// one block per scene
+__global__ __launch_bounds__(kBoundsThreads) void kb_bounds_final(const BuildSeg *g, const int *starts) {
+    const BuildSeg &S = g[blockIdx.x];
+    bounds_final_body(S.partials, starts[blockIdx.x + 1] - starts[blockIdx.x], S.scene6);
 }
 
 __global__ __launch_bounds__(256) void kb_morton(const BuildSeg *g, const int *starts, int ns) {
@@ -293,6 +299,8 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         PT_HIP(hipGetLastError());                                                                        \
     }
     PT_BATCH_LAUNCH(kb_bounds, kStBounds, kBoundsThreads);
+    hipLaunchKernelGGL(kb_bounds_final, dim3(ns), dim3(kBoundsThreads), 0, st, d_segs, d_starts + table(kStBounds));
+    PT_HIP(hipGetLastError());
     PT_BATCH_LAUNCH(kb_morton, kStMorton, 256);
     PT_BATCH_LAUNCH(kb_karras, kStKarras, 256);
     PT_BATCH_LAUNCH(kb_sort_hist<BITS>, kStHist, kSortThreads);

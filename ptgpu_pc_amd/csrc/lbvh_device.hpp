@@ -59,13 +59,10 @@ __device__ __forceinline__ void reduce6(float *v, float *sh /*6*kBoundsThreads*/
 }
 
 // Per shape: bound -> Shape.b, index -> Shape.index, centroid; block partial of the centroid
-// min/max (max starts at FLT_MIN as CLBVH.cpp:92-96 does); the last block to finish folds the
-// partials into scene6 = {min_x, max_x, min_y, max_y, min_z, max_z}.
-// (block `bid` of `nblocks` working on one scene; the kernels below map blocks to scenes)
-__device__ __forceinline__ void bounds_body(Shape *shapes, int n, float *partials, unsigned *done_counter,
-                                            float *scene6, int bid, int nblocks) {
+// min/max (max starts at FLT_MIN as CLBVH.cpp:92-96 does) -> partials[bid].
+// (block `bid` of `nblocks` working on one scene; the kernels map blocks to scenes)
+__device__ __forceinline__ void bounds_body(Shape *shapes, int n, float *partials, int bid, int nblocks) {
     __shared__ float sh[6 * kBoundsThreads];
-    __shared__ bool s_last;
     float acc[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
     for (int i = bid * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x) {
         Shape *s = shapes + i;
@@ -92,26 +89,22 @@ __device__ __forceinline__ void bounds_body(Shape *shapes, int n, float *partial
         if (pz > acc[5]) acc[5] = pz;
     }
     reduce6(acc, sh);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) partials[bid * 6 + k] = acc[k];
-        unsigned prev = add_acq_rel(done_counter, 1u);
-        s_last = (prev == (unsigned)nblocks - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+    if (threadIdx.x < 6) partials[bid * 6 + threadIdx.x] = acc[threadIdx.x];
+}
+
+// One block folds a scene's block partials into scene6 = {min_x, max_x, min_y, max_y, min_z, max_z}.
+// A separate launch rather than a last-block hand-off: on gfx950 a device-scope release per block
+// writes back the L2, which holds every Shape.b just written.
+__device__ __forceinline__ void bounds_final_body(const float *partials, int nblocks, float *scene6) {
+    __shared__ float sh[6 * kBoundsThreads];
     float fin[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
-    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < 6; k++) {
-            float v = __hip_atomic_load(partials + b * 6 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const float v = partials[b * 6 + k];
             fin[k] = (k & 1) ? sel_max(fin[k], v) : sel_min(fin[k], v);
         }
-    }
     reduce6(fin, sh);
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < 6; k++) scene6[k] = fin[k];
-        *done_counter = 0;   // ready for the next build
-    }
+    if (threadIdx.x < 6) scene6[threadIdx.x] = fin[threadIdx.x];
 }
 
 __device__ __forceinline__ unsigned f2u_code(float v) { return v >= 0.f ? (unsigned)v : 0u; }

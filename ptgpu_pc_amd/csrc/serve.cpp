@@ -9,13 +9,14 @@
 //     frame accumulates over ticks and until the caller takes the result;
 //   * the queue, in arrival order.
 // One tick retires the batches that have finished, admits queued requests while their footprint fits
-// the HBM budget (head of line first), builds the scenes they bring in, and launches one batched render
-// in which every admitted request that is not already in flight advances by min(spp_per_tick,
-// remaining) samples.  Two batches may be in flight on two streams, so a new batch fills the GPU while
-// the previous one drains; a request is in at most one of them.  Splitting a frame's samples over
-// launches gives the same bits as one launch (the running mean and the RNG state carry over in
-// colors/seeds), so every request's result equals rendering it alone.  Frame buffers of taken results
-// go to a pool for later requests (freeing device memory would wait for the batches in flight).
+// the HBM budget (head of line first), builds the scenes they bring in (all of them in one batched
+// build), and launches one batched render in which every admitted request that is not already in
+// flight advances by min(spp_per_tick, remaining) samples.  Two batches may be in flight on two
+// streams, so a new batch fills the GPU while the previous one drains; a request is in at most one of
+// them.  Splitting a frame's samples over launches gives the same bits as one launch (the running
+// mean and the RNG state carry over in colors/seeds), so every request's result equals rendering it
+// alone.  Frame buffers of taken results go to a pool for later requests (freeing device memory would
+// wait for the batches in flight).
 #include <algorithm>
 #include <cstddef>
 #include <cstring>
@@ -93,6 +94,7 @@ struct ptgpu_server {
     unsigned long long launched_at[kChannels] = {};   // launch sequence number, 0 = idle
     unsigned long long launches = 0;
     std::vector<ptgpu::DevBuf> pool;              // frame buffers of taken results
+    std::vector<Scene *> unbuilt;                 // admitted this tick, built together after admission
 };
 
 namespace ptgpu {
@@ -175,20 +177,15 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
                                            hipMemcpyHostToDevice, st);
             if (he != hipSuccess) e = hip_fail(he, "hipMemcpyAsync", __FILE__, __LINE__);
         }
-        if (e == PTGPU_OK) e = ptgpu_clbvh_create_device(s->ctx, sc->shapes.as<Shape>(), sc->n, &sc->bvh);
-        if (e == PTGPU_OK) e = ptgpu_clbvh_build_radix_tree(sc->bvh);
-        if (e == PTGPU_OK) e = ptgpu_clbvh_build_bvh_tree(sc->bvh);
-        size_t hb = 0;
-        if (e == PTGPU_OK) e = ptgpu_clbvh_device_bytes(sc->bvh, &hb);
         if (e != PTGPU_OK) {
             release_scene(sc);
             delete sc;
             return e;
         }
-        sc->bytes = sc->shapes.bytes + hb;
+        sc->bytes = sb;   // the estimate until the build reports the real footprint
         s->used += sc->bytes;
         s->scenes[sc->key] = sc;
-        s->builds++;
+        s->unbuilt.push_back(sc);
     }
     Slot *sl = new Slot();
     sl->id = q.id;
@@ -221,6 +218,33 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
     s->active.push_back(sl);
     s->by_id[sl->id] = sl;
     *fitted = true;
+    return PTGPU_OK;
+}
+
+// Builds every scene admitted since the last build in one batched build (one launch per stage for
+// all of them, one wait); their requests are launched from the same tick on.
+static int build_unbuilt(ptgpu_server *s) {
+    if (s->unbuilt.empty()) return PTGPU_OK;
+    const size_t k = s->unbuilt.size();
+    std::vector<Shape *> shapes(k);
+    std::vector<int> counts(k);
+    std::vector<ptgpu_clbvh *> out(k, nullptr);
+    for (size_t i = 0; i < k; i++) {
+        shapes[i] = s->unbuilt[i]->shapes.as<Shape>();
+        counts[i] = s->unbuilt[i]->n;
+    }
+    PT_TRY(ptgpu_clbvh_create_batch(s->ctx, shapes.data(), counts.data(), (int)k, out.data()));
+    for (size_t i = 0; i < k; i++) {
+        Scene *sc = s->unbuilt[i];
+        sc->bvh = out[i];
+        size_t hb = 0;
+        PT_TRY(ptgpu_clbvh_device_bytes(sc->bvh, &hb));
+        const size_t real = sc->shapes.bytes + hb;
+        s->used = s->used - sc->bytes + real;
+        sc->bytes = real;
+        s->builds++;
+    }
+    s->unbuilt.clear();
     return PTGPU_OK;
 }
 
@@ -308,6 +332,7 @@ int ptgpu_server_tick(ptgpu_server *s, int *n_active, int *n_queued) {
         admitted = true;
         s->queue.pop_front();
     }
+    PT_TRY(build_unbuilt(s));   // waits for the context's stream, which also covers the uploads above
     if (admitted) PT_HIP(hipStreamSynchronize(s->ctx->stream));
     // 3. one batch of every admitted request not in flight, on a free channel
     int c = -1;
@@ -320,7 +345,7 @@ int ptgpu_server_tick(ptgpu_server *s, int *n_active, int *n_queued) {
     s->job_slots.clear();
     if (c >= 0) {
         for (Slot *sl : s->active) {
-            if (sl->finished || sl->in_flight) continue;
+            if (sl->finished || sl->in_flight || !sl->scene->bvh) continue;
             const ptgpu_request &r = sl->r;
             const size_t px = (size_t)r.width * (size_t)r.height;
             char *base = static_cast<char *>(sl->frame.p);

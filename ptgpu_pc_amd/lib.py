@@ -57,6 +57,15 @@ class RenderJob(C.Structure):
     _fields_ = [("bvh", vp), ("kd", vp), ("args", RenderArgs)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("blocks_per_cu", i32), ("regroup", i32), ("wide", i32), ("batch_blocks_per_cu", i32),
+                ("batch_regroup", i32), ("digit_bits", i32), ("treelet", i32), ("render_ms", f32), ("build_ms", f32)]
+
+    def as_dict(self) -> dict:
+        return {k: (round(getattr(self, k), 4) if isinstance(getattr(self, k), float) else getattr(self, k))
+                for k, _ in self._fields_}
+
+
 class RadianceArgs(C.Structure):
     _fields_ = [("colors", vp), ("seeds", vp), ("camera", vp), ("shapes", vp), ("shapeCnt", u32),
                 ("lightCnt", u32), ("btn", vp), ("btl", vp), ("kng", vp), ("kngCnt", i32), ("kn", vp),
@@ -116,6 +125,8 @@ _SIGS = {
     "ptgpu_server_stats": [vp, vp, vp, vp, vp],
     "ptgpu_server_destroy": [vp],
     "ptgpu_clbvh_device_bytes": [vp, vp],
+    "ptgpu_autotune": [vp, vp, C.POINTER(RenderArgs), i32, i32, C.POINTER(Tuning)],
+    "ptgpu_autotune_batch": [vp, vp, i32, i32, C.POINTER(Tuning)],
     "ptgpu_trace_closest": [vp, vp, vp, vp, u32, vp, i32, vp, vp],
     "ptgpu_trace_any": [vp, vp, vp, vp, u32, vp, vp, i32, vp],
     "ptgpu_update_camera": [vp],
@@ -485,15 +496,38 @@ def _h(x):
     return x.handle if x is not None else None
 
 
+def _render_args(colors, seeds, camera, shapes, shapeCnt, lightCnt, width, height, currentSample=0, spp=1,
+                 max_depth=6, pixels=None, ray_counter=None, trav_stats=None) -> RenderArgs:
+    return RenderArgs(colors.ptr, seeds.ptr, camera.ptr, shapes.ptr, shapeCnt, lightCnt, width, height, currentSample,
+                      spp, max_depth, pixels.ptr if pixels else None, ray_counter.ptr if ray_counter else None,
+                      trav_stats.ptr if trav_stats else None)
+
+
 def render(ctx: Context, bvh: CLBVH | None, kd: KDTree | None, *, colors: DeviceArray, seeds: DeviceArray,
            camera: DeviceArray, shapes: DeviceArray, shapeCnt: int, lightCnt: int, width: int, height: int,
            currentSample: int = 0, spp: int = 1, max_depth: int = 6, pixels: DeviceArray | None = None,
            ray_counter: DeviceArray | None = None, trav_stats: DeviceArray | None = None) -> None:
     """`spp` RadianceGPU passes in one launch over a prepared BVH / KD tree (or brute force)."""
-    a = RenderArgs(colors.ptr, seeds.ptr, camera.ptr, shapes.ptr, shapeCnt, lightCnt, width, height, currentSample,
-                   spp, max_depth, pixels.ptr if pixels else None, ray_counter.ptr if ray_counter else None,
-                   trav_stats.ptr if trav_stats else None)
+    a = _render_args(colors, seeds, camera, shapes, shapeCnt, lightCnt, width, height, currentSample, spp, max_depth,
+                     pixels, ray_counter, trav_stats)
     _check(ctx.lib.ptgpu_render(ctx.handle, _h(bvh), _h(kd), C.byref(a)))
+
+
+def autotune(ctx: Context, bvh: CLBVH, *, frames_per_build: int = 1, trials: int = 2, **render_kwargs) -> dict:
+    """ptgpu_autotune: digit width, launch shape and treelet size for this scene and frame (the keyword
+    arguments of `render`); applied to ctx and bvh, which is left holding a tree built with them."""
+    a = _render_args(**render_kwargs)
+    t = Tuning()
+    _check(ctx.lib.ptgpu_autotune(ctx.handle, _h(bvh), C.byref(a), int(frames_per_build), int(trials), C.byref(t)))
+    return t.as_dict()
+
+
+def autotune_batch(ctx: Context, jobs: list, trials: int = 2) -> dict:
+    """ptgpu_autotune_batch: launch shape of ptgpu_render_batch on these jobs (dicts as for render_batch)."""
+    arr = _job_table(jobs)
+    t = Tuning()
+    _check(ctx.lib.ptgpu_autotune_batch(ctx.handle, C.cast(arr, vp), len(jobs), int(trials), C.byref(t)))
+    return t.as_dict()
 
 
 class Server:
@@ -573,10 +607,8 @@ class Server:
         self._keep.clear()
 
 
-def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = None) -> None:
-    """Many frames in one launch.  `jobs`: dicts with the keyword arguments of `render` plus exactly
-    one of `bvh` / `kd`; each job's outputs equal its own `render` call."""
-    arr = (RenderJob * len(jobs))()
+def _job_table(jobs: list):
+    arr = (RenderJob * max(len(jobs), 1))()
     for k, j in enumerate(jobs):
         px = j.get("pixels")
         arr[k].bvh = _h(j.get("bvh"))
@@ -584,6 +616,13 @@ def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = Non
         arr[k].args = RenderArgs(j["colors"].ptr, j["seeds"].ptr, j["camera"].ptr, j["shapes"].ptr, j["shapeCnt"],
                                  j["lightCnt"], j["width"], j["height"], j.get("currentSample", 0), j.get("spp", 1),
                                  j.get("max_depth", 6), px.ptr if px else None, None, None)
+    return arr
+
+
+def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = None) -> None:
+    """Many frames in one launch.  `jobs`: dicts with the keyword arguments of `render` plus exactly
+    one of `bvh` / `kd`; each job's outputs equal its own `render` call."""
+    arr = _job_table(jobs)
     _check(ctx.lib.ptgpu_render_batch(ctx.handle, C.cast(arr, vp) if jobs else None, len(jobs),
                                       ray_counter.ptr if ray_counter else None))
 
