@@ -159,10 +159,54 @@ def profile_summary(d: Path, out_md: Path) -> None:
         for k in sorted(sq):
             lines.append(f"* {k}: {sq[k]:.4g}")
     out_md.write_text("\n".join(lines) + "\n")
+    ktb = next(d.glob("ktb/**/*results.db"), None)
+    if ktb is not None:
+        batch_build_summary(ktb, cmds, ROOT / "profiles" / "lbvh_batch_build_64scenes.md")
     TRAFFIC_FILE.write_text(json.dumps({"kernel": RENDER_KERNEL, "workload": WORKLOAD, "traffic_bytes": traffic,
                                         "fetch_kib": f_k, "write_kib": w_k, "fetch_factor": f_scale,
                                         "write_factor": w_scale, "avg_ms_rocprof": render_avg,
                                         "summary": out_md.name}, indent=1) + "\n")
+    print(out_md.read_text())
+
+
+# stage of each batched-build kernel, for the per-stage algorithmic GB/s of the 64-scene build
+BATCH_STAGE = {"kb_fill": "topology", "kb_bounds": "morton", "kb_bounds_final": "morton", "kb_morton": "morton",
+               "kb_karras": "topology", "kb_sort_hist": "sort", "kb_sort_scan": "sort", "kb_sort_pass": "sort",
+               "kb_refit": "refit", "kb_karras_morton": "refit", "kb_mrefit_chunk": "refit", "kb_mrefit_top": "refit",
+               "kb_gather": "refit"}
+
+
+def batch_build_summary(db: Path, cmds: list, out_md: Path) -> None:
+    """Per-kernel and per-stage times of the batched 64-scene build (bench.py --batch under rocprofv3)."""
+    rows = _db_rows(db, "select name, total_calls, average from top_kernels where name like '%kb_%'")
+    n_total = sum(len(w["shapes"]) for w in batch_workload())
+    steps = None
+    per_stage, table = {}, []
+    for name, calls, avg_us in rows:
+        m = re.search(r"(kb_\w+)(<[^>(]*>)?", name)
+        short = m.group(1) + (m.group(2) or "")
+        if steps is None:
+            steps = _db_rows(db, "select total_calls from top_kernels where name like '%kb_gather%'")[0][0]
+        ms = avg_us / 1e3 * calls / steps          # per batched build
+        stage = BATCH_STAGE[m.group(1)]
+        per_stage[stage] = per_stage.get(stage, 0.0) + ms
+        table.append((short, calls // steps, avg_us / 1e3, ms))
+    total = sum(per_stage.values())
+    lines = ["# Batched LBVH build — 64 scenes, one launch per stage, MI355X", "",
+             f"Workload: the 64-scene set of `bench.py --batch` ({n_total:,} shapes; triangles log-uniform in "
+             "[1e3, 1e6], soup or displaced icosphere by scene seed), every stage of every scene's build in one "
+             "launch.", "", "Command (run from the repository root):", "",
+             "    " + next(c for c in cmds if "--batch" in c), "",
+             "| kernel | launches per build | average ms | ms per build |", "|---|---|---|---|"]
+    for short, k, avg, ms in sorted(table, key=lambda r: -r[3]):
+        lines.append(f"| `{short}` | {k} | {avg:.4f} | {ms:.4f} |")
+    lines += ["", f"Total kernel time per batched build: {total:.3f} ms = {n_total / total / 1e3:.0f} Mtri/s.", "",
+              "| stage | ms | algorithmic B/prim | achieved GB/s | fraction of 8 TB/s |", "|---|---|---|---|---|"]
+    for st in ("morton", "sort", "topology", "refit"):
+        ms = per_stage.get(st, 0.0)
+        gbs = STAGE_BYTES[st] * n_total / (ms * 1e-3) / 1e9 if ms else 0.0
+        lines.append(f"| {st} | {ms:.3f} | {STAGE_BYTES[st]} | {gbs:.0f} | {gbs / HBM_PEAK_GBS:.3f} |")
+    out_md.write_text("\n".join(lines) + "\n")
     print(out_md.read_text())
 
 
