@@ -517,6 +517,77 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: boo
                         "frame rendered per step"}
 
 
+def graph_compare(frames: int = 200, sizes=(128, 256), spp: int = 1):
+    """Per-frame sequence of an interactive viewer on the C2 scene (camera upload from pinned memory,
+    seed reset, colour clear, render, pixel readback, wait), issued call by call against replayed as
+    one recorded HIP graph (ptgpu_graph_*)."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import CAMERA_DTYPE, default_seeds
+    shapes, lc, _ = scenes.soup(N_TRI, seed=1)
+    ctx = pt.Context(0)
+    d_sh = ctx.to_device(shapes)
+    bvh = pt.CLBVH(ctx, d_sh, len(shapes))
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    out = {}
+    for res in sizes:
+        cams = []
+        for k in range(frames):
+            c = scenes.orbit_camera(2 * np.pi * k / frames, res, res)
+            pt.update_camera(c)
+            cams.append(c)
+        h_cam = ctx.pinned(CAMERA_DTYPE.itemsize, CAMERA_DTYPE)
+        h_pix = ctx.pinned(res * res * 4, np.uint32)
+        d_cam = ctx.alloc(CAMERA_DTYPE.itemsize, CAMERA_DTYPE)
+        d_seeds0 = ctx.to_device(default_seeds(res, res))
+        d_seeds = ctx.alloc(res * res * 8, np.uint32)
+        d_col = ctx.alloc(res * res * 12, np.float32)
+        d_pix = ctx.alloc(res * res * 4, np.uint32)
+
+        def frame():
+            ctx.memcpy_async(d_cam, h_cam, CAMERA_DTYPE.itemsize)
+            ctx.memcpy_async(d_seeds, d_seeds0, res * res * 8)
+            d_col.zero()
+            pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=len(shapes),
+                      lightCnt=lc, width=res, height=res, currentSample=0, spp=spp, max_depth=DEPTH, pixels=d_pix)
+            ctx.memcpy_async(h_pix, d_pix, res * res * 4)
+
+        h_cam.array[:] = cams[0]
+        frame()                      # prepares the handle (height, four-wide nodes, light list)
+        ctx.synchronize()
+        ctx.graph_begin()
+        try:
+            frame()
+        finally:
+            g = ctx.graph_end()
+        times = {}
+        last = {}
+        for mode in ("eager", "graph", "eager", "graph"):
+            t0 = time.perf_counter()
+            for k in range(frames):
+                h_cam.array[:] = cams[k]
+                if mode == "graph":
+                    g.launch()
+                else:
+                    frame()
+                ctx.synchronize()
+            times[mode] = min(times.get(mode, 1e9), time.perf_counter() - t0)
+            last[mode] = h_pix.array.copy()
+        g.close()
+        h_cam.close()
+        h_pix.close()
+        out[f"{res}x{res}"] = {"eager_frames_s": round(frames / times["eager"], 1),
+                               "graph_frames_s": round(frames / times["graph"], 1),
+                               "speedup": round(times["eager"] / times["graph"], 3),
+                               "pixels_identical": last["eager"].tobytes() == last["graph"].tobytes()}
+    bvh.close()
+    ctx.close()
+    return {"frames": frames, "spp": spp, "sizes": out,
+            "workload": "C2 scene, orbiting camera, per frame: camera upload, seed reset, colour clear, render, "
+                        "pixel readback into pinned memory, wait"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -535,6 +606,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched comparison in the default line")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in defaults instead of autotuning")
+    ap.add_argument("--graph", action="store_true", help="only the recorded-graph vs eager per-frame comparison")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph comparison in the default line")
     ap.add_argument("--tuning", default=None, help="JSON file with the tuning object of an earlier run (no autotune)")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=500.0)
@@ -546,6 +619,9 @@ def main():
     ap.add_argument("--build-sweep", type=str, default=None,
                     help="comma-separated log2 sizes, e.g. 16,18,20,22,24: LBVH build GB/s per stage")
     args = ap.parse_args()
+    if args.graph:
+        print(json.dumps(graph_compare()), flush=True)
+        return
     if args.batch:
         print(json.dumps(batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)), flush=True)
         return
@@ -673,6 +749,10 @@ def main():
         served = serve_compare(args.requests, args.rate, args.pool, args.spp_per_tick, args.budget_gb,
                                args.blocks_per_cu, None)
 
+    graphed = None
+    if not args.no_graph:
+        graphed = graph_compare()
+
     batched = None
     if not args.no_batch:
         batched = batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)
@@ -710,6 +790,7 @@ def main():
         "tuning": tuning,
         "served": served,
         "batched": batched,
+        "graph": graphed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

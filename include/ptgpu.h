@@ -126,7 +126,7 @@ int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);
 int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr);
 int ptgpu_memcpy_h2d(ptgpu_ctx *ctx, void *d_dst, const void *h_src, size_t bytes); /* synchronous */
 int ptgpu_memcpy_d2h(ptgpu_ctx *ctx, void *h_dst, const void *d_src, size_t bytes); /* synchronous */
-int ptgpu_memset_d(ptgpu_ctx *ctx, void *d_dst, int value, size_t bytes);          /* stream-ordered */
+int ptgpu_memset_d(ptgpu_ctx *ctx, void *d_dst, int value, size_t bytes);          /* stream-ordered (a fill kernel) */
 int ptgpu_memcpy_d2d(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t bytes); /* stream-ordered */
 /* Host buffers returned by ptgpu_kdtree_get_trees are released with this. */
 void ptgpu_host_free(void *h_ptr);
@@ -284,6 +284,27 @@ typedef struct {
     ptgpu_render_args args;
 } ptgpu_render_job;
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter);
+
+/* ---- HIP graphs of a per-frame call sequence (no reference counterpart) ----
+ * Between ptgpu_graph_begin and ptgpu_graph_end the library calls on ctx are recorded into a HIP
+ * graph instead of run; ptgpu_graph_launch replays the recording as one submission on ctx's stream.
+ * A replay re-runs the recorded kernels and copies with their recorded arguments: inputs that change
+ * per frame (camera, seeds) go through device buffers that the recorded sequence fills, e.g. with
+ * ptgpu_memcpy_async from pinned memory the caller rewrites between replays (once the previous
+ * replay has finished).  Recordable: ptgpu_render on a prepared handle (built, traversed once, same
+ * lightCnt as before), ptgpu_memcpy_async, ptgpu_memcpy_d2d, ptgpu_memset_d, and the build calls of
+ * an existing handle (whose first traversal afterwards is not recordable).  Calls that wait for the
+ * device return PTGPU_ERR_UNSUPPORTED while recording and record nothing. */
+typedef struct ptgpu_graph ptgpu_graph;
+int ptgpu_graph_begin(ptgpu_ctx *ctx);
+int ptgpu_graph_end(ptgpu_ctx *ctx, ptgpu_graph **out);
+int ptgpu_graph_launch(ptgpu_ctx *ctx, ptgpu_graph *g);
+int ptgpu_graph_destroy(ptgpu_graph *g);
+/* Stream-ordered copy in any direction (host memory must be pinned for a recorded copy). */
+int ptgpu_memcpy_async(ptgpu_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Page-locked host memory (hipHostMalloc). */
+int ptgpu_pinned_alloc(ptgpu_ctx *ctx, size_t bytes, void **h_out);
+int ptgpu_pinned_free(void *h_ptr);
 
 /* ---- Autotuning (no reference counterpart) ----
  * Chooses the settings that change speed only -- every value gives bit-identical images, hits and

@@ -100,6 +100,12 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
         }                                    \
     } while (0)
 
+// Refuses a call that waits for the device while the context's stream is being captured into a graph.
+#define PT_NO_CAPTURE(ctx, what)                                                                       \
+    PT_CHECK(!(ctx)->capturing, PTGPU_ERR_UNSUPPORTED,                                                 \
+             "%s waits for the device and cannot be recorded into a graph (prepare it before "        \
+             "ptgpu_graph_begin)", what)
+
 #define PT_TRY(expr)                  \
     do {                              \
         int _s = (expr);              \
@@ -158,8 +164,9 @@ struct ptgpu_ctx {
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
-    ptgpu::BatchChannel batch;
-    ptgpu::BuildStaging build_staging;    // ptgpu_render_batch launches (on the context's stream)
+    ptgpu::BatchChannel batch;            // ptgpu_render_batch launches (on the context's stream)
+    ptgpu::BuildStaging build_staging;    // batched LBVH builds
+    bool capturing = false;               // between ptgpu_graph_begin and ptgpu_graph_end
 };
 
 namespace ptgpu {

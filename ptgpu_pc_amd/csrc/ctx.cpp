@@ -4,7 +4,7 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "common.hpp"
+#include "device_util.hpp"
 
 namespace ptgpu {
 
@@ -138,6 +138,7 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
 
 int ptgpu_synchronize(ptgpu_ctx *ctx) {
     PT_CHECK(ctx, PTGPU_ERR_INVALID, "ctx is NULL");
+    PT_NO_CAPTURE(ctx, "ptgpu_synchronize");
     PT_HIP(hipStreamSynchronize(ctx->stream));
     return PTGPU_OK;
 }
@@ -191,6 +192,7 @@ int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out) {
 int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr) {
     PT_CHECK(ctx, PTGPU_ERR_INVALID, "ctx is NULL");
     if (!d_ptr) return PTGPU_OK;
+    PT_NO_CAPTURE(ctx, "ptgpu_free");
     PT_HIP(hipStreamSynchronize(ctx->stream));
     PT_HIP(hipFree(d_ptr));
     return PTGPU_OK;
@@ -198,6 +200,7 @@ int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr) {
 
 int ptgpu_memcpy_h2d(ptgpu_ctx *ctx, void *d_dst, const void *h_src, size_t bytes) {
     PT_CHECK(ctx && d_dst && h_src, PTGPU_ERR_INVALID, "null argument");
+    PT_NO_CAPTURE(ctx, "ptgpu_memcpy_h2d (use ptgpu_memcpy_async from pinned memory)");
     PT_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
     PT_HIP(hipStreamSynchronize(ctx->stream));
     return PTGPU_OK;
@@ -205,6 +208,7 @@ int ptgpu_memcpy_h2d(ptgpu_ctx *ctx, void *d_dst, const void *h_src, size_t byte
 
 int ptgpu_memcpy_d2h(ptgpu_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
     PT_CHECK(ctx && h_dst && d_src, PTGPU_ERR_INVALID, "null argument");
+    PT_NO_CAPTURE(ctx, "ptgpu_memcpy_d2h (use ptgpu_memcpy_async into pinned memory)");
     PT_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     PT_HIP(hipStreamSynchronize(ctx->stream));
     return PTGPU_OK;
@@ -212,8 +216,8 @@ int ptgpu_memcpy_d2h(ptgpu_ctx *ctx, void *h_dst, const void *d_src, size_t byte
 
 int ptgpu_memset_d(ptgpu_ctx *ctx, void *d_dst, int value, size_t bytes) {
     PT_CHECK(ctx && d_dst, PTGPU_ERR_INVALID, "null argument");
-    PT_HIP(hipMemsetAsync(d_dst, value, bytes, ctx->stream));
-    return PTGPU_OK;
+    // a fill kernel, not hipMemsetAsync, so that the call can be recorded into a graph
+    return ptgpu::fill_bytes(ctx->stream, d_dst, (unsigned char)value, bytes);
 }
 
 int ptgpu_memcpy_d2d(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {

@@ -45,5 +45,6 @@ __device__ __forceinline__ void st_relaxed(unsigned *p, unsigned v) {
 // Fill kernel (used instead of hipMemsetAsync so that sequences stay graph-capture safe).
 __global__ void k_fill_u32(unsigned *p, unsigned v, size_t n);
 int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words);
+int fill_bytes(hipStream_t s, void *p, unsigned char v, size_t n_bytes);
 
 }  // namespace ptgpu

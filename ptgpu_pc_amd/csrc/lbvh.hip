@@ -22,12 +22,28 @@ __global__ void k_fill_u32(unsigned *p, unsigned v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void k_fill_u8(unsigned char *p, unsigned char v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
 // Plain streaming copy (16-byte vectors, grid-stride): a kernel of known traffic for calibrating
 // the FETCH_SIZE / WRITE_SIZE counters.
 __global__ __launch_bounds__(256) void k_copy_calib(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) dst[i] = src[i];
+}
+
+int fill_bytes(hipStream_t s, void *p, unsigned char v, size_t n_bytes) {
+    if (((uintptr_t)p & 3) == 0 && (n_bytes & 3) == 0) return fill_u32(s, p, 0x01010101u * v, n_bytes / 4);
+    if (n_bytes == 0) return PTGPU_OK;
+    size_t blocks = (n_bytes + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)blocks), dim3(256), 0, s, (unsigned char *)p, v, n_bytes);
+    PT_HIP(hipGetLastError());
+    return PTGPU_OK;
 }
 
 int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words) {
@@ -298,6 +314,7 @@ namespace ptgpu {
 // sized from it, so the first use after a build reads it back (one 4-byte copy).
 int clbvh_prepare_traversal(ptgpu_clbvh *h) {
     if (h->height_pending) {
+        PT_NO_CAPTURE(h->ctx, "the first traversal after a build (it reads the tree height back)");
         unsigned height = 0;
         PT_HIP(hipMemcpyAsync(&height, misc_counters(h) + 11, 4, hipMemcpyDeviceToHost, h->ctx->stream));
         PT_HIP(hipStreamSynchronize(h->ctx->stream));
@@ -583,6 +600,7 @@ int ptgpu_clbvh_stage_times(ptgpu_clbvh *h, float *ms5) {
 
 int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
     if (!h) return PTGPU_OK;
+    if (h->ctx) PT_NO_CAPTURE(h->ctx, "ptgpu_clbvh_destroy");
     if (h->ctx && h->ctx->stream) (void)hipStreamSynchronize(h->ctx->stream);
     h->keysA.release(); h->keysB.release(); h->valsA.release(); h->valsB.release();
     h->btn.release(); h->btl.release(); h->tnodes.release(); h->prims.release();

@@ -361,6 +361,7 @@ int ptgpu_clbvh_build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *handles, int cou
     PT_CHECK(ctx && (handles || count == 0), PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(count >= 0, PTGPU_ERR_INVALID, "negative scene count %d", count);
     if (count == 0) return PTGPU_OK;
+    PT_NO_CAPTURE(ctx, "ptgpu_clbvh_build_batch");
     std::unordered_set<const ptgpu_clbvh *> seen;
     const int bits = handles[0] ? handles[0]->digit_bits : 8;
     for (int s = 0; s < count; s++) {

@@ -745,6 +745,7 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
         PT_TRY(out.ensure(4));
         return PTGPU_OK;
     }
+    PT_NO_CAPTURE(ctx, "building a light list");
     const int nb = (n + kLightChunk - 1) / kLightChunk;
     PT_TRY(scan.ensure(((size_t)nb + 4) * 4));
     PT_TRY(out.ensure((size_t)light_cnt * 4));
@@ -795,6 +796,7 @@ int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeG
 int convert_kd_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU *d_kng, int kngCnt, const int *d_kn,
                    int knCnt, DevBuf &nodes, DevBuf &prims, DevAccel *acc) {
     PT_CHECK(kngCnt >= 2, PTGPU_ERR_INVALID, "kngCnt %d < 2", kngCnt);
+    PT_NO_CAPTURE(ctx, "converting a KD heap");
     PT_TRY(nodes.ensure((size_t)kngCnt * sizeof(TNode)));
     PT_TRY(prims.ensure((size_t)(knCnt > 0 ? knCnt : 1) * sizeof(Prim)));
     const int m = kngCnt > knCnt ? kngCnt : knCnt;
@@ -1034,6 +1036,8 @@ int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu
         }
         lights = bvh->lights.as<int>();
         nl = bvh->nlights;
+    } else if (kd) {
+        PT_TRY(kdtree_lights(kd, a->shapes, (int)a->shapeCnt, (int)a->lightCnt, &lights, &nl));
     } else {
         PT_TRY(build_light_list(ctx, a->shapes, (int)a->shapeCnt, (int)a->lightCnt, ctx->tmp_lights, ctx->tmp_scan,
                                 &nl));
@@ -1061,6 +1065,7 @@ int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter) {
     PT_CHECK(ctx && (jobs || njobs == 0) && njobs >= 0, PTGPU_ERR_INVALID, "null argument");
     if (njobs == 0) return PTGPU_OK;
+    PT_NO_CAPTURE(ctx, "ptgpu_render_batch (its job table is staged through reused pinned memory)");
     PT_HIP(hipSetDevice(ctx->device));
     ctx->batch.stream = ctx->stream;
     return launch_batch(ctx, ctx->batch, jobs, njobs, d_ray_counter);

@@ -126,6 +126,13 @@ _SIGS = {
     "ptgpu_server_destroy": [vp],
     "ptgpu_clbvh_device_bytes": [vp, vp],
     "ptgpu_autotune": [vp, vp, C.POINTER(RenderArgs), i32, i32, C.POINTER(Tuning)],
+    "ptgpu_graph_begin": [vp],
+    "ptgpu_graph_end": [vp, C.POINTER(vp)],
+    "ptgpu_graph_launch": [vp, vp],
+    "ptgpu_graph_destroy": [vp],
+    "ptgpu_memcpy_async": [vp, vp, vp, C.c_size_t],
+    "ptgpu_pinned_alloc": [vp, C.c_size_t, C.POINTER(vp)],
+    "ptgpu_pinned_free": [vp],
     "ptgpu_autotune_batch": [vp, vp, i32, i32, C.POINTER(Tuning)],
     "ptgpu_trace_closest": [vp, vp, vp, vp, u32, vp, i32, vp, vp],
     "ptgpu_trace_any": [vp, vp, vp, vp, u32, vp, vp, i32, vp],
@@ -297,6 +304,22 @@ class Context:
     def copy_calibration(self, dst: DeviceArray, src: DeviceArray, nbytes: int) -> None:
         _check(self.lib.ptgpu_copy_calibration(self.handle, dst.ptr, src.ptr, nbytes))
 
+    def graph_begin(self) -> None:
+        """Start recording this context's library calls into a HIP graph (see ptgpu_graph_begin)."""
+        _check(self.lib.ptgpu_graph_begin(self.handle))
+
+    def graph_end(self) -> "Graph":
+        g = vp()
+        _check(self.lib.ptgpu_graph_end(self.handle, C.byref(g)))
+        return Graph(self, g.value)
+
+    def pinned(self, nbytes: int, dtype=np.uint8) -> "PinnedArray":
+        return PinnedArray(self, nbytes, dtype)
+
+    def memcpy_async(self, dst, src, nbytes: int) -> None:
+        """Stream-ordered copy between DeviceArrays and PinnedArrays (recordable into a graph)."""
+        _check(self.lib.ptgpu_memcpy_async(self.handle, dst.ptr, src.ptr, int(nbytes)))
+
     def record(self, slot: int) -> None:
         _check(self.lib.ptgpu_event_record(self.handle, slot))
 
@@ -304,6 +327,39 @@ class Context:
         ms = f32()
         _check(self.lib.ptgpu_event_elapsed_ms(self.handle, a, b, C.byref(ms)))
         return ms.value
+
+
+class Graph:
+    """A recorded per-frame call sequence (ptgpu_graph_*); launch() replays it on the context's stream."""
+
+    def __init__(self, ctx: "Context", handle):
+        self.ctx, self.handle = ctx, handle
+        ctx._adopt(self)
+
+    def launch(self) -> None:
+        _check(self.ctx.lib.ptgpu_graph_launch(self.ctx.handle, self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.ptgpu_graph_destroy(self.handle)
+        self.handle = None
+
+
+class PinnedArray:
+    """Page-locked host memory (ptgpu_pinned_alloc) with a numpy view, for recordable copies."""
+
+    def __init__(self, ctx: "Context", nbytes: int, dtype=np.uint8):
+        p = vp()
+        _check(ctx.lib.ptgpu_pinned_alloc(ctx.handle, int(nbytes), C.byref(p)))
+        self.ctx, self.ptr, self.nbytes = ctx, p.value, int(nbytes)
+        self.array = np.ctypeslib.as_array((C.c_ubyte * self.nbytes).from_address(self.ptr)).view(dtype)
+        ctx._adopt(self)
+
+    def close(self) -> None:
+        if self.ptr:
+            self.array = None
+            self.ctx.lib.ptgpu_pinned_free(self.ptr)
+        self.ptr = None
 
 
 class CLBVH:
