@@ -13,6 +13,7 @@ Prints one JSON line.
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-serve]
   python bench.py --serve            # only the served comparison
   python bench.py --batch            # only the batched (64-scene) comparison
+  python bench.py --c3               # only the 327K-triangle icosphere, LBVH vs KD trees
   python bench.py --build-sweep 16,20,23
 """
 from __future__ import annotations
@@ -588,6 +589,75 @@ def graph_compare(frames: int = 200, sizes=(128, 256), spp: int = 1):
                         "pixel readback into pinned memory, wait"}
 
 
+def c3_compare(steps: int = 3):
+    """The 327,680-triangle displaced icosphere (seed 3) in a 12-triangle box with an emissive top,
+    1024x768, 16 spp, MAX_DEPTH 6: the LBVH (plain and after treelet-7 restructuring) against the
+    reference's KD tree (mean split) and the SAH KD tree (depth 24).  Every tree renders the same
+    image; per tree: build time and Mrays/s of one fused 16-spp render."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import default_seeds
+    W3, H3, SPP3, D3 = 1024, 768, 16, 6
+    shapes, lc, cam = scenes.icosphere_in_box(7, seed=3, width=W3, height=H3)
+    pt.update_camera(cam)
+    n = len(shapes)
+    ctx = pt.Context(0)
+    d_sh = ctx.to_device(shapes)
+    d_cam = ctx.to_device(cam)
+    seeds0 = ctx.to_device(default_seeds(W3, H3))
+    d_seeds = ctx.alloc(W3 * H3 * 8, np.uint32)
+    d_col = ctx.alloc(W3 * H3 * 12, np.float32)
+    d_pix = ctx.alloc(W3 * H3 * 4, np.uint32)
+    d_rays = ctx.alloc(16, np.uint64).zero()
+
+    def render(bvh, kd):
+        best, rays = 1e9, 0
+        for k in range(steps + 1):
+            d_seeds.copy_from(seeds0)
+            d_rays.zero()
+            ctx.record(0)
+            pt.render(ctx, bvh, kd, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=n, lightCnt=lc,
+                      width=W3, height=H3, currentSample=0, spp=SPP3, max_depth=D3, pixels=d_pix, ray_counter=d_rays)
+            ctx.record(1)
+            ms = ctx.elapsed_ms(0, 1)
+            if k:   # the first render also prepares the tree (height, four-wide nodes, light list)
+                best = min(best, ms)
+            rays = int(d_rays.download(np.uint64)[:2].sum())
+        return best, rays, d_pix.download(np.uint32)
+
+    out = {}
+    bvh = pt.CLBVH(ctx, d_sh, n)
+    for name, treelet in (("lbvh", 0), ("lbvh_treelet7", 7)):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        bvh.reset()
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        if treelet:
+            bvh.optimize(treelet)
+        ctx.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        ms, rays, pix = render(bvh, None)
+        out[name] = {"build_ms": round(build_ms, 2), "render_ms": round(ms, 2), "mrays_s": round(rays / ms / 1e3, 1),
+                     "rays": rays}
+        ref_pix = out.setdefault("_pix", pix)
+        out[name]["pixels_identical"] = pix.tobytes() == ref_pix.tobytes()
+    bvh.close()
+    for name, sah in (("kd_mean_split", False), ("kd_sah", True)):
+        t0 = time.perf_counter()
+        kd = pt.KDTree(ctx, shapes, sah=sah, max_depth=24, max_leaf=4) if sah else pt.KDTree(ctx, shapes)
+        build_ms = (time.perf_counter() - t0) * 1e3
+        ms, rays, pix = render(None, kd)
+        out[name] = {"build_ms_host": round(build_ms, 1), "render_ms": round(ms, 2),
+                     "mrays_s": round(rays / ms / 1e3, 1), "rays": rays,
+                     "pixels_identical": pix.tobytes() == out["_pix"].tobytes(), **kd.stats()}
+        kd.close()
+    out.pop("_pix")
+    ctx.close()
+    return {"workload": "327,680-triangle displaced icosphere (seed 3) + 12-triangle box, 2 emissive, 1024x768, "
+                        "16 spp, MAX_DEPTH 6", "shapes": n, "trees": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -607,6 +677,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in defaults instead of autotuning")
     ap.add_argument("--graph", action="store_true", help="only the recorded-graph vs eager per-frame comparison")
+    ap.add_argument("--c3", action="store_true", help="only the 327K-triangle icosphere: LBVH (+treelet 7) vs KD trees")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph comparison in the default line")
     ap.add_argument("--tuning", default=None, help="JSON file with the tuning object of an earlier run (no autotune)")
     ap.add_argument("--requests", type=int, default=256)
@@ -621,6 +692,9 @@ def main():
     args = ap.parse_args()
     if args.graph:
         print(json.dumps(graph_compare()), flush=True)
+        return
+    if args.c3:
+        print(json.dumps(c3_compare()), flush=True)
         return
     if args.batch:
         print(json.dumps(batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)), flush=True)
