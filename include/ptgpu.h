@@ -191,8 +191,12 @@ int ptgpu_clbvh_set_traversal_topology(ptgpu_clbvh *h, int mode);
 int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12, int *height);
 /* Radix-sort digit width for the next build_bvh_tree (4 or 8 bits; default 8). */
 int ptgpu_clbvh_set_digit_bits(ptgpu_clbvh *h, int bits);
-/* Per-stage device times of the last build in ms: morton, sort, topology, refit, optimize. */
+/* Per-stage device times of the last build in ms: morton, sort, topology, refit, optimize (refit
+ * includes the Morton-keyed traversal tree).  A batched build leaves them at 0. */
 int ptgpu_clbvh_stage_times(ptgpu_clbvh *h, float *ms5);
+/* The same with the refit split: morton, sort, topology, refit of the exported tree, traversal tree,
+ * optimize. */
+int ptgpu_clbvh_stage_times6(ptgpu_clbvh *h, float *ms6);
 int ptgpu_clbvh_destroy(ptgpu_clbvh *h);
 
 /* ------------------------------------------------------------------------------------------ */

@@ -111,15 +111,25 @@ __global__ __launch_bounds__(256) void k_karras_morton(const unsigned *keys, int
 
 __global__ __launch_bounds__(256) void k_mrefit_chunk(int n, const int4 *mnode, const int *mpar_int,
                                                       const int *mpar_leaf, const BVHNodeGPU *btl, TNode *tnodes,
-                                                      float *gbi, int *gh, int2 *xlist, unsigned *xcount,
-                                                      unsigned *root_height) {
-    mrefit_chunk_body(n, mnode, mpar_int, mpar_leaf, btl, tnodes, gbi, gh, xlist, xcount, root_height, blockIdx.x);
+                                                      float *gbi, int *gh, int2 *xlist, unsigned *xcnt,
+                                                      unsigned *root_height, unsigned *err) {
+    mrefit_chunk_body(n, mnode, mpar_int, mpar_leaf, btl, tnodes, gbi, gh, xlist, xcnt, root_height, err, blockIdx.x);
+}
+
+__global__ __launch_bounds__(1024) void k_mrefit_mid(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
+                                                     TNode *tnodes, float *gbi, int *gh, const int2 *xlist,
+                                                     const unsigned *xcnt, unsigned *gcnt, int2 *xtop,
+                                                     unsigned *xtop_count, unsigned xtop_cap, unsigned *root_height,
+                                                     unsigned *err) {
+    mrefit_mid_body(mnode, mpar_int, btl, tnodes, gbi, gh, xlist, xcnt, gcnt, xtop, xtop_count, xtop_cap, root_height,
+                    err, blockIdx.x);
 }
 
 __global__ __launch_bounds__(1024) void k_mrefit_top(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
-                                                     TNode *tnodes, float *gbi, int *gh, const int2 *xlist,
-                                                     const unsigned *xcount, unsigned *gcnt, unsigned *root_height) {
-    mrefit_top_body(mnode, mpar_int, btl, tnodes, gbi, gh, xlist, xcount, gcnt, root_height);
+                                                     TNode *tnodes, float *gbi, int *gh, const int2 *xtop,
+                                                     const unsigned *xtop_count, unsigned xtop_cap, unsigned *gcnt,
+                                                     unsigned *root_height) {
+    mrefit_top_body(mnode, mpar_int, btl, tnodes, gbi, gh, xtop, xtop_count, xtop_cap, gcnt, root_height);
 }
 
 }  // namespace ptgpu
@@ -165,14 +175,20 @@ int clbvh_morton_scratch(ptgpu_clbvh *h, MortonScratch *m) {
     PT_TRY(h->mnode.ensure((N > 1 ? N - 1 : 1) * sizeof(int4)));
     PT_TRY(h->mpar.ensure((2 * N + 2) * sizeof(int)));
     PT_TRY(h->gbi.ensure((N > 1 ? N - 1 : 1) * 6 * sizeof(float) + (N + 2) * sizeof(int)));
-    PT_TRY(h->xlist.ensure((2 * N + 2) * sizeof(int2) + (N + 2) * sizeof(unsigned)));
+    const size_t nsuper = (N + kMSuper - 1) / kMSuper;
+    const size_t xcap = nsuper * kXCap, tcap = mtop_cap((unsigned)nsuper);
+    PT_TRY(h->xlist.ensure((xcap + tcap) * sizeof(int2) + (nsuper + N + 2) * sizeof(unsigned)));
     m->mnode = h->mnode.as<int4>();
     m->mpar_int = h->mpar.as<int>();
     m->mpar_leaf = h->mpar.as<int>() + N;
     m->gbi = h->gbi.as<float>();
     m->gh = reinterpret_cast<int *>(m->gbi + (N > 1 ? N - 1 : 1) * 6);
     m->xlist = h->xlist.as<int2>();
-    m->gcnt = reinterpret_cast<unsigned *>(m->xlist + 2 * N + 2);
+    m->xtop = m->xlist + xcap;
+    m->xtop_cap = (unsigned)tcap;
+    m->xcnt = reinterpret_cast<unsigned *>(m->xtop + tcap);
+    m->gcnt = m->xcnt + nsuper;
+    m->nsuper = (unsigned)nsuper;
     return PTGPU_OK;
 }
 }  // namespace ptgpu
@@ -254,21 +270,25 @@ static int clbvh_traversal_morton(ptgpu_clbvh *h) {
     int *gh = ms.gh;
     int2 *xlist = ms.xlist;
     unsigned *gcnt = ms.gcnt;
-    unsigned *ctr = misc_counters(h);   // [10] crossing count, [11] root height
+    unsigned *ctr = misc_counters(h);   // [9] error word, [10] arrivals for pass 3, [11] root height
     PT_TRY(fill_u32(st, ctr + 10, 0u, 2));
     if (n > 1) {
-        PT_TRY(fill_u32(st, gcnt, 0u, N - 1));
+        PT_TRY(fill_u32(st, ms.xcnt, 0u, ms.nsuper + N - 1));   // per-super-chunk counts, then gcnt
         hipLaunchKernelGGL(k_karras_morton, dim3((n - 1 + 255) / 256), dim3(256), 0, st, h->keysA.as<unsigned>(), n,
                            h->mnode.as<int4>(), mpar_int, mpar_leaf);
         PT_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_mrefit_chunk, dim3((n + kMChunk - 1) / kMChunk), dim3(256), 0, st, n,
                            h->mnode.as<int4>(), mpar_int, mpar_leaf, h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(),
-                           gbi, gh, xlist, ctr + 10, ctr + 11);
+                           gbi, gh, xlist, ms.xcnt, ctr + 11, ctr + 9);
         PT_HIP(hipGetLastError());
         if (n > kMChunk) {
+            hipLaunchKernelGGL(k_mrefit_mid, dim3(ms.nsuper), dim3(1024), 0, st, h->mnode.as<int4>(), mpar_int,
+                               h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(), gbi, gh, (const int2 *)xlist,
+                               (const unsigned *)ms.xcnt, gcnt, ms.xtop, ctr + 10, ms.xtop_cap, ctr + 11, ctr + 9);
+            PT_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_mrefit_top, dim3(1), dim3(1024), 0, st, h->mnode.as<int4>(), mpar_int,
-                               h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(), gbi, gh, xlist, ctr + 10, gcnt,
-                               ctr + 11);
+                               h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(), gbi, gh, (const int2 *)ms.xtop,
+                               (const unsigned *)(ctr + 10), ms.xtop_cap, gcnt, ctr + 11);
             PT_HIP(hipGetLastError());
         }
     }
@@ -315,10 +335,12 @@ namespace ptgpu {
 int clbvh_prepare_traversal(ptgpu_clbvh *h) {
     if (h->height_pending) {
         PT_NO_CAPTURE(h->ctx, "the first traversal after a build (it reads the tree height back)");
-        unsigned height = 0;
-        PT_HIP(hipMemcpyAsync(&height, misc_counters(h) + 11, 4, hipMemcpyDeviceToHost, h->ctx->stream));
+        unsigned w[3] = {0, 0, 0};   // error word, pass-3 arrivals, height
+        PT_HIP(hipMemcpyAsync(w, misc_counters(h) + 9, 12, hipMemcpyDeviceToHost, h->ctx->stream));
         PT_HIP(hipStreamSynchronize(h->ctx->stream));
-        h->acc.depth = (int)height;
+        PT_CHECK(w[0] == 0, PTGPU_ERR_HIP, "LBVH build failed (error word %u: 1 sort look-back spin limit, 2 "
+                 "traversal-tree list overflow)", w[0]);
+        h->acc.depth = (int)w[2];
         h->height_pending = false;
     }
     if (h->wide_pending) {
@@ -521,6 +543,7 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     else PT_TRY(clbvh_sort<8>(h));
     PT_HIP(hipEventRecord(h->ev[5], st));
     PT_TRY(clbvh_refit_levels(h));
+    PT_HIP(hipEventRecord(h->ev[9], st));
     if (h->trav_topology == 1) PT_TRY(clbvh_traversal_morton(h));
     PT_HIP(hipEventRecord(h->ev[6], st));
     h->timed[1] = true;
@@ -551,7 +574,8 @@ int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl
     unsigned err = 0;
     PT_HIP(hipMemcpyAsync(&err, misc_counters(h) + 9, 4, hipMemcpyDeviceToHost, h->ctx->stream));
     PT_HIP(hipStreamSynchronize(h->ctx->stream));
-    PT_CHECK(err == 0, PTGPU_ERR_HIP, "radix sort look-back exceeded its spin limit");
+    PT_CHECK(err == 0, PTGPU_ERR_HIP, "LBVH build failed (error word %u: 1 sort look-back spin limit, 2 "
+             "traversal-tree list overflow)", err);
     *h_btn = h->h_btn.data();
     *h_btl = h->h_btl.data();
     return PTGPU_OK;
@@ -594,6 +618,19 @@ int ptgpu_clbvh_stage_times(ptgpu_clbvh *h, float *ms5) {
         ms5[k] = 0.f;
         if (!h->timed[k]) continue;
         PT_HIP(hipEventElapsedTime(&ms5[k], h->ev[pairs[k][0]], h->ev[pairs[k][1]]));
+    }
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_stage_times6(ptgpu_clbvh *h, float *ms6) {
+    PT_CHECK(h && ms6, PTGPU_ERR_INVALID, "null argument");
+    PT_HIP(hipStreamSynchronize(h->ctx->stream));
+    const int pairs[6][2] = {{0, 1}, {4, 5}, {2, 3}, {5, 9}, {9, 6}, {7, 8}};
+    const int timed[6] = {0, 1, 2, 3, 3, 4};
+    for (int k = 0; k < 6; k++) {
+        ms6[k] = 0.f;
+        if (!h->timed[timed[k]]) continue;
+        PT_HIP(hipEventElapsedTime(&ms6[k], h->ev[pairs[k][0]], h->ev[pairs[k][1]]));
     }
     return PTGPU_OK;
 }

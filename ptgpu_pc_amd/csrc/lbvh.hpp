@@ -15,8 +15,12 @@ struct MortonScratch {
     int *mpar_int, *mpar_leaf;
     float *gbi;      // bounds of internal nodes whose subtree spans 1024-leaf chunks
     int *gh;         // their heights
-    int2 *xlist;     // (node, child) arrivals at chunk-spanning nodes
-    unsigned *gcnt;  // visit counters of the top pass
+    int2 *xlist;     // (node, child) arrivals at chunk-spanning nodes, kXCap per super-chunk
+    unsigned *xcnt;  // their count per super-chunk
+    int2 *xtop;      // arrivals at nodes spanning super-chunks
+    unsigned xtop_cap;
+    unsigned *gcnt;  // visit counters of the passes over chunk-spanning nodes
+    unsigned nsuper; // super-chunks (kMSuper leaves each)
 };
 int clbvh_morton_scratch(ptgpu_clbvh *h, MortonScratch *m);
 // Bound buffers handed between the level passes of the index-keyed refit (two halves of `per` floats).

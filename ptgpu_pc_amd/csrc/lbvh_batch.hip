@@ -36,8 +36,9 @@ struct BuildSeg {
     int *mpar_int, *mpar_leaf;
     float *gbi;
     int *gh;
-    int2 *xlist;
-    unsigned *gcnt;
+    int2 *xlist, *xtop;
+    unsigned *xcnt, *gcnt;
+    unsigned xtop_cap, pad;
     long long lvl_per;
     int n, tiles, levels_total, topo;
 };
@@ -49,7 +50,7 @@ struct FillSeg {
 };
 
 enum Stage { kStBounds, kStMorton, kStKarras, kStHist, kStTiles, kStRefit0, kStRefit1, kStRefit2, kStKMorton,
-             kStMChunk, kStages };
+             kStMChunk, kStMSuper, kStages };
 constexpr int kMaxRefitPasses = 3;   // ceil(30 / kRefitLog) levels for N < 2^30
 constexpr int kFillWords = 4096;     // words cleared per block of the fill launch
 
@@ -153,18 +154,26 @@ __global__ __launch_bounds__(256) void kb_karras_morton(const BuildSeg *g, const
 __global__ __launch_bounds__(256) void kb_mrefit_chunk(const BuildSeg *g, const int *starts, int ns) {
     const int s = seg_find(starts, ns, blockIdx.x);
     const BuildSeg &S = g[s];
-    mrefit_chunk_body(S.n, S.mnode, S.mpar_int, S.mpar_leaf, S.btl, S.tnodes, S.gbi, S.gh, S.xlist, S.ctr + 10,
-                      S.ctr + 11, blockIdx.x - starts[s]);
+    mrefit_chunk_body(S.n, S.mnode, S.mpar_int, S.mpar_leaf, S.btl, S.tnodes, S.gbi, S.gh, S.xlist, S.xcnt,
+                      S.ctr + 11, S.ctr + 9, blockIdx.x - starts[s]);
+}
+
+__global__ __launch_bounds__(1024) void kb_mrefit_mid(const BuildSeg *g, const int *starts, int ns) {
+    const int s = seg_find(starts, ns, blockIdx.x);
+    const BuildSeg &S = g[s];
+    mrefit_mid_body(S.mnode, S.mpar_int, S.btl, S.tnodes, S.gbi, S.gh, S.xlist, S.xcnt, S.gcnt, S.xtop, S.ctr + 10,
+                    S.xtop_cap, S.ctr + 11, S.ctr + 9, (unsigned)(blockIdx.x - starts[s]));
 }
 
 // One workgroup per scene: the chunk-spanning nodes of every scene are refitted concurrently.
 __global__ __launch_bounds__(1024) void kb_mrefit_top(const BuildSeg *g) {
     const BuildSeg &S = g[blockIdx.x];
     if (S.topo != 1 || S.n <= kMChunk) return;
-    mrefit_top_body(S.mnode, S.mpar_int, S.btl, S.tnodes, S.gbi, S.gh, S.xlist, S.ctr + 10, S.gcnt, S.ctr + 11);
+    mrefit_top_body(S.mnode, S.mpar_int, S.btl, S.tnodes, S.gbi, S.gh, S.xtop, S.ctr + 10, S.xtop_cap, S.gcnt,
+                    S.ctr + 11);
 }
 
-// heights[s] = traversal-tree height, errs[s] = sort look-back error flag
+// heights[s] = traversal-tree height, errs[s] = build error word
 __global__ __launch_bounds__(256) void kb_gather(const BuildSeg *g, int ns, unsigned *out) {
     for (int s = threadIdx.x; s < ns; s += 256) {
         out[s] = g[s].ctr[11];
@@ -222,6 +231,9 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
             S.gbi = ms.gbi;
             S.gh = ms.gh;
             S.xlist = ms.xlist;
+            S.xtop = ms.xtop;
+            S.xcnt = ms.xcnt;
+            S.xtop_cap = ms.xtop_cap;
             S.gcnt = ms.gcnt;
         }
         // block counts: the single-scene launches' grids
@@ -240,13 +252,14 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         }
         blocks(kStKMorton, s) = mtree ? (n - 1 + 255) / 256 : 0;
         blocks(kStMChunk, s) = mtree ? (n + kMChunk - 1) / kMChunk : 0;
+        blocks(kStMSuper, s) = mtree && n > kMChunk ? (int)((n + kMSuper - 1) / kMSuper) : 0;
         // clears done by the single-scene path (CLBVH.cpp:83-88 for btn/btl)
         if (n > 1) fills.push_back({(unsigned *)S.btn, (unsigned long long)(n - 1) * sizeof(BVHNodeGPU) / 4, 0u, 0u});
         fills.push_back({(unsigned *)S.btl, (unsigned long long)n * sizeof(BVHNodeGPU) / 4, 0xFFFFFFFFu, 0u});
         fills.push_back({S.hist, (unsigned long long)C::kPasses * C::kRadix, 0u, 0u});
         fills.push_back({S.status, (unsigned long long)C::kPasses * S.tiles * C::kRadix, 0u, 0u});
         fills.push_back({S.ctr + 9, 3ull, 0u, 0u});
-        if (mtree) fills.push_back({S.gcnt, (unsigned long long)(n - 1), 0u, 0u});
+        if (mtree) fills.push_back({S.xcnt, (unsigned long long)(S.gcnt - S.xcnt) + (n - 1), 0u, 0u});
     }
     for (int k = 0; k < kStages; k++)
         for (int s = 0; s < ns; s++) starts[(size_t)k * (ns + 1) + s + 1] += starts[(size_t)k * (ns + 1) + s];
@@ -315,6 +328,7 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
     }
     PT_BATCH_LAUNCH(kb_karras_morton, kStKMorton, 256);
     PT_BATCH_LAUNCH(kb_mrefit_chunk, kStMChunk, 256);
+    PT_BATCH_LAUNCH(kb_mrefit_mid, kStMSuper, 1024);
 #undef PT_BATCH_LAUNCH
     if (total(kStMChunk) > 0) {
         hipLaunchKernelGGL(kb_mrefit_top, dim3(ns), dim3(1024), 0, st, d_segs);
@@ -338,7 +352,8 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         }
         sort_err |= h_out[ns + s] != 0;
     }
-    PT_CHECK(!sort_err, PTGPU_ERR_HIP, "radix sort look-back exceeded its spin limit");
+    PT_CHECK(!sort_err, PTGPU_ERR_HIP, "LBVH build failed (sort look-back spin limit or traversal-tree list "
+             "overflow)");
     return PTGPU_OK;
 }
 

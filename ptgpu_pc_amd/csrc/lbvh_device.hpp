@@ -539,12 +539,22 @@ __device__ __forceinline__ void karras_morton_body(const unsigned *keys, int n, 
 // Pass 1: one workgroup per 1024 consecutive leaves.  Every node whose leaf range lies inside the
 // chunk has its Karras index inside the chunk too, so visit counters and bounds live in LDS and the
 // bottom-up climb synchronises through workgroup-scope LDS atomics only.  A climb that reaches a
-// node spanning chunks publishes its child's bound and height and queues the (node, child) arrival.
+// node spanning chunks publishes its child's bound and height and queues the (node, child) arrival
+// in the list of the 64-chunk super-chunk that holds the node's first leaf.
+//
+// List capacities follow from the tree height h <= 62 (30-bit codes, ties split by a 32-bit index):
+// at most h nodes cross any boundary and each node receives two arrivals, so a super-chunk's list
+// holds at most 2 * 62 * 64 entries and the list of nodes spanning super-chunks 2 * 62 per boundary.
+// An overflow (a malformed tree) sets the build's error word instead of writing out of bounds.
 constexpr int kMChunk = 1024;
+constexpr unsigned kMSuper = 65536;            // leaves per super-chunk (64 chunks)
+constexpr unsigned kXCap = 8192;               // arrivals per super-chunk list
+__host__ __device__ constexpr unsigned mtop_cap(unsigned nsuper) { return 128u * (nsuper + 1u); }
 
 __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, const int *mpar_int, const int *mpar_leaf,
                                                   const BVHNodeGPU *btl, TNode *tnodes, float *gbi, int *gh,
-                                                  int2 *xlist, unsigned *xcount, unsigned *root_height, int bid) {
+                                                  int2 *xlist, unsigned *xcnt, unsigned *root_height, unsigned *err,
+                                                  int bid) {
     __shared__ float lb[6][kMChunk];
     __shared__ float nb[6][kMChunk];
     __shared__ int nh[kMChunk];
@@ -574,8 +584,10 @@ __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, cons
                     for (int f = 0; f < 6; f++) gbi[(size_t)child * 6 + f] = nb[f][c];
                     gh[child] = ch;
                 }
-                const unsigned slot = atomicAdd(xcount, 1u);
-                xlist[slot] = make_int2(p, child);
+                const unsigned sc = (unsigned)P.z / kMSuper;
+                const unsigned slot = atomicAdd(xcnt + sc, 1u);
+                if (slot < kXCap) xlist[(size_t)sc * kXCap + slot] = make_int2(p, child);
+                else atomicOr(err, 2u);
                 break;
             }
             const unsigned old = __hip_atomic_fetch_add(&cnt[p - base], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -610,39 +622,90 @@ __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, cons
     }
 }
 
-// Pass 2: the nodes spanning chunks, one workgroup, arrivals from pass 1's queue.  Child bounds
-// written by pass 1 are visible (earlier launch); those written here by another thread of the
-// same workgroup are ordered by the workgroup-scope acquire/release counter.
+// Passes 2 and 3: child bounds written by an earlier pass are visible (earlier launch); those written
+// by another thread of the same workgroup are ordered by the workgroup-scope acquire/release counter.
+// One node's refit from its two children (the second arrival's work in the passes below): bound and
+// height into gbi / gh, the traversal node into tnodes; returns the height.
+__device__ __forceinline__ int mrefit_node(int p, const int4 &P, const BVHNodeGPU *btl, TNode *tnodes, float *gbi,
+                                           int *gh) {
+    Bound L, R;
+    int hl = 0, hr = 0;
+    if (P.x < 0) L = load_bound(&btl[~P.x].bound);
+    else {
+        const float *f = gbi + (size_t)P.x * 6;
+        L.min_x = f[0]; L.max_x = f[1]; L.min_y = f[2]; L.max_y = f[3]; L.min_z = f[4]; L.max_z = f[5];
+        hl = gh[P.x];
+    }
+    if (P.y < 0) R = load_bound(&btl[~P.y].bound);
+    else {
+        const float *f = gbi + (size_t)P.y * 6;
+        R.min_x = f[0]; R.max_x = f[1]; R.min_y = f[2]; R.max_y = f[3]; R.min_z = f[4]; R.max_z = f[5];
+        hr = gh[P.y];
+    }
+    const Bound M = merge_bounds(L, R);
+    float *o = gbi + (size_t)p * 6;
+    o[0] = M.min_x; o[1] = M.max_x; o[2] = M.min_y; o[3] = M.max_y; o[4] = M.min_z; o[5] = M.max_z;
+    const int hp = (hl > hr ? hl : hr) + 1;
+    gh[p] = hp;
+    tnodes[p] = make_tnode(L, R, P.x, P.y, P.x < 0 ? 1 : 0, P.y < 0 ? 1 : 0);
+    return hp;
+}
+
+// Pass 2: one workgroup per super-chunk climbs from its list's arrivals through the nodes whose leaf
+// range lies inside the super-chunk (their visit counters see no other workgroup); a climb that
+// reaches a node spanning super-chunks queues the arrival for pass 3.
+__device__ __forceinline__ void mrefit_mid_body(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
+                                                TNode *tnodes, float *gbi, int *gh, const int2 *xlist,
+                                                const unsigned *xcnt, unsigned *gcnt, int2 *xtop, unsigned *xtop_count,
+                                                unsigned xtop_cap, unsigned *root_height, unsigned *err, unsigned sc) {
+    const unsigned c = xcnt[sc];
+    const unsigned count = c < kXCap ? c : kXCap;
+    auto defer = [&](int node, int child) {
+        const unsigned slot = atomicAdd(xtop_count, 1u);
+        if (slot < xtop_cap) xtop[slot] = make_int2(node, child);
+        else atomicOr(err, 2u);
+    };
+    for (unsigned e = threadIdx.x; e < count; e += blockDim.x) {
+        const int2 E = xlist[(size_t)sc * kXCap + e];
+        int p = E.x;
+        int4 P = mnode[p];
+        if ((unsigned)P.w / kMSuper != sc) {   // spans super-chunks: pass 3
+            defer(p, E.y);
+            continue;
+        }
+        for (;;) {
+            const unsigned old = __hip_atomic_fetch_add(gcnt + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old == 0) break;
+            const int hp = mrefit_node(p, P, btl, tnodes, gbi, gh);
+            const int pp = mpar_int[p];
+            if (pp < 0) {
+                *root_height = (unsigned)hp;
+                break;
+            }
+            const int4 PP = mnode[pp];
+            if ((unsigned)PP.z / kMSuper != (unsigned)PP.w / kMSuper) {
+                defer(pp, p);
+                break;
+            }
+            p = pp;
+            P = PP;
+        }
+    }
+}
+
+// Pass 3: the nodes spanning super-chunks, one workgroup, arrivals from pass 2's queue.
 __device__ __forceinline__ void mrefit_top_body(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
                                                 TNode *tnodes, float *gbi, int *gh, const int2 *xlist,
-                                                const unsigned *xcount, unsigned *gcnt, unsigned *root_height) {
-    const unsigned count = *xcount;
+                                                const unsigned *xcount, unsigned xcap, unsigned *gcnt,
+                                                unsigned *root_height) {
+    const unsigned c = *xcount;
+    const unsigned count = c < xcap ? c : xcap;
     for (unsigned e = threadIdx.x; e < count; e += blockDim.x) {
         int p = xlist[e].x;
         for (;;) {
             const unsigned old = __hip_atomic_fetch_add(gcnt + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0) break;
-            const int4 P = mnode[p];
-            Bound L, R;
-            int hl = 0, hr = 0;
-            if (P.x < 0) L = load_bound(&btl[~P.x].bound);
-            else {
-                const float *f = gbi + (size_t)P.x * 6;
-                L.min_x = f[0]; L.max_x = f[1]; L.min_y = f[2]; L.max_y = f[3]; L.min_z = f[4]; L.max_z = f[5];
-                hl = gh[P.x];
-            }
-            if (P.y < 0) R = load_bound(&btl[~P.y].bound);
-            else {
-                const float *f = gbi + (size_t)P.y * 6;
-                R.min_x = f[0]; R.max_x = f[1]; R.min_y = f[2]; R.max_y = f[3]; R.min_z = f[4]; R.max_z = f[5];
-                hr = gh[P.y];
-            }
-            const Bound M = merge_bounds(L, R);
-            float *o = gbi + (size_t)p * 6;
-            o[0] = M.min_x; o[1] = M.max_x; o[2] = M.min_y; o[3] = M.max_y; o[4] = M.min_z; o[5] = M.max_z;
-            const int hp = (hl > hr ? hl : hr) + 1;
-            gh[p] = hp;
-            tnodes[p] = make_tnode(L, R, P.x, P.y, P.x < 0 ? 1 : 0, P.y < 0 ? 1 : 0);
+            const int hp = mrefit_node(p, mnode[p], btl, tnodes, gbi, gh);
             const int pp = mpar_int[p];
             if (pp < 0) {
                 *root_height = (unsigned)hp;
@@ -652,6 +715,5 @@ __device__ __forceinline__ void mrefit_top_body(const int4 *mnode, const int *mp
         }
     }
 }
-
 
 }  // namespace ptgpu

@@ -105,6 +105,7 @@ _SIGS = {
     "ptgpu_clbvh_device_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
     "ptgpu_clbvh_device_sorted": [vp, C.POINTER(vp)],
     "ptgpu_clbvh_stage_times": [vp, C.POINTER(f32)],
+    "ptgpu_clbvh_stage_times6": [vp, C.POINTER(f32)],
     "ptgpu_clbvh_destroy": [vp],
     "ptgpu_clbvh_build_batch": [vp, vp, i32],
     "ptgpu_clbvh_create_batch": [vp, vp, vp, i32, vp],
@@ -474,6 +475,12 @@ class CLBVH:
         ms = (f32 * 5)()
         _check(self.ctx.lib.ptgpu_clbvh_stage_times(self.handle, ms))
         return dict(zip(["morton", "sort", "topology", "refit", "optimize"], list(ms)))
+
+    def stage_times6(self) -> dict:
+        """As stage_times, with the refit split into the exported tree's refit and the traversal tree."""
+        ms = (f32 * 6)()
+        _check(self.ctx.lib.ptgpu_clbvh_stage_times6(self.handle, ms))
+        return dict(zip(["morton", "sort", "topology", "refit", "traversal_tree", "optimize"], list(ms)))
 
     def close(self) -> None:
         if self.handle and self.ctx.handle:
