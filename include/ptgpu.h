@@ -289,6 +289,19 @@ typedef struct {
 } ptgpu_render_job;
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter);
 
+/* ---- Scene ingest and image output (host only) ----
+ * ReadTxt (displayfunc.cpp:144-505): reads the reference's text scene (gamma/sky line, camera, materials,
+ * spheres, triangles and OBJ models, the first `objects` entries) into a malloc'ed Shape array (free
+ * with ptgpu_host_free) in the reference's order, with its areas, DIFF/SPEC/REFR mapping and light
+ * count.  Model paths are resolved against base_dir (NULL: the working directory, as the reference
+ * does).  camera gets orig, target, width and height with pitch = yaw = 0; call ptgpu_update_camera.
+ * OBJ numbers are converted as the reference's bundled OBJ reader does, so vertices are bit-identical. */
+int ptgpu_scene_read(const char *path, const char *base_dir, int width, int height, Shape **h_shapes,
+                     unsigned int *shapeCnt, unsigned int *lightCnt, Camera *camera);
+/* The 'p' key's image writer (smallptGPU.cpp:1238-1258): ASCII PPM, top image row first, the r, g, b
+ * bytes of each packed pixel. */
+int ptgpu_write_ppm(const char *path, const unsigned int *h_pixels, int width, int height);
+
 /* ---- HIP graphs of a per-frame call sequence (no reference counterpart) ----
  * Between ptgpu_graph_begin and ptgpu_graph_end the library calls on ctx are recorded into a HIP
  * graph instead of run; ptgpu_graph_launch replays the recording as one submission on ctx's stream.

@@ -127,6 +127,8 @@ _SIGS = {
     "ptgpu_server_destroy": [vp],
     "ptgpu_clbvh_device_bytes": [vp, vp],
     "ptgpu_autotune": [vp, vp, C.POINTER(RenderArgs), i32, i32, C.POINTER(Tuning)],
+    "ptgpu_scene_read": [C.c_char_p, C.c_char_p, i32, i32, C.POINTER(vp), C.POINTER(u32), C.POINTER(u32), vp],
+    "ptgpu_write_ppm": [C.c_char_p, vp, i32, i32],
     "ptgpu_graph_begin": [vp],
     "ptgpu_graph_end": [vp, C.POINTER(vp)],
     "ptgpu_graph_launch": [vp, vp],
@@ -183,6 +185,30 @@ def device_count() -> int:
     c = i32()
     _check(lib.ptgpu_device_count(C.byref(c)))
     return c.value
+
+
+def read_scene(path, base_dir=None, width: int = 640, height: int = 480):
+    """ReadTxt: (shapes SHAPE_DTYPE array, lightCnt, camera CAMERA_DTYPE array with the basis not yet
+    computed -- pass it to update_camera) from a reference text scene; models resolve against base_dir."""
+    lib = load_library()
+    p, n, lc = vp(), u32(), u32()
+    cam = np.zeros(1, dtype=CAMERA_DTYPE)
+    _check(lib.ptgpu_scene_read(str(path).encode(), str(base_dir).encode() if base_dir else None, int(width),
+                                int(height), C.byref(p), C.byref(n), C.byref(lc), _ptr(cam)))
+    try:
+        shapes = np.empty(n.value, dtype=SHAPE_DTYPE)
+        C.memmove(_ptr(shapes), p.value, shapes.nbytes)
+    finally:
+        lib.ptgpu_host_free(p.value)
+    return shapes, lc.value, cam
+
+
+def write_ppm(path, pixels: np.ndarray, width: int, height: int) -> None:
+    """The reference's PPM writer over packed RGBA pixels (width*height uint32)."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint32)
+    if px.size != width * height:
+        raise ValueError("pixels: one uint32 per pixel")
+    _check(load_library().ptgpu_write_ppm(str(path).encode(), _ptr(px), int(width), int(height)))
 
 
 def update_camera(cam: np.ndarray) -> np.ndarray:
