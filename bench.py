@@ -163,6 +163,10 @@ def profile_summary(d: Path, out_md: Path) -> None:
     ktb = next(d.glob("ktb/**/*results.db"), None)
     if ktb is not None:
         batch_build_summary(ktb, cmds, ROOT / "profiles" / "lbvh_batch_build_64scenes.md")
+    ktc = next(d.glob("ktc/**/*results.db"), None)
+    if ktc is not None and (d / "c3.json").exists():
+        c3_summary(ktc, json.loads((d / "c3.json").read_text().strip().splitlines()[-1]), cmds,
+                   ROOT / "profiles" / "c3_icosphere_1024x768x16spp.md")
     TRAFFIC_FILE.write_text(json.dumps({"kernel": RENDER_KERNEL, "workload": WORKLOAD, "traffic_bytes": traffic,
                                         "fetch_kib": f_k, "write_kib": w_k, "fetch_factor": f_scale,
                                         "write_factor": w_scale, "avg_ms_rocprof": render_avg,
@@ -173,7 +177,8 @@ def profile_summary(d: Path, out_md: Path) -> None:
 # stage of each batched-build kernel, for the per-stage algorithmic GB/s of the 64-scene build
 BATCH_STAGE = {"kb_fill": "topology", "kb_bounds": "morton", "kb_bounds_final": "morton", "kb_morton": "morton",
                "kb_karras": "topology", "kb_sort_hist": "sort", "kb_sort_scan": "sort", "kb_sort_pass": "sort",
-               "kb_refit": "refit", "kb_karras_morton": "refit", "kb_mrefit_chunk": "refit", "kb_mrefit_top": "refit",
+               "kb_refit": "refit", "kb_karras_morton": "refit", "kb_mrefit_chunk": "refit", "kb_mrefit_mid": "refit",
+               "kb_mrefit_top": "refit",
                "kb_gather": "refit"}
 
 
@@ -207,6 +212,26 @@ def batch_build_summary(db: Path, cmds: list, out_md: Path) -> None:
         ms = per_stage.get(st, 0.0)
         gbs = STAGE_BYTES[st] * n_total / (ms * 1e-3) / 1e9 if ms else 0.0
         lines.append(f"| {st} | {ms:.3f} | {STAGE_BYTES[st]} | {gbs:.0f} | {gbs / HBM_PEAK_GBS:.3f} |")
+    out_md.write_text("\n".join(lines) + "\n")
+    print(out_md.read_text())
+
+
+def c3_summary(db: Path, c3: dict, cmds: list, out_md: Path) -> None:
+    """The 327K-triangle icosphere comparison (bench.py --c3) with its kernel trace."""
+    rows = _db_rows(db, "select name, total_calls, average, percentage from top_kernels order by total_duration desc "
+                        "limit 10")
+    lines = ["# C3 — 327,680-triangle icosphere in a box, 1024x768, 16 spp, depth 6, MI355X", "",
+             "Commands (run from the repository root):", ""]
+    lines += ["    " + c for c in cmds if "--c3" in c]
+    lines += ["", "| tree | build | render ms | Mrays/s | same pixels |", "|---|---|---|---|---|"]
+    for name, t in c3["trees"].items():
+        build = f"{t['build_ms']} ms (GPU)" if "build_ms" in t else f"{t['build_ms_host']} ms (host)"
+        lines.append(f"| {name} | {build} | {t['render_ms']} | {t['mrays_s']:,} | {t['pixels_identical']} |")
+    lines += ["", f"Autotuner choice for one frame per build: `{json.dumps(c3.get('tuning'))}`", "",
+              "## Kernel trace (rocprofv3 --kernel-trace --stats)", "", "| kernel | calls | average ms | % of GPU time |",
+              "|---|---|---|---|"]
+    for name, calls, avg, pct in rows:
+        lines.append(f"| `{name[:90]}` | {calls} | {avg / 1e3:.4f} | {pct:.2f} |")
     out_md.write_text("\n".join(lines) + "\n")
     print(out_md.read_text())
 
@@ -657,9 +682,18 @@ def c3_compare(steps: int = 3):
                      "pixels_identical": pix.tobytes() == out["_pix"].tobytes(), **kd.stats()}
         kd.close()
     out.pop("_pix")
+    # the autotuner on this scene and frame (one frame per build): digit width, launch shape, treelet size
+    bvh = pt.CLBVH(ctx, d_sh, n)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    d_seeds.copy_from(seeds0)
+    tuning = pt.autotune(ctx, bvh, frames_per_build=1, trials=1, colors=d_col, seeds=d_seeds, camera=d_cam,
+                         shapes=d_sh, shapeCnt=n, lightCnt=lc, width=W3, height=H3, currentSample=0, spp=SPP3,
+                         max_depth=D3, pixels=d_pix)
+    bvh.close()
     ctx.close()
     return {"workload": "327,680-triangle displaced icosphere (seed 3) + 12-triangle box, 2 emissive, 1024x768, "
-                        "16 spp, MAX_DEPTH 6", "shapes": n, "trees": out}
+                        "16 spp, MAX_DEPTH 6", "shapes": n, "trees": out, "tuning": tuning}
 
 
 def main():

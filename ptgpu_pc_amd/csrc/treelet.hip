@@ -11,10 +11,13 @@
 //   * Nodes are processed deepest first, one launch per depth.  Nodes of equal depth own disjoint
 //     subtrees, so their treelets never overlap, and every descendant's cost is final before its
 //     ancestors look at it -- the bottom-up order of the paper without device-scope atomics.
-//   * One wave per treelet root.  Lane 0 grows the treelet; the lanes evaluate the subset
-//     unions and the DP by subset size; lane 0 rebuilds the treelet from the DP choices, reusing
-//     the treelet's own internal nodes.  A treelet is rewritten only when its optimal cost is
-//     lower than its current cost, so the tree's SAH cost never increases.
+//   * Eight treelet roots per wave, eight lanes each.  The group leader grows the treelet; the
+//     lanes evaluate the subset unions and the DP by subset size; the leader rebuilds the treelet
+//     from the DP choices, reusing the treelet's own internal nodes.  A treelet is rewritten only
+//     when its optimal cost is lower than its current cost, so the tree's SAH cost never increases.
+//   * A node record holds its children's boxes and subtree costs, so growing a treelet costs one
+//     dependent fetch per expanded node, and a processed node hands its cost up by writing it into
+//     its parent's record.
 //
 // The traversal layout (TNode, and the four-wide view) is rebuilt from the result; the exported
 // btn/btl arrays are rewritten in the reference layout on first request (leaves renumbered in
@@ -35,46 +38,43 @@ constexpr int kMaxTreelet = 7;
 constexpr int kMaxLevels = 4096;   // trees taller than this are not restructured
 
 struct OptNode {
-    float lo[3], hi[3];
-    float cost;
+    float cb[2][6];   // child boxes (lo xyz, hi xyz)
+    float cc[2];      // child subtree costs: a leaf's from the start, an internal child's once its depth is done
+    int c[2];         // >= 0 internal node, < 0 leaf ~k (k = primitive slot in sorted order)
     int parent;
-    int c0, c1;   // >= 0 internal node, < 0 leaf ~k (k = primitive slot in sorted order)
     int depth;
-    int pad;
+    int pad[2];
 };
-static_assert(sizeof(OptNode) == 48, "OptNode");
+static_assert(sizeof(OptNode) == 80, "OptNode");
+
+__device__ __host__ inline void box_union(const float a[6], const float b[6], float out[6]) {
+    for (int k = 0; k < 3; k++) {
+        out[k] = a[k] < b[k] ? a[k] : b[k];
+        out[3 + k] = a[3 + k] > b[3 + k] ? a[3 + k] : b[3 + k];
+    }
+}
 
 __device__ __forceinline__ float surface_area(const float lo[3], const float hi[3]) {
     const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
     return 2.f * (dx * dy + dy * dz + dz * dx);
 }
 
-// Optimiser state from the traversal layout: child refs, node boxes, parents, leaf boxes/costs.
-__global__ __launch_bounds__(256) void k_opt_init(const TNode *__restrict__ tn, int m, OptNode *on, float *lbox,
-                                                  float *lcost) {
+// Optimiser state from the traversal layout: child refs, child boxes, leaf costs, parents.
+__global__ __launch_bounds__(256) void k_opt_init(const TNode *__restrict__ tn, int m, OptNode *on) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const TNode t = tn[i];
     const float b0[6] = {t.a.x, t.a.y, t.a.z, t.a.w, t.b.x, t.b.y};   // lo xyz, hi xyz
     const float b1[6] = {t.b.z, t.b.w, t.c.x, t.c.y, t.c.z, t.c.w};
     OptNode &o = on[i];
-    for (int a = 0; a < 3; a++) {
-        o.lo[a] = fminf(b0[a], b1[a]);
-        o.hi[a] = fmaxf(b0[3 + a], b1[3 + a]);
-    }
-    o.c0 = t.r.x;
-    o.c1 = t.r.y;
-    o.cost = 0.f;
     if (i == 0) o.parent = -1;
     for (int side = 0; side < 2; side++) {
         const int r = side ? t.r.y : t.r.x;
         const float *b = side ? b1 : b0;
-        if (r < 0) {
-            for (int f = 0; f < 6; f++) lbox[(size_t)~r * 6 + f] = b[f];
-            lcost[~r] = kCt * surface_area(b, b + 3);
-        } else if (r < m) {
-            on[r].parent = i;
-        }
+        for (int f = 0; f < 6; f++) o.cb[side][f] = b[f];
+        o.c[side] = r;
+        o.cc[side] = r < 0 ? kCt * surface_area(b, b + 3) : 0.f;
+        if (r >= 0 && r < m) on[r].parent = i;
     }
 }
 
@@ -108,8 +108,7 @@ __global__ void k_opt_height(const unsigned *max_depth, unsigned *height) { *hei
 constexpr int kGroup = 8;
 constexpr int kGroups = 64 / kGroup;
 
-__global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m, int depth, int tsize, OptNode *on,
-                                                const float *__restrict__ lbox, const float *__restrict__ lcost) {
+__global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m, int depth, int tsize, OptNode *on) {
     __shared__ float s_area[kGroups][1 << kMaxTreelet];
     __shared__ float s_cost[kGroups][1 << kMaxTreelet];
     __shared__ unsigned char s_best[kGroups][1 << kMaxTreelet];
@@ -119,19 +118,11 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
     __shared__ int s_nl[kGroups];
     __shared__ float s_cur[kGroups];
     const int lane = threadIdx.x, grp = lane / kGroup, sub = lane % kGroup;
-
-    auto load_ref = [&](int r, float *b, float *c) {
-        if (r < 0) {
-            for (int f = 0; f < 6; f++) b[f] = lbox[(size_t)~r * 6 + f];
-            *c = lcost[~r];
-        } else {
-            const OptNode &o = on[r];
-            for (int a = 0; a < 3; a++) {
-                b[a] = o.lo[a];
-                b[3 + a] = o.hi[a];
-            }
-            *c = o.cost;
-        }
+    // a treelet leaf j: ref, box and subtree cost, taken from the record of its parent
+    auto put_leaf = [&](int j, int ref, const float *box, float cost) {
+        s_ref[grp][j] = ref;
+        for (int f = 0; f < 6; f++) s_box[grp][j][f] = box[f];
+        s_cost[grp][1 << j] = cost;
     };
 
     for (int base = blockIdx.x * 64; base < m; base += gridDim.x * 64) {
@@ -150,42 +141,37 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
                 hit = h;
             }
             if (root >= 0 && sub == 0) {
-                // grow: replace the treelet leaf of largest area that is an internal node
-                int ref[kMaxTreelet], inl[kMaxTreelet], nl = 2, ni = 1;
+                // grow: replace the treelet leaf of largest area that is an internal node (one fetch:
+                // its record carries its children's boxes and costs)
+                int inl[kMaxTreelet], nl = 2, ni = 1;
                 float ar[kMaxTreelet];
                 inl[0] = root;
                 const OptNode R = on[root];
-                ref[0] = R.c0;
-                ref[1] = R.c1;
-                for (int j = 0; j < 2; j++) ar[j] = ref[j] >= 0 ? surface_area(on[ref[j]].lo, on[ref[j]].hi) : -1.f;
+                for (int j = 0; j < 2; j++) {
+                    put_leaf(j, R.c[j], R.cb[j], R.cc[j]);
+                    ar[j] = R.c[j] >= 0 ? surface_area(R.cb[j], R.cb[j] + 3) : -1.f;
+                }
                 while (nl < tsize) {
                     int best = -1;
                     float best_a = -1.f;
                     for (int j = 0; j < nl; j++)
-                        if (ref[j] >= 0 && ar[j] > best_a) {
+                        if (ar[j] > best_a) {
                             best_a = ar[j];
                             best = j;
                         }
                     if (best < 0) break;
-                    const int node = ref[best];
+                    const int node = s_ref[grp][best];
                     inl[ni++] = node;
                     const OptNode N = on[node];
-                    ref[best] = N.c0;
-                    ref[nl] = N.c1;
-                    ar[best] = N.c0 >= 0 ? surface_area(on[N.c0].lo, on[N.c0].hi) : -1.f;
-                    ar[nl] = N.c1 >= 0 ? surface_area(on[N.c1].lo, on[N.c1].hi) : -1.f;
+                    put_leaf(best, N.c[0], N.cb[0], N.cc[0]);
+                    put_leaf(nl, N.c[1], N.cb[1], N.cc[1]);
+                    ar[best] = N.c[0] >= 0 ? surface_area(N.cb[0], N.cb[0] + 3) : -1.f;
+                    ar[nl] = N.c[1] >= 0 ? surface_area(N.cb[1], N.cb[1] + 3) : -1.f;
                     nl++;
                 }
-                float c0, c1, bx[6];
-                load_ref(R.c0, bx, &c0);
-                load_ref(R.c1, bx, &c1);
-                s_cur[grp] = kCi * surface_area(R.lo, R.hi) + c0 + c1;   // the current shape's cost
-                for (int j = 0; j < nl; j++) {
-                    float c;
-                    load_ref(ref[j], s_box[grp][j], &c);
-                    s_ref[grp][j] = ref[j];
-                    s_cost[grp][1 << j] = c;
-                }
+                float rb[6];
+                box_union(R.cb[0], R.cb[1], rb);
+                s_cur[grp] = kCi * surface_area(rb, rb + 3) + R.cc[0] + R.cc[1];   // the current shape's cost
                 for (int j = 0; j < ni; j++) s_int[grp][j] = inl[j];
                 s_nl[grp] = nl;
             }
@@ -228,9 +214,9 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
                 __syncthreads();
             }
             if (root >= 0 && sub == 0) {
-                if (nl < 3 || !(s_cost[grp][full] < s_cur[grp] * (1.f - 1e-6f))) {
-                    on[root].cost = s_cur[grp];   // one shape only, or no gain: keep it
-                } else {
+                float root_cost = s_cur[grp];
+                if (nl >= 3 && s_cost[grp][full] < s_cur[grp] * (1.f - 1e-6f)) {
+                    root_cost = s_cost[grp][full];
                     // rebuild top-down, reusing the treelet's internal nodes (the root keeps its slot)
                     int st_set[kMaxTreelet], st_node[kMaxTreelet], sp = 0, next = 1;
                     st_set[sp] = full;
@@ -239,38 +225,39 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
                         --sp;
                         const int S = st_set[sp], node = st_node[sp];
                         const int parts[2] = {s_best[grp][S], S ^ s_best[grp][S]};
-                        int cref[2];
+                        OptNode &o = on[node];
                         for (int side = 0; side < 2; side++) {
                             const int X = parts[side];
+                            int cref;
                             if (__popc((unsigned)X) == 1) {
-                                cref[side] = s_ref[grp][__ffs(X) - 1];
+                                cref = s_ref[grp][__ffs(X) - 1];
                             } else {
-                                const int nid = s_int[grp][next++];
-                                cref[side] = nid;
+                                cref = s_int[grp][next++];
                                 st_set[sp] = X;
-                                st_node[sp++] = nid;
+                                st_node[sp++] = cref;
                             }
-                            if (cref[side] >= 0) on[cref[side]].parent = node;
-                        }
-                        OptNode &o = on[node];
-                        o.c0 = cref[0];
-                        o.c1 = cref[1];
-                        float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-                        float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-                        for (int j = 0; j < nl; j++) {
-                            if (!((S >> j) & 1)) continue;
+                            if (cref >= 0) on[cref].parent = node;
+                            float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
+                            float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
+                            for (int j = 0; j < nl; j++) {
+                                if (!((X >> j) & 1)) continue;
+                                for (int a = 0; a < 3; a++) {
+                                    lo[a] = fminf(lo[a], s_box[grp][j][a]);
+                                    hi[a] = fmaxf(hi[a], s_box[grp][j][3 + a]);
+                                }
+                            }
                             for (int a = 0; a < 3; a++) {
-                                lo[a] = fminf(lo[a], s_box[grp][j][a]);
-                                hi[a] = fmaxf(hi[a], s_box[grp][j][3 + a]);
+                                o.cb[side][a] = lo[a];
+                                o.cb[side][3 + a] = hi[a];
                             }
+                            o.c[side] = cref;
+                            o.cc[side] = s_cost[grp][X];
                         }
-                        for (int a = 0; a < 3; a++) {
-                            o.lo[a] = lo[a];
-                            o.hi[a] = hi[a];
-                        }
-                        o.cost = s_cost[grp][S];
                     }
                 }
+                // hand the subtree's cost to the parent's record (siblings write different slots)
+                const int P = on[root].parent;
+                if (P >= 0) on[P].cc[on[P].c[0] == root ? 0 : 1] = root_cost;
             }
             __syncthreads();
         }
@@ -278,31 +265,16 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
 }
 
 // Traversal layout from the optimised tree.
-__global__ __launch_bounds__(256) void k_opt_emit(const OptNode *__restrict__ on, int m, const float *__restrict__ lbox,
-                                                  TNode *tn) {
+__global__ __launch_bounds__(256) void k_opt_emit(const OptNode *__restrict__ on, int m, TNode *tn) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const OptNode &o = on[i];
-    Bound B[2];
-    const int refs[2] = {o.c0, o.c1};
-    for (int side = 0; side < 2; side++) {
-        const int r = refs[side];
-        float f[6];
-        if (r < 0) {
-            for (int q = 0; q < 6; q++) f[q] = lbox[(size_t)~r * 6 + q];
-        } else {
-            const OptNode &c = on[r];
-            f[0] = c.lo[0]; f[1] = c.lo[1]; f[2] = c.lo[2];
-            f[3] = c.hi[0]; f[4] = c.hi[1]; f[5] = c.hi[2];
-        }
-        B[side].min_x = f[0]; B[side].min_y = f[1]; B[side].min_z = f[2];
-        B[side].max_x = f[3]; B[side].max_y = f[4]; B[side].max_z = f[5];
-    }
+    const float *b0 = o.cb[0], *b1 = o.cb[1];   // lo xyz, hi xyz
     TNode t;
-    t.a = make_float4(B[0].min_x, B[0].min_y, B[0].min_z, B[0].max_x);
-    t.b = make_float4(B[0].max_y, B[0].max_z, B[1].min_x, B[1].min_y);
-    t.c = make_float4(B[1].min_z, B[1].max_x, B[1].max_y, B[1].max_z);
-    t.r = make_int4(o.c0, o.c1, o.c0 < 0 ? 1 : 0, o.c1 < 0 ? 1 : 0);
+    t.a = make_float4(b0[0], b0[1], b0[2], b0[3]);
+    t.b = make_float4(b0[4], b0[5], b1[0], b1[1]);
+    t.c = make_float4(b1[2], b1[3], b1[4], b1[5]);
+    t.r = make_int4(o.c[0], o.c[1], o.c[0] < 0 ? 1 : 0, o.c[1] < 0 ? 1 : 0);
     tn[i] = t;
 }
 
@@ -310,22 +282,15 @@ namespace {
 
 struct OptLayout {
     OptNode *nodes;
-    float *lbox, *lcost;
     unsigned *maxd;
 };
 
 int opt_layout(ptgpu_clbvh *h, OptLayout *L) {
     const size_t n = (size_t)h->n, m = n - 1;
-    const size_t bytes = m * sizeof(OptNode) + n * 6 * sizeof(float) + n * sizeof(float) + 256;
-    PT_TRY(h->treelet.ensure(bytes));
+    PT_TRY(h->treelet.ensure(m * sizeof(OptNode) + 256));
     char *p = static_cast<char *>(h->treelet.p);
     L->nodes = reinterpret_cast<OptNode *>(p);
-    p += m * sizeof(OptNode);
-    L->lbox = reinterpret_cast<float *>(p);
-    p += n * 6 * sizeof(float);
-    L->lcost = reinterpret_cast<float *>(p);
-    p += n * sizeof(float);
-    L->maxd = reinterpret_cast<unsigned *>(p);
+    L->maxd = reinterpret_cast<unsigned *>(p + m * sizeof(OptNode));
     return PTGPU_OK;
 }
 
@@ -341,7 +306,7 @@ int clbvh_optimize(ptgpu_clbvh *h, int tsize) {
     OptLayout L;
     PT_TRY(opt_layout(h, &L));
     const dim3 g((m + 255) / 256), b(256);
-    hipLaunchKernelGGL(k_opt_init, g, b, 0, st, h->tnodes.as<TNode>(), m, L.nodes, L.lbox, L.lcost);
+    hipLaunchKernelGGL(k_opt_init, g, b, 0, st, h->tnodes.as<TNode>(), m, L.nodes);
     PT_TRY(fill_u32(st, L.maxd, 0u, 1));
     hipLaunchKernelGGL(k_opt_depth, g, b, 0, st, L.nodes, m, L.maxd);
     PT_HIP(hipGetLastError());
@@ -349,9 +314,9 @@ int clbvh_optimize(ptgpu_clbvh *h, int tsize) {
     int grid = h->ctx->cu_count * 32;
     if (grid > (m + 63) / 64) grid = (m + 63) / 64;
     for (int d = levels - 1; d >= 0; d--)
-        hipLaunchKernelGGL(k_treelet, dim3(grid), dim3(64), 0, st, L.nodes, m, d, tsize, L.nodes, L.lbox, L.lcost);
+        hipLaunchKernelGGL(k_treelet, dim3(grid), dim3(64), 0, st, L.nodes, m, d, tsize, L.nodes);
     PT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_opt_emit, g, b, 0, st, L.nodes, m, L.lbox, h->tnodes.as<TNode>());
+    hipLaunchKernelGGL(k_opt_emit, g, b, 0, st, L.nodes, m, h->tnodes.as<TNode>());
     PT_TRY(fill_u32(st, L.maxd, 0u, 1));
     hipLaunchKernelGGL(k_opt_depth, g, b, 0, st, L.nodes, m, L.maxd);
     hipLaunchKernelGGL(k_opt_height, dim3(1), dim3(1), 0, st, L.maxd, h->height_word());
@@ -371,10 +336,8 @@ int clbvh_export_optimized(ptgpu_clbvh *h) {
     OptLayout L;
     PT_TRY(opt_layout(h, &L));
     std::vector<OptNode> on((size_t)m);
-    std::vector<float> lbox((size_t)n * 6);
     std::vector<unsigned> sorted((size_t)n);
     PT_HIP(hipMemcpyAsync(on.data(), L.nodes, (size_t)m * sizeof(OptNode), hipMemcpyDeviceToHost, st));
-    PT_HIP(hipMemcpyAsync(lbox.data(), L.lbox, (size_t)n * 6 * sizeof(float), hipMemcpyDeviceToHost, st));
     PT_HIP(hipMemcpyAsync(sorted.data(), h->sorted, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost, st));
     PT_HIP(hipStreamSynchronize(st));
     // leaf counts per internal node (children before parents: reverse pre-order)
@@ -385,21 +348,21 @@ int clbvh_export_optimized(ptgpu_clbvh *h) {
         const int v = stack.back();
         stack.pop_back();
         pre.push_back(v);
-        if (on[v].c1 >= 0) stack.push_back(on[v].c1);
-        if (on[v].c0 >= 0) stack.push_back(on[v].c0);
+        if (on[v].c[1] >= 0) stack.push_back(on[v].c[1]);
+        if (on[v].c[0] >= 0) stack.push_back(on[v].c[0]);
     }
     PT_CHECK((int)pre.size() == m, PTGPU_ERR_HIP, "optimised tree is not a tree (%zu of %d nodes reached)",
              pre.size(), m);
     std::vector<int> size((size_t)m, 0);
     for (auto it = pre.rbegin(); it != pre.rend(); ++it) {
         const OptNode &o = on[*it];
-        size[*it] = (o.c0 < 0 ? 1 : size[o.c0]) + (o.c1 < 0 ? 1 : size[o.c1]);
+        size[*it] = (o.c[0] < 0 ? 1 : size[o.c[0]]) + (o.c[1] < 0 ? 1 : size[o.c[1]]);
     }
     // CLBVH.cpp:83-88 conventions: internal nodes zeroed, leaves 0xFF bytes; then the fields
     std::vector<BVHNodeGPU> btn((size_t)m), btl((size_t)n);
     std::memset(static_cast<void *>(btn.data()), 0, btn.size() * sizeof(BVHNodeGPU));
     std::memset(static_cast<void *>(btl.data()), 0xFF, btl.size() * sizeof(BVHNodeGPU));
-    auto set_bound = [](BVHNodeGPU &d, const float lo[3], const float hi[3]) {
+    auto set_bound = [](BVHNodeGPU &d, const float *lo, const float *hi) {
         d.bound.min_x = lo[0]; d.bound.max_x = hi[0];
         d.bound.min_y = lo[1]; d.bound.max_y = hi[1];
         d.bound.min_z = lo[2]; d.bound.max_z = hi[2];
@@ -411,24 +374,25 @@ int clbvh_export_optimized(ptgpu_clbvh *h) {
         work.pop_back();
         const OptNode &o = on[w.node];
         const int b = w.a + size[w.node] - 1;
-        const int s = w.a + (o.c0 < 0 ? 1 : size[o.c0]) - 1;   // last leaf of the left child
+        const int s = w.a + (o.c[0] < 0 ? 1 : size[o.c[0]]) - 1;   // last leaf of the left child
         BVHNodeGPU &d = btn[w.idx];
         d.min = w.a;
         d.max = b;
         d.nLeft = s;
         d.nRight = s + 1;
         d.nParent = w.parent;
-        d.leaf = (o.c0 < 0 || o.c1 < 0) ? 2 : 0;
-        set_bound(d, o.lo, o.hi);
-        const int kids[2] = {o.c0, o.c1};
+        d.leaf = (o.c[0] < 0 || o.c[1] < 0) ? 2 : 0;
+        float nb[6];
+        box_union(o.cb[0], o.cb[1], nb);
+        set_bound(d, nb, nb + 3);
+        const int kids[2] = {o.c[0], o.c[1]};
         const int first[2] = {w.a, s + 1};
         const int slot[2] = {s, s + 1};
         for (int side = 0; side < 2; side++) {
             const int r = kids[side];
             if (r < 0) {
                 BVHNodeGPU &l = btl[slot[side]];
-                const float *f = &lbox[(size_t)~r * 6];
-                set_bound(l, f, f + 3);
+                set_bound(l, o.cb[side], o.cb[side] + 3);
                 l.nShape = (int)sorted[~r];
                 l.nParent = w.idx;
             } else {
