@@ -119,8 +119,9 @@ int ptgpu_ctx_set_max_depth(ptgpu_ctx *ctx, int max_depth);
 int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine);
 /* Launch tuning, no effect on results: "regroup" (1..64, ready lanes that start a shading pass in
  * engine 3, default 16), "blocks_per_cu" (1..64, persistent workgroups per CU, default 2), "wide"
- * (0/1, engine 3 walks four-wide nodes, default 1); "batch_regroup" and "batch_blocks_per_cu" (the
- * same for ptgpu_render_batch, defaults 32 and 3). */
+ * (engine 3's node format: 0 binary, 1 four-wide fp32, 2 four-wide with binary16 boxes rounded
+ * outward, half the bytes per node, where the tree's leaves allow it; default 2);
+ * "batch_regroup" and "batch_blocks_per_cu" (the same for ptgpu_render_batch, defaults 32 and 3). */
 int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value);
 int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);
 int ptgpu_free(ptgpu_ctx *ctx, void *d_ptr);
@@ -339,8 +340,8 @@ typedef struct {
     float render_ms;          /* fastest trial render with the chosen settings */
     float build_ms;           /* fastest rebuild (+ optimise) with the chosen settings */
 } ptgpu_tuning;
-/* Tunes the digit width, the single-frame launch shape (blocks per CU 1..4, regroup 8/16/32, four-
- * wide or binary nodes) and the treelet size (none, 5, 7; minimising rebuild + optimise +
+/* Tunes the digit width, the single-frame launch shape (blocks per CU 1..4, regroup 8/16/32, node
+ * format: wide 2, 1 or 0) and the treelet size (none, 5, 7; minimising rebuild + optimise +
  * frames_per_build renders).  Applies the result to ctx and h and leaves h holding a tree built
  * (and optimised) with it.  args: one ptgpu_render call's arguments on h (h: >= 2 shapes). */
 int ptgpu_autotune(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *args, int frames_per_build,

@@ -57,6 +57,18 @@ static_assert(sizeof(TNode4) == 128, "TNode4");
 constexpr int kEmptyRef = (int)0x80000000;
 constexpr float kFarBox = 1e30f;   // coordinates of an unused TNode4 slot
 
+// The same four-wide node in half the bytes (four 16-byte loads instead of eight): each child box
+// as binary16 bounds rounded outward (lo toward -inf, hi toward +inf), so it contains the fp32 box
+// and every ray that hits the fp32 box hits it too; the walk may visit a few more nodes, never
+// fewer, and the hits are the same.  Child k of one axis in x/y/z component k, lo in the low half.
+// Leaves are runs of primitives ending in kLastInLeaf (trees with leaf_runs only).  An unused slot
+// holds +inf bounds, which no ray hits.
+struct alignas(16) TNode4H {
+    uint4 x, y, z;
+    int4 ref;
+};
+static_assert(sizeof(TNode4H) == 64, "TNode4H");
+
 struct alignas(16) Prim {
     float4 p0;  // v0.xyz | centre.xyz, w = shape index (bits)
     float4 p1;  // e1.xyz | (rad, 0, 0), w = type (bits 0-7) | kLastInLeaf
@@ -71,6 +83,7 @@ constexpr int kLastInLeaf = 0x100;
 struct DevAccel {
     const TNode *nodes = nullptr;
     const TNode4 *nodes4 = nullptr;   // optional four-wide view of `nodes` (same indices)
+    const TNode4H *nodes4h = nullptr; // optional binary16 copy of nodes4 (leaf_runs trees only)
     const Prim *prims = nullptr;
     int root = 0;          // >= 0 internal node; < 0: the whole scene is one leaf ~root
     int root_cnt = 0;      // primitive count when the root is a leaf
@@ -160,7 +173,7 @@ struct ptgpu_ctx {
     int batch_regroup = 32;
     int blocks_per_cu = 2;        // single-frame renders: few waves shorten each pixel's chain
     int batch_blocks_per_cu = 3;  // batched renders: throughput-bound, more waves per SIMD
-    int wide = 1;        // per-lane-state megakernel: walk the four-wide nodes when they exist
+    int wide = 2;        // per-lane-state megakernel: 0 binary nodes, 1 four-wide, 2 four-wide binary16 (when they exist)
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;

@@ -166,7 +166,7 @@ int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value) {
         PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "batch_regroup %d not in [1, 64]", value);
         ctx->batch_regroup = value;
     } else if (!strcmp(name, "wide")) {
-        PT_CHECK(value == 0 || value == 1, PTGPU_ERR_INVALID, "wide %d not in {0, 1}", value);
+        PT_CHECK(value >= 0 && value <= 2, PTGPU_ERR_INVALID, "wide %d not in {0, 1, 2}", value);
         ctx->wide = value;
     } else {
         PT_CHECK(false, PTGPU_ERR_INVALID, "unknown tuning parameter '%s'", name);

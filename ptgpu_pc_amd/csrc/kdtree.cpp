@@ -366,6 +366,7 @@ int upload_device(ptgpu_kdtree *T) {
     }
     A.depth = T->maxdepth;
     A.nodes4 = nullptr;
+    A.nodes4h = nullptr;
     PT_TRY(build_wide(T->ctx, &A, T->d_nodes4));
     return PTGPU_OK;
 }

@@ -387,6 +387,7 @@ void ptgpu::clbvh_mark_built(ptgpu_clbvh *h) {
     h->acc.mode = PTGPU_ACCEL_BVH;
     h->acc.leaf_runs = 1;
     h->acc.nodes4 = nullptr;
+    h->acc.nodes4h = nullptr;
     h->wide_pending = n >= 2;   // four-wide nodes are derived at the first traversal after a build
     if (n >= 2) {
         h->acc.root = 0;
@@ -598,7 +599,7 @@ int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes) {
     size_t t = 0;
     for (const DevBuf *b : bufs) t += b->bytes;
     // the four-wide view is derived at the first traversal: count it before it exists
-    if (!h->tnodes4.p && h->n > 1) t += (size_t)(h->n - 1) * sizeof(TNode4);
+    if (!h->tnodes4.p && h->n > 1) t += (size_t)(h->n - 1) * (sizeof(TNode4) + sizeof(TNode4H));
     *bytes = t;
     return PTGPU_OK;
 }

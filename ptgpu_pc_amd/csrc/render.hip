@@ -212,7 +212,7 @@ enum : int {
     kIdle = 7,    // no work left
 };
 
-template <int STACK, bool STATS, bool WIDE, bool MULTI>
+template <int STACK, bool STATS, bool WIDE, bool MULTI, bool HALF>
 __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
@@ -223,6 +223,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     // the tree of the lane's current query (uniform for a single frame)
     const TNode *t_nodes = P.acc.nodes;
     const TNode4 *t_nodes4 = P.acc.nodes4;
+    const TNode4H *t_nodes4h = P.acc.nodes4h;
     const Prim *t_prims = P.acc.prims;
     int t_nnodes = P.acc.nnodes, t_nprims = P.acc.nprims, t_nshapes = P.acc.nshapes, t_root = P.acc.root;
 
@@ -255,6 +256,24 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         }
         return false;
     };
+    // the primitive run of one leaf (binary16 nodes carry no counts: a run ends at kLastInLeaf)
+    auto leaf_run = [&](int first) -> bool {
+        if (first < 0 || first >= t_nprims) return false;   // malformed caller tree
+        for (int q = first; q < t_nprims; q++) {
+            int id;
+            bool last;
+            const float t = prim_isect_run(t_prims, q, ro, rd, &id, &last);
+            if (STATS) st_prims++;
+            if (qany) {
+                if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; return true; }
+            } else if (t != 0.f && (t < tb || (t == tb && id > bid))) {
+                tb = t;
+                bid = id;
+            }
+            if (last) break;
+        }
+        return false;
+    };
     // start the lane's query along (ro, rd); tb = maxt for any-hit
     auto start = [&](bool any, float maxt) {
         qany = any;
@@ -265,6 +284,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             const DevAccel &A = P.jobs[jb].acc;
             t_nodes = A.nodes;
             t_nodes4 = A.nodes4;
+            t_nodes4h = A.nodes4h;
             t_prims = A.prims;
             t_nnodes = A.nnodes;
             t_nprims = A.nprims;
@@ -332,10 +352,26 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 if ((unsigned)node >= (unsigned)t_nnodes || ++nv > kMaxVisits) {
                     done = true;   // malformed caller tree
                 } else {
-                    const TNode4 *np = t_nodes4 + node;
-                    const float4 lx = np->lox, ly = np->loy, lz = np->loz;
-                    const float4 hx = np->hix, hy = np->hiy, hz = np->hiz;
-                    const int4 rf = np->ref, cn = np->cnt;
+                    float4 lx, ly, lz, hx, hy, hz;
+                    int4 rf, cn;
+                    if (HALF) {
+                        const TNode4H *np = t_nodes4h + node;
+                        const uint4 qx = np->x, qy = np->y, qz = np->z;
+                        rf = np->ref;
+                        cn = make_int4(0, 0, 0, 0);
+                        lx = make_float4(half_lo(qx.x), half_lo(qx.y), half_lo(qx.z), half_lo(qx.w));
+                        hx = make_float4(half_hi(qx.x), half_hi(qx.y), half_hi(qx.z), half_hi(qx.w));
+                        ly = make_float4(half_lo(qy.x), half_lo(qy.y), half_lo(qy.z), half_lo(qy.w));
+                        hy = make_float4(half_hi(qy.x), half_hi(qy.y), half_hi(qy.z), half_hi(qy.w));
+                        lz = make_float4(half_lo(qz.x), half_lo(qz.y), half_lo(qz.z), half_lo(qz.w));
+                        hz = make_float4(half_hi(qz.x), half_hi(qz.y), half_hi(qz.z), half_hi(qz.w));
+                    } else {
+                        const TNode4 *np = t_nodes4 + node;
+                        lx = np->lox; ly = np->loy; lz = np->loz;
+                        hx = np->hix; hy = np->hiy; hz = np->hiz;
+                        rf = np->ref;
+                        cn = np->cnt;
+                    }
                     constexpr float inf = __builtin_inff();
                     // branch-free: unused slots hold a box beyond every query's clip distance
                     float t0 = slab(lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, ro, inv, tb);
@@ -355,8 +391,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         const unsigned k = (unsigned)__builtin_ctz(lm);
                         lm &= lm - 1u;
                         const int lr = k == 0 ? rf.x : k == 1 ? rf.y : k == 2 ? rf.z : rf.w;
-                        const int lc = k == 0 ? cn.x : k == 1 ? cn.y : k == 2 ? cn.z : cn.w;
-                        done = leaf(~lr, lc);
+                        if (HALF) {
+                            done = leaf_run(~lr);
+                        } else {
+                            const int lc = k == 0 ? cn.x : k == 1 ? cn.y : k == 2 ? cn.z : cn.w;
+                            done = leaf(~lr, lc);
+                        }
                     }
                     if (!done) {
                         // inner children: nearest first (order only changes the visit count)
@@ -677,6 +717,55 @@ __global__ __launch_bounds__(256) void k_collapse4(const TNode *__restrict__ bn,
     out[i] = o;
 }
 
+// binary16 bounds rounded outward: the nearest binary16 (round to nearest even), stepped one
+// binary16 ulp down (lo) or up (hi) when it lies on the wrong side of x.  A NaN bound becomes the
+// infinity on its own side; values beyond the binary16 range round to +-65504 or the infinity.
+__device__ __forceinline__ unsigned short half_down(float x) {
+    if (x != x) return 0xFC00u;   // -inf
+    const _Float16 h = (_Float16)x;
+    unsigned short b = __builtin_bit_cast(unsigned short, h);
+    if ((float)h > x) {
+        if (b == 0x0000u || b == 0x8000u) b = 0x8001u;   // +-0 -> the largest negative subnormal
+        else if (b & 0x8000u) b = (unsigned short)(b + 1u);   // negative: away from zero
+        else b = (unsigned short)(b - 1u);                    // positive: toward zero
+    }
+    return b;
+}
+__device__ __forceinline__ unsigned short half_up(float x) {
+    if (x != x) return 0x7C00u;   // +inf
+    const _Float16 h = (_Float16)x;
+    unsigned short b = __builtin_bit_cast(unsigned short, h);
+    if ((float)h < x) {
+        if (b == 0x0000u || b == 0x8000u) b = 0x0001u;   // +-0 -> the smallest positive subnormal
+        else if (b & 0x8000u) b = (unsigned short)(b - 1u);   // negative: toward zero
+        else b = (unsigned short)(b + 1u);                    // positive: away from zero
+    }
+    return b;
+}
+
+// TNode4 -> TNode4H (leaf counts dropped: leaves are runs ending in kLastInLeaf)
+__global__ __launch_bounds__(256) void k_half4(const TNode4 *__restrict__ in, int nnodes, TNode4H *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnodes) return;
+    const TNode4 t = in[i];
+    const float lo[3][4] = {{t.lox.x, t.lox.y, t.lox.z, t.lox.w}, {t.loy.x, t.loy.y, t.loy.z, t.loy.w},
+                            {t.loz.x, t.loz.y, t.loz.z, t.loz.w}};
+    const float hi[3][4] = {{t.hix.x, t.hix.y, t.hix.z, t.hix.w}, {t.hiy.x, t.hiy.y, t.hiy.z, t.hiy.w},
+                            {t.hiz.x, t.hiz.y, t.hiz.z, t.hiz.w}};
+    const int ref[4] = {t.ref.x, t.ref.y, t.ref.z, t.ref.w};
+    unsigned w[3][4];
+    for (int a = 0; a < 3; a++)
+        for (int k = 0; k < 4; k++)
+            w[a][k] = ref[k] == kEmptyRef ? 0x7C007C00u
+                                          : (unsigned)half_down(lo[a][k]) | ((unsigned)half_up(hi[a][k]) << 16);
+    TNode4H o;
+    o.x = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
+    o.y = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
+    o.z = make_uint4(w[2][0], w[2][1], w[2][2], w[2][3]);
+    o.ref = t.ref;
+    out[i] = o;
+}
+
 // reference BVH layout -> TNode/Prim (child is a leaf iff nLeft == min / nRight == max)
 __global__ void k_convert_bvh(const Shape *shapes, int n, const BVHNodeGPU *btn, const BVHNodeGPU *btl,
                               TNode *nodes, Prim *prims) {
@@ -729,12 +818,22 @@ int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_
 
 int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4) {
     acc->nodes4 = nullptr;
+    acc->nodes4h = nullptr;
     if (acc->mode == PTGPU_ACCEL_NONE || acc->root < 0 || acc->nnodes <= 0 || !acc->nodes) return PTGPU_OK;
-    PT_TRY(out_nodes4.ensure((size_t)acc->nnodes * sizeof(TNode4)));
-    hipLaunchKernelGGL(k_collapse4, dim3((acc->nnodes + 255) / 256), dim3(256), 0, ctx->stream, acc->nodes,
-                       acc->nnodes, out_nodes4.as<TNode4>());
+    // one buffer: the fp32 four-wide nodes, then their binary16 copy
+    const size_t n4 = (size_t)acc->nnodes;
+    PT_TRY(out_nodes4.ensure(n4 * (sizeof(TNode4) + sizeof(TNode4H))));
+    TNode4 *w4 = out_nodes4.as<TNode4>();
+    const dim3 grid((unsigned)((n4 + 255) / 256));
+    hipLaunchKernelGGL(k_collapse4, grid, dim3(256), 0, ctx->stream, acc->nodes, acc->nnodes, w4);
     PT_HIP(hipGetLastError());
-    acc->nodes4 = out_nodes4.as<TNode4>();
+    acc->nodes4 = w4;
+    if (acc->leaf_runs) {
+        TNode4H *w4h = reinterpret_cast<TNode4H *>(w4 + n4);
+        hipLaunchKernelGGL(k_half4, grid, dim3(256), 0, ctx->stream, w4, acc->nnodes, w4h);
+        PT_HIP(hipGetLastError());
+        acc->nodes4h = w4h;
+    }
     return PTGPU_OK;
 }
 
@@ -884,25 +983,29 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     // size tried); brute force and single-leaf scenes keep the lockstep megakernel
     if ((ctx->engine == 0 || ctx->engine == 3) && acc.mode != PTGPU_ACCEL_NONE && acc.root >= 0) {
         const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc) : -1;
-#define PT_LANES(ST, STATS_, WIDE_) hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_, false>), dim3(blocks), dim3(256), 0, \
-                                                       ctx->stream, P)
+        const bool half = ctx->wide == 2 && acc.nodes4h;
+#define PT_LANES(ST, STATS_, WIDE_, HALF_)                                                                      \
+    hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_, false, HALF_>), dim3(blocks), dim3(256), 0, ctx->stream, P)
+#define PT_LANES_W(ST, STATS_) \
+    do { if (half) PT_LANES(ST, STATS_, true, true); else PT_LANES(ST, STATS_, true, false); } while (0)
         if (sw > 0) {
             if (L.trav_stats) {
-                if (sw == 32) PT_LANES(32, true, true);
-                else if (sw == 48) PT_LANES(48, true, true);
-                else PT_LANES(64, true, true);
+                if (sw == 32) PT_LANES_W(32, true);
+                else if (sw == 48) PT_LANES_W(48, true);
+                else PT_LANES_W(64, true);
             } else {
-                if (sw == 32) PT_LANES(32, false, true);
-                else if (sw == 48) PT_LANES(48, false, true);
-                else PT_LANES(64, false, true);
+                if (sw == 32) PT_LANES_W(32, false);
+                else if (sw == 48) PT_LANES_W(48, false);
+                else PT_LANES_W(64, false);
             }
         } else if (L.trav_stats) {
-            if (st == 24) PT_LANES(24, true, false);
-            else PT_LANES(64, true, false);
+            if (st == 24) PT_LANES(24, true, false, false);
+            else PT_LANES(64, true, false, false);
         } else {
-            if (st == 24) PT_LANES(24, false, false);
-            else PT_LANES(64, false, false);
+            if (st == 24) PT_LANES(24, false, false, false);
+            else PT_LANES(64, false, false, false);
         }
+#undef PT_LANES_W
 #undef PT_LANES
     } else if (L.trav_stats) {
         if (st == 24) hipLaunchKernelGGL((k_render<24, true>), dim3(blocks), dim3(256), 0, ctx->stream, P);
@@ -1079,6 +1182,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
     std::vector<RenderJob> table((size_t)njobs);
     std::vector<unsigned> begin((size_t)njobs + 1, 0u);
     int st = 0;
+    bool half = ctx->wide == 2;   // binary16 nodes only when every job's tree has them
     unsigned long long total = 0;
     for (int j = 0; j < njobs; j++) {
         const ptgpu_render_job &J = jobs[j];
@@ -1100,6 +1204,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         const int sw = stack_for_wide(R.acc);
         PT_CHECK(sw > 0, PTGPU_ERR_UNSUPPORTED, "job %d: tree height %d exceeds the traversal stack", j, R.acc.depth);
         st = sw > st ? sw : st;
+        half = half && R.acc.nodes4h;
         if (J.bvh) {
             if (J.bvh->lights_for != (int)a->lightCnt) {
                 PT_TRY(build_light_list(ctx, a->shapes, (int)a->shapeCnt, (int)a->lightCnt, J.bvh->lights,
@@ -1168,12 +1273,18 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
     const long long need = ((long long)total + 255) / 256;
     if (blocks > need) blocks = need;
     if (blocks < 1) blocks = 1;
-    if (st == 32) hipLaunchKernelGGL((k_render_lanes<32, false, true, true>), dim3((unsigned)blocks), dim3(256), 0,
-                                     stream, P);
-    else if (st == 48) hipLaunchKernelGGL((k_render_lanes<48, false, true, true>), dim3((unsigned)blocks), dim3(256),
-                                          0, stream, P);
-    else hipLaunchKernelGGL((k_render_lanes<64, false, true, true>), dim3((unsigned)blocks), dim3(256), 0, stream,
-                            P);
+#define PT_BATCH(ST, HALF_) \
+    hipLaunchKernelGGL((k_render_lanes<ST, false, true, true, HALF_>), dim3((unsigned)blocks), dim3(256), 0, stream, P)
+    if (half) {
+        if (st == 32) PT_BATCH(32, true);
+        else if (st == 48) PT_BATCH(48, true);
+        else PT_BATCH(64, true);
+    } else {
+        if (st == 32) PT_BATCH(32, false);
+        else if (st == 48) PT_BATCH(48, false);
+        else PT_BATCH(64, false);
+    }
+#undef PT_BATCH
     PT_HIP(hipGetLastError());
     return PTGPU_OK;
 }

@@ -73,6 +73,22 @@ __device__ __forceinline__ float prim_isect(const Prim *__restrict__ prims, int 
     return tri_isect(p0, p1, p2, o, d);
 }
 
+// prim_isect that also reports whether primitive k ends its leaf run (kLastInLeaf)
+__device__ __forceinline__ float prim_isect_run(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id,
+                                                bool *last) {
+    const float4 p0 = prims[k].p0, p1 = prims[k].p1, p2 = prims[k].p2;
+    *id = __float_as_int(p0.w);
+    *last = (__float_as_int(p1.w) & kLastInLeaf) != 0;
+    if ((__float_as_int(p2.w) & 0xff) == SPHERE) return sph_isect(p0, p1.x, o, d);
+    return tri_isect(p0, p1, p2, o, d);
+}
+
+// the two binary16 halves of a TNode4H word, widened exactly to fp32
+__device__ __forceinline__ float half_lo(unsigned w) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(unsigned w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+
 // slab test of one box; returns entry distance (>= 0) or +inf on a miss
 __device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
                                       float tlimit) {

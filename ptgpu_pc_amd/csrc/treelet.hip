@@ -323,6 +323,7 @@ int clbvh_optimize(ptgpu_clbvh *h, int tsize) {
     PT_HIP(hipGetLastError());
     h->height_pending = true;   // new height read back before the next traversal
     h->acc.nodes4 = nullptr;
+    h->acc.nodes4h = nullptr;
     h->wide_pending = true;
     h->export_pending = true;
     return PTGPU_OK;

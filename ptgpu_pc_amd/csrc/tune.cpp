@@ -5,7 +5,7 @@
 // and frame with HIP events on the context's stream and keeps the fastest setting:
 //   * LBVH sort digit width (4 or 8 bits): full rebuild time;
 //   * single-frame render: workgroups per CU (1..4), shading-pass threshold `regroup` (8, 16, 32),
-//     four-wide or binary nodes;
+//     node format (four-wide binary16, four-wide fp32 or binary);
 //   * treelet restructuring (none, 5 or 7 leaves): build + optimise time plus `frames_per_build`
 //     renders on the resulting tree, the cost a caller pays per rebuild;
 //   * batched renders: workgroups per CU (2..4) and `batch_regroup` (16, 32, 64) on the caller's job list.
@@ -137,7 +137,7 @@ int autotune_frame(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *a, i
     PT_TRY(render());   // first use after a build: derived views and light list
     float best_render = 1e30f;
     int bb = ctx->blocks_per_cu, br = ctx->regroup, bw = ctx->wide;
-    for (int wide : {1, 0})
+    for (int wide : {2, 1, 0})
         for (int bpc : {1, 2, 3, 4})
             for (int rg : {8, 16, 32}) {
                 ctx->wide = wide;

@@ -1,0 +1,95 @@
+"""Four-wide nodes with binary16 boxes rounded outward (tuning "wide" 2).
+
+Each binary16 box contains its fp32 box, so every ray that hits the fp32 box hits the binary16 one;
+the walk may test a few more boxes, and the closest hits, shadow-ray outcomes and so the images are
+the same bits.  Checked against the fp32 four-wide nodes ("wide" 1) on scenes whose coordinates
+land in every binary16 regime: normal numbers, subnormals and signed zeros (a scene straddling 0), beyond the
+binary16 range (scaled by 1e5: bounds become +-65504 or infinities), and coarse ulps far from 0,
+through ptgpu_render and ptgpu_render_batch, on LBVH, treelet-restructured and KD trees.
+"""
+import numpy as np
+import pytest
+
+import ptgpu_pc_amd as pt
+from ptgpu_pc_amd import scenes
+
+from test_gpu_batch import _run
+from test_gpu_parity import render_gpu, say
+
+pytestmark = pytest.mark.gpu
+
+
+def _transform(shapes, cam, scale, shift):
+    """Every position times `scale` plus `shift` (areas by scale^2), camera with it."""
+    s = shapes.copy()
+    for f in ("p1", "p2", "p3"):
+        tri = s["type"] == 1
+        s[f][tri] = s[f][tri] * np.float32(scale) + np.float32(shift)
+    sph = s["type"] == 0
+    s["p"][sph] = s["p"][sph] * np.float32(scale) + np.float32(shift)
+    s["rad"][sph] = s["rad"][sph] * np.float32(scale)
+    s["area"] = s["area"] * np.float32(scale * scale)
+    c = cam.copy()
+    c["orig"] = c["orig"] * np.float32(scale) + np.float32(shift)
+    c["target"] = c["target"] * np.float32(scale) + np.float32(shift)
+    return s, c
+
+
+def _both(ctx, s, lc, cam, w, h, spp, bvh=None, kd=None):
+    d_sh = ctx.to_device(s)
+    out = []
+    for wide in (1, 2):
+        ctx.set_tuning("wide", wide)
+        out.append(render_gpu(ctx, s, lc, cam, w, h, spp, 5, None, bvh=bvh, kd=kd, d_shapes=d_sh)[:3])
+    ctx.set_tuning("wide", 2)
+    return out
+
+
+@pytest.mark.parametrize("scene,scale,shift", [
+    ("soup", 1.0, 0.0), ("soup", 1.0, -0.5), ("soup", 1e5, 0.0), ("soup", 3.0, -1000.0),
+    ("ico", 1.0, 0.0), ("ico", 1e5, -7e4), ("cornell", 1.0, 0.0)])
+def test_half_nodes_same_image(ctx, scene, scale, shift):
+    say("case binary16 nodes", scene, scale, shift)
+    w, h = 48, 40
+    if scene == "soup":
+        shapes, lc, cam = scenes.soup(6000, seed=21, width=w, height=h)
+    elif scene == "ico":
+        shapes, lc, cam = scenes.icosphere_in_box(4, seed=5, width=w, height=h)
+    else:
+        shapes, lc, cam = scenes.cornell(w, h)
+    s, cam = _transform(shapes, cam, scale, shift)
+    pt.update_camera(cam)
+    bvh = pt.CLBVH(ctx, s)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    a, b = _both(ctx, s, lc, cam, w, h, 3, bvh=bvh)
+    for x, y in zip(a, b):
+        assert x.tobytes() == y.tobytes()
+    assert a[0].any()
+    if scene != "cornell":
+        bvh.optimize(5)   # a restructured tree keeps one primitive per leaf
+        a2, b2 = _both(ctx, s, lc, cam, w, h, 3, bvh=bvh)
+        for x, y in zip(a2, b2):
+            assert x.tobytes() == y.tobytes()
+    bvh.close()
+    kd = pt.KDTree(ctx, s, sah=True, max_depth=20, max_leaf=3)   # leaves with several primitives
+    a3, b3 = _both(ctx, s, lc, cam, w, h, 2, kd=kd)
+    for x, y in zip(a3, b3):
+        assert x.tobytes() == y.tobytes()
+    kd.close()
+
+
+def test_half_nodes_batch(ctx):
+    say("case binary16 nodes, batch")
+    from test_gpu_batch import _jobs
+    jobs, cache = _jobs(ctx)
+    res = {}
+    for wide in (1, 2):
+        ctx.set_tuning("wide", wide)
+        res[wide] = _run(ctx, jobs, batch=True)
+    ctx.set_tuning("wide", 2)
+    for a, b in zip(res[1], res[2]):
+        for x, y in zip(a, b):
+            assert x.tobytes() == y.tobytes()
+    for _, bvh, kd, _ in cache.values():
+        (bvh if bvh is not None else kd).close()

@@ -328,7 +328,7 @@ def test_wavefront_equals_megakernel(ctx, scene):
         ctx.set_engine(eng)
         out[eng] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd, d_shapes=d_sh)[:3]
     # per-lane states with few lanes (several pixels per lane) and extreme regroup thresholds
-    for regroup, bpc, wide in ((1, 1, 1), (64, 1, 0), (7, 2, 1), (8, 8, 0)):
+    for regroup, bpc, wide in ((1, 1, 1), (64, 1, 0), (7, 2, 1), (8, 8, 0), (3, 1, 2), (64, 3, 2)):
         ctx.set_tuning("regroup", regroup)
         ctx.set_tuning("blocks_per_cu", bpc)
         ctx.set_tuning("wide", wide)
@@ -336,7 +336,7 @@ def test_wavefront_equals_megakernel(ctx, scene):
                                                        d_shapes=d_sh)[:3]
     ctx.set_tuning("regroup", 16)
     ctx.set_tuning("blocks_per_cu", 2)
-    ctx.set_tuning("wide", 1)
+    ctx.set_tuning("wide", 2)
     ctx.set_engine("auto")
     for eng, res in out.items():
         for a, b in zip(out["megakernel"], res):
