@@ -120,7 +120,9 @@ int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine);
 /* Launch tuning, no effect on results: "regroup" (1..64, ready lanes that start a shading pass in
  * engine 3, default 16), "blocks_per_cu" (1..64, persistent workgroups per CU, default 2), "wide"
  * (engine 3's node format: 0 binary, 1 four-wide fp32, 2 four-wide with binary16 boxes rounded
- * outward, half the bytes per node, where the tree's leaves allow it; default 2);
+ * outward, half the bytes per node, where the tree's leaves allow it; default 2), "quad" (0..16,
+ * with wide 2: when at most this many lanes of a wave are traversing, four lanes share each query,
+ * one child box and leaf each; default 16, 0 off);
  * "batch_regroup" and "batch_blocks_per_cu" (the same for ptgpu_render_batch, defaults 32 and 3). */
 int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value);
 int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);

@@ -60,12 +60,12 @@ constexpr float kFarBox = 1e30f;   // coordinates of an unused TNode4 slot
 // The same four-wide node in half the bytes (four 16-byte loads instead of eight): each child box
 // as binary16 bounds rounded outward (lo toward -inf, hi toward +inf), so it contains the fp32 box
 // and every ray that hits the fp32 box hits it too; the walk may visit a few more nodes, never
-// fewer, and the hits are the same.  Child k of one axis in x/y/z component k, lo in the low half.
-// Leaves are runs of primitives ending in kLastInLeaf (trees with leaf_runs only).  An unused slot
-// holds +inf bounds, which no ray hits.
+// fewer, and the hits are the same.  One 16-byte word per child k: (x, y, z, ref), each axis as
+// lo in the low half and hi in the high half, so one lane can fetch one child in one load.  Leaves
+// are runs of primitives ending in kLastInLeaf (trees with leaf_runs only).  An unused slot holds
+// +inf bounds, which no ray hits.
 struct alignas(16) TNode4H {
-    uint4 x, y, z;
-    int4 ref;
+    uint4 c[4];
 };
 static_assert(sizeof(TNode4H) == 64, "TNode4H");
 
@@ -170,6 +170,7 @@ struct ptgpu_ctx {
     int max_depth = 6;   // MAX_DEPTH (geom.h:32) for ptgpu_radiance_gpu
     int engine = 0;      // 0 auto, 1 megakernel (lockstep queries), 2 wavefront, 3 megakernel (per-lane states)
     int regroup = 16;   // per-lane-state megakernel: ready lanes that trigger a shading pass
+    int quad = 16;      // per-lane-state megakernel, binary16 nodes: traversing lanes at or below which four lanes share a query
     int batch_regroup = 32;
     int blocks_per_cu = 2;        // single-frame renders: few waves shorten each pixel's chain
     int batch_blocks_per_cu = 3;  // batched renders: throughput-bound, more waves per SIMD

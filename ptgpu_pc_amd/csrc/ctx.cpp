@@ -165,6 +165,9 @@ int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value) {
     } else if (!strcmp(name, "batch_regroup")) {
         PT_CHECK(value >= 1 && value <= 64, PTGPU_ERR_INVALID, "batch_regroup %d not in [1, 64]", value);
         ctx->batch_regroup = value;
+    } else if (!strcmp(name, "quad")) {
+        PT_CHECK(value >= 0 && value <= 16, PTGPU_ERR_INVALID, "quad %d not in [0, 16]", value);
+        ctx->quad = value;
     } else if (!strcmp(name, "wide")) {
         PT_CHECK(value >= 0 && value <= 2, PTGPU_ERR_INVALID, "wide %d not in {0, 1, 2}", value);
         ctx->wide = value;

@@ -53,6 +53,7 @@ struct RenderParams {
     unsigned *work_counter;
     int total;
     int regroup;
+    int quad;                     // traversing lanes at or below which four lanes share each query
     const RenderJob *jobs;        // batch launches: job table, pixel prefix sums (njobs + 1)
     const unsigned *job_begin;
     int njobs;
@@ -309,6 +310,158 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         for (;;) {
             const unsigned long long trav = __ballot(st == kTrav);
             if (!trav || (unsigned)__popcll(live & ~trav) >= regroup) break;
+            if (HALF && (unsigned)__popcll(trav) <= (unsigned)P.quad) {
+                // ---- quad phase: a sparse wave lends its idle lanes to the queries in flight.  The
+                // k-th traversing lane's query is walked by lanes 4k..4k+3, lane c of the quad taking
+                // child c of each node: one 16-byte load, one box test and (for a leaf child) one
+                // primitive run per lane, then the quad agrees on the hit, the next node and the
+                // pushes through DPP.  Same nodes, same hits, shorter steps. ----
+                const int qk = (int)(__lane_id() >> 2), qc = (int)(__lane_id() & 3);
+                const int nq = __popcll(trav);
+                int owner = 0;
+                {
+                    unsigned long long m = trav;
+                    for (int j = 0; j < nq; j++) {
+                        const int b = __ffsll((long long)m) - 1;
+                        if (j == qk) owner = b;
+                        m &= m - 1;
+                    }
+                }
+                const bool inq = qk < nq;
+                // the owner's query (every lane runs the shuffles; lanes beyond nq read lane 0)
+                f3 q_o, q_d, q_inv;
+                q_o.x = __shfl(ro.x, owner); q_o.y = __shfl(ro.y, owner); q_o.z = __shfl(ro.z, owner);
+                q_d.x = __shfl(rd.x, owner); q_d.y = __shfl(rd.y, owner); q_d.z = __shfl(rd.z, owner);
+                q_inv.x = __shfl(inv.x, owner); q_inv.y = __shfl(inv.y, owner); q_inv.z = __shfl(inv.z, owner);
+                float q_tb = __shfl(tb, owner);
+                int q_bid = __shfl(bid, owner), q_node = __shfl(node, owner), q_sp = __shfl(sp, owner);
+                unsigned q_nv = (unsigned)__shfl((int)nv, owner);
+                const bool q_any = __shfl((int)qany, owner) != 0;
+                const TNode4H *q_nodes = t_nodes4h;
+                const Prim *q_prims = t_prims;
+                int q_nnodes = t_nnodes, q_nprims = t_nprims, q_nshapes = t_nshapes;
+                if (MULTI) {
+                    auto shp = [&](const void *ptr) -> const void * {
+                        const unsigned long long v = reinterpret_cast<unsigned long long>(ptr);
+                        const unsigned lo = (unsigned)__shfl((int)(unsigned)v, owner);
+                        const unsigned hi = (unsigned)__shfl((int)(unsigned)(v >> 32), owner);
+                        return reinterpret_cast<const void *>(((unsigned long long)hi << 32) | lo);
+                    };
+                    q_nodes = static_cast<const TNode4H *>(shp(t_nodes4h));
+                    q_prims = static_cast<const Prim *>(shp(t_prims));
+                    q_nnodes = __shfl(t_nnodes, owner);
+                    q_nprims = __shfl(t_nprims, owner);
+                    q_nshapes = __shfl(t_nshapes, owner);
+                }
+                int *q_stk = s_stack + (threadIdx.x & ~63u) + owner;   // the owner's stack column
+                bool q_done = !inq;
+                bool q_end = false;   // the query finished (as opposed to the phase ending)
+                constexpr float inf = __builtin_inff();
+                const unsigned ready0 = (unsigned)__popcll(live & ~trav);
+                for (;;) {
+                    const unsigned long long fin = __ballot(inq && q_end && qc == 0);
+                    if (ready0 + (unsigned)__popcll(fin) >= regroup || (unsigned)__popcll(fin) >= (unsigned)nq) break;
+                    if (!q_done) {
+                        if ((unsigned)q_node >= (unsigned)q_nnodes || ++q_nv > kMaxVisits) {
+                            q_done = q_end = true;   // malformed caller tree
+                        } else {
+                            const uint4 w = q_nodes[q_node].c[qc];
+                            const int ref = (int)w.w;
+                            float tc = slab(half_lo(w.x), half_lo(w.y), half_lo(w.z), half_hi(w.x), half_hi(w.y),
+                                            half_hi(w.z), q_o, q_inv, q_tb);
+                            if (STATS) st_nodes += ref != kEmptyRef;
+                            // this lane's leaf child: its primitive run against the step's tb
+                            float bt = q_tb;
+                            int bb = q_bid;
+                            bool occl = false;
+                            if (ref < 0 && tc != inf) {
+                                const int first = ~ref;
+                                if (first >= 0 && first < q_nprims) {
+                                    for (int q = first; q < q_nprims; q++) {
+                                        int id;
+                                        bool last;
+                                        const float t = prim_isect_run(q_prims, q, q_o, q_d, &id, &last);
+                                        if (STATS) st_prims++;
+                                        if (q_any) {
+                                            if (t != 0.f && t < q_tb && id != q_nshapes - 1) { occl = true; break; }
+                                        } else if (t != 0.f && (t < bt || (t == bt && id > bb))) {
+                                            bt = t;
+                                            bb = id;
+                                        }
+                                        if (last) break;
+                                    }
+                                }
+                            }
+                            if (ref < 0) tc = inf;   // only internal children from here on
+                            // the quad's hit: occlusion by any lane; closest by (t, then higher id)
+                            occl = (qxor<1>((int)occl) | qxor<2>((int)occl) | qxor<1>(qxor<2>((int)occl)) |
+                                    (int)occl) != 0;
+                            {
+                                float ot = qxorf<1>(bt);
+                                int oi = qxor<1>(bb);
+                                if (ot < bt || (ot == bt && oi > bb)) { bt = ot; bb = oi; }
+                                ot = qxorf<2>(bt);
+                                oi = qxor<2>(bb);
+                                if (ot < bt || (ot == bt && oi > bb)) { bt = ot; bb = oi; }
+                            }
+                            if (q_any) {
+                                if (occl) {
+                                    q_bid = 1;
+                                    q_done = q_end = true;
+                                }
+                            } else {
+                                q_tb = bt;
+                                q_bid = bb;
+                            }
+                            if (!q_done) {
+                                // internal children: nearest next, the rest pushed farthest first; order
+                                // by (t, slot), which every lane of the quad evaluates alike
+                                const float t1 = qrotf<1>(tc), t2 = qrotf<2>(tc), t3 = qrotf<3>(tc);
+                                const int r1 = qrot<1>(ref), r2 = qrot<2>(ref), r3 = qrot<3>(ref);
+                                const int s1 = (qc + 1) & 3, s2 = (qc + 2) & 3, s3 = (qc + 3) & 3;
+                                const int nh = (tc != inf) + (t1 != inf) + (t2 != inf) + (t3 != inf);
+                                auto before = [](float ta, int sa, float tb_, int sb) {
+                                    return ta < tb_ || (ta == tb_ && sa < sb);
+                                };
+                                const int rank = before(t1, s1, tc, qc) + before(t2, s2, tc, qc) + before(t3, s3, tc, qc);
+                                float nt = tc;
+                                int nr = ref, ns = qc;
+                                if (before(t1, s1, nt, ns)) { nt = t1; nr = r1; ns = s1; }
+                                if (before(t2, s2, nt, ns)) { nt = t2; nr = r2; ns = s2; }
+                                if (before(t3, s3, nt, ns)) { nt = t3; nr = r3; ns = s3; }
+                                if (nh > 0) {
+                                    if (tc != inf && rank > 0) {
+                                        const int pos = q_sp + nh - 1 - rank;
+                                        if (pos < STACK) q_stk[pos * 256] = ref;
+                                    }
+                                    q_sp += nh - 1;
+                                    if (q_sp > STACK) q_sp = STACK;
+                                    q_node = nr;
+                                } else if (q_sp == 0) {
+                                    q_done = q_end = true;
+                                } else {
+                                    q_node = q_stk[(--q_sp) * 256];
+                                }
+                            }
+                        }
+                    }
+                }
+                // hand each query back to its owner: the owner of quad k reads lane 4k
+                const int src = 4 * (int)__popcll(trav & lt);
+                const int b_node = __shfl(q_node, src), b_sp = __shfl(q_sp, src), b_bid = __shfl(q_bid, src);
+                const float b_tb = __shfl(q_tb, src);
+                const unsigned b_nv = (unsigned)__shfl((int)q_nv, src);
+                const bool b_end = __shfl((int)q_end, src) != 0;
+                if (st == kTrav) {
+                    node = b_node;
+                    sp = b_sp;
+                    bid = b_bid;
+                    tb = b_tb;
+                    nv = b_nv;
+                    if (b_end) st = qany ? kADone : kCDone;
+                }
+                continue;
+            }
             if (!WIDE && st == kTrav) {
                 bool done = false;
                 if ((unsigned)node >= (unsigned)t_nnodes || ++nv > kMaxVisits) {
@@ -356,15 +509,15 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     int4 rf, cn;
                     if (HALF) {
                         const TNode4H *np = t_nodes4h + node;
-                        const uint4 qx = np->x, qy = np->y, qz = np->z;
-                        rf = np->ref;
+                        const uint4 q0 = np->c[0], q1 = np->c[1], q2 = np->c[2], q3 = np->c[3];
+                        rf = make_int4((int)q0.w, (int)q1.w, (int)q2.w, (int)q3.w);
                         cn = make_int4(0, 0, 0, 0);
-                        lx = make_float4(half_lo(qx.x), half_lo(qx.y), half_lo(qx.z), half_lo(qx.w));
-                        hx = make_float4(half_hi(qx.x), half_hi(qx.y), half_hi(qx.z), half_hi(qx.w));
-                        ly = make_float4(half_lo(qy.x), half_lo(qy.y), half_lo(qy.z), half_lo(qy.w));
-                        hy = make_float4(half_hi(qy.x), half_hi(qy.y), half_hi(qy.z), half_hi(qy.w));
-                        lz = make_float4(half_lo(qz.x), half_lo(qz.y), half_lo(qz.z), half_lo(qz.w));
-                        hz = make_float4(half_hi(qz.x), half_hi(qz.y), half_hi(qz.z), half_hi(qz.w));
+                        lx = make_float4(half_lo(q0.x), half_lo(q1.x), half_lo(q2.x), half_lo(q3.x));
+                        hx = make_float4(half_hi(q0.x), half_hi(q1.x), half_hi(q2.x), half_hi(q3.x));
+                        ly = make_float4(half_lo(q0.y), half_lo(q1.y), half_lo(q2.y), half_lo(q3.y));
+                        hy = make_float4(half_hi(q0.y), half_hi(q1.y), half_hi(q2.y), half_hi(q3.y));
+                        lz = make_float4(half_lo(q0.z), half_lo(q1.z), half_lo(q2.z), half_lo(q3.z));
+                        hz = make_float4(half_hi(q0.z), half_hi(q1.z), half_hi(q2.z), half_hi(q3.z));
                     } else {
                         const TNode4 *np = t_nodes4 + node;
                         lx = np->lox; ly = np->loy; lz = np->loz;
@@ -759,10 +912,7 @@ __global__ __launch_bounds__(256) void k_half4(const TNode4 *__restrict__ in, in
             w[a][k] = ref[k] == kEmptyRef ? 0x7C007C00u
                                           : (unsigned)half_down(lo[a][k]) | ((unsigned)half_up(hi[a][k]) << 16);
     TNode4H o;
-    o.x = make_uint4(w[0][0], w[0][1], w[0][2], w[0][3]);
-    o.y = make_uint4(w[1][0], w[1][1], w[1][2], w[1][3]);
-    o.z = make_uint4(w[2][0], w[2][1], w[2][2], w[2][3]);
-    o.ref = t.ref;
+    for (int k = 0; k < 4; k++) o.c[k] = make_uint4(w[0][k], w[1][k], w[2][k], (unsigned)ref[k]);
     out[i] = o;
 }
 
@@ -971,6 +1121,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.work_counter = ctx->tmp_counter.as<unsigned>();
     P.total = total;
     P.regroup = ctx->regroup;
+    P.quad = ctx->quad;
     P.jobs = nullptr;
     P.job_begin = nullptr;
     P.njobs = 0;
@@ -1266,6 +1417,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
     P.work_counter = ch.counter.as<unsigned>();
     P.total = (int)total;
     P.regroup = ctx->batch_regroup;
+    P.quad = ctx->quad;
     P.jobs = ch.table.as<RenderJob>();
     P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(ch.table.p) + boff);
     P.njobs = njobs;

@@ -89,6 +89,25 @@ __device__ __forceinline__ float half_lo(unsigned w) {
 }
 __device__ __forceinline__ float half_hi(unsigned w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
 
+// Lane exchange inside a quad (four consecutive lanes) with DPP quad_perm; every lane of the quad
+// must be active.  qrot<K>: the value of lane (i + K) mod 4; qxor<K>: the value of lane i ^ K.
+template <int K>
+__device__ __forceinline__ int qrot(int v) {
+    static_assert(K >= 1 && K <= 3, "qrot");
+    constexpr int ctrl = K == 1 ? 0x39 : K == 2 ? 0x4E : 0x93;   // quad_perm [1,2,3,0] [2,3,0,1] [3,0,1,2]
+    return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ float qrotf(float v) { return __int_as_float(qrot<K>(__float_as_int(v))); }
+template <int K>
+__device__ __forceinline__ int qxor(int v) {
+    static_assert(K == 1 || K == 2, "qxor");
+    constexpr int ctrl = K == 1 ? 0xB1 : 0x4E;   // quad_perm [1,0,3,2] and [2,3,0,1]
+    return __builtin_amdgcn_mov_dpp(v, ctrl, 0xF, 0xF, false);
+}
+template <int K>
+__device__ __forceinline__ float qxorf(float v) { return __int_as_float(qxor<K>(__float_as_int(v))); }
+
 // slab test of one box; returns entry distance (>= 0) or +inf on a miss
 __device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
                                       float tlimit) {

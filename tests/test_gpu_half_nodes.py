@@ -36,13 +36,19 @@ def _transform(shapes, cam, scale, shift):
 
 
 def _both(ctx, s, lc, cam, w, h, spp, bvh=None, kd=None):
+    """fp32 nodes, binary16 nodes one lane per query, binary16 nodes with quads in sparse waves."""
     d_sh = ctx.to_device(s)
     out = []
-    for wide in (1, 2):
+    for wide, quad in ((1, 16), (2, 0), (2, 16)):
         ctx.set_tuning("wide", wide)
+        ctx.set_tuning("quad", quad)
+        ctx.set_tuning("blocks_per_cu", 1)   # few waves: many pixels per lane, long sparse tails
         out.append(render_gpu(ctx, s, lc, cam, w, h, spp, 5, None, bvh=bvh, kd=kd, d_shapes=d_sh)[:3])
     ctx.set_tuning("wide", 2)
-    return out
+    ctx.set_tuning("quad", 16)
+    ctx.set_tuning("blocks_per_cu", 2)
+    assert all(x.tobytes() == y.tobytes() for x, y in zip(out[1], out[2]))
+    return out[:2]
 
 
 @pytest.mark.parametrize("scene,scale,shift", [
@@ -84,12 +90,15 @@ def test_half_nodes_batch(ctx):
     from test_gpu_batch import _jobs
     jobs, cache = _jobs(ctx)
     res = {}
-    for wide in (1, 2):
+    for wide, quad in ((1, 16), (2, 0), (2, 16)):
         ctx.set_tuning("wide", wide)
-        res[wide] = _run(ctx, jobs, batch=True)
+        ctx.set_tuning("quad", quad)
+        res[(wide, quad)] = _run(ctx, jobs, batch=True)
     ctx.set_tuning("wide", 2)
-    for a, b in zip(res[1], res[2]):
-        for x, y in zip(a, b):
-            assert x.tobytes() == y.tobytes()
+    ctx.set_tuning("quad", 16)
+    for other in ((2, 0), (2, 16)):
+        for a, b in zip(res[(1, 16)], res[other]):
+            for x, y in zip(a, b):
+                assert x.tobytes() == y.tobytes()
     for _, bvh, kd, _ in cache.values():
         (bvh if bvh is not None else kd).close()

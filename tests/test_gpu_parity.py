@@ -328,15 +328,19 @@ def test_wavefront_equals_megakernel(ctx, scene):
         ctx.set_engine(eng)
         out[eng] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd, d_shapes=d_sh)[:3]
     # per-lane states with few lanes (several pixels per lane) and extreme regroup thresholds
-    for regroup, bpc, wide in ((1, 1, 1), (64, 1, 0), (7, 2, 1), (8, 8, 0), (3, 1, 2), (64, 3, 2)):
+    # (quad: traversing lanes at or below which four lanes share each query, binary16 nodes only)
+    for regroup, bpc, wide, quad in ((1, 1, 1, 16), (64, 1, 0, 16), (7, 2, 1, 16), (8, 8, 0, 16), (3, 1, 2, 16),
+                                     (64, 3, 2, 0), (64, 1, 2, 16), (1, 1, 2, 4), (16, 8, 2, 16)):
         ctx.set_tuning("regroup", regroup)
         ctx.set_tuning("blocks_per_cu", bpc)
         ctx.set_tuning("wide", wide)
-        out[f"lanes{regroup}/{bpc}/{wide}"] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd,
-                                                       d_shapes=d_sh)[:3]
+        ctx.set_tuning("quad", quad)
+        out[f"lanes{regroup}/{bpc}/{wide}/{quad}"] = render_gpu(ctx, s, lc, cam, w, h, 6, 5, None, bvh=bvh, kd=kd,
+                                                              d_shapes=d_sh)[:3]
     ctx.set_tuning("regroup", 16)
     ctx.set_tuning("blocks_per_cu", 2)
     ctx.set_tuning("wide", 2)
+    ctx.set_tuning("quad", 16)
     ctx.set_engine("auto")
     for eng, res in out.items():
         for a, b in zip(out["megakernel"], res):

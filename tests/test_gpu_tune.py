@@ -15,7 +15,7 @@ from test_gpu_parity import gpu_lbvh, render_gpu, say
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
-DEFAULTS = {"regroup": 16, "blocks_per_cu": 2, "wide": 2, "batch_regroup": 32, "batch_blocks_per_cu": 3}
+DEFAULTS = {"regroup": 16, "blocks_per_cu": 2, "wide": 2, "quad": 16, "batch_regroup": 32, "batch_blocks_per_cu": 3}
 
 
 def _restore_defaults(ctx):
