@@ -55,16 +55,18 @@ int fill_u32(hipStream_t s, void *p, unsigned v, size_t n_words) {
     return PTGPU_OK;
 }
 
-__global__ __launch_bounds__(kBoundsThreads) void k_bounds(Shape *shapes, int n, float *partials) {
-    bounds_body(shapes, n, partials, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kBoundsThreads) void k_bounds(const Shape *shapes, int n, float *partials,
+                                                           ShapeBox *boxes) {
+    bounds_body(shapes, n, partials, boxes, blockIdx.x, gridDim.x);
 }
 
 __global__ __launch_bounds__(kBoundsThreads) void k_bounds_final(const float *partials, int nblocks, float *scene6) {
     bounds_final_body(partials, nblocks, scene6);
 }
 
-__global__ __launch_bounds__(256) void k_morton(Shape *shapes, int n, const float *scene6, unsigned *keys) {
-    morton_body(shapes, n, scene6, keys, blockIdx.x);
+__global__ __launch_bounds__(256) void k_morton(Shape *shapes, int n, const float *scene6, unsigned *keys,
+                                                const ShapeBox *boxes) {
+    morton_body(shapes, n, scene6, keys, boxes, blockIdx.x);
 }
 
 template <int BITS>
@@ -97,10 +99,10 @@ __global__ __launch_bounds__(256) void k_karras(int n, BVHNodeGPU *btn, BVHNodeG
 }
 
 template <bool LEAF, bool TNODES>
-__global__ __launch_bounds__(256) void k_refit_levels(const Shape *shapes, const unsigned *sorted, int n, int s0,
+__global__ __launch_bounds__(256) void k_refit_levels(const ShapeBox *boxes, const unsigned *sorted, int n, int s0,
                                                       int levels, const float *in_bounds, float *out_bounds,
                                                       BVHNodeGPU *btn, BVHNodeGPU *btl, TNode *tnodes, Prim *prims) {
-    refit_levels_body<LEAF>(shapes, sorted, n, s0, levels, in_bounds, out_bounds, btn, btl, tnodes, prims, TNODES,
+    refit_levels_body<LEAF>(boxes, sorted, n, s0, levels, in_bounds, out_bounds, btn, btl, tnodes, prims, TNODES,
                             blockIdx.x);
 }
 
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(256) void k_karras_morton(const unsigned *keys, int
     karras_morton_body(keys, n, mnode, mpar_int, mpar_leaf, blockIdx.x);
 }
 
-__global__ __launch_bounds__(256) void k_mrefit_chunk(int n, const int4 *mnode, const int *mpar_int,
+__global__ __launch_bounds__(kMChunk) void k_mrefit_chunk(int n, const int4 *mnode, const int *mpar_int,
                                                       const int *mpar_leaf, const BVHNodeGPU *btl, TNode *tnodes,
                                                       float *gbi, int *gh, int2 *xlist, unsigned *xcnt,
                                                       unsigned *root_height, unsigned *err) {
@@ -150,6 +152,7 @@ static int clbvh_alloc(ptgpu_clbvh *h, int n) {
     PT_TRY(h->btl.ensure(N * sizeof(BVHNodeGPU)));
     PT_TRY(h->tnodes.ensure((N > 1 ? N - 1 : 1) * sizeof(TNode)));
     PT_TRY(h->prims.ensure(N * sizeof(Prim)));
+    PT_TRY(h->boxes.ensure(N * sizeof(ShapeBox)));
     // sort scratch: histograms (<= 8 passes x 256), status words for every pass, counters
     PT_TRY(h->hist.ensure(8 * 256 * 4));
     PT_TRY(h->status.ensure(8 * tiles * 256 * 4 + 64));
@@ -200,13 +203,14 @@ static int clbvh_construct(ptgpu_clbvh *h) {
     int blocks = (n + kBoundsThreads - 1) / kBoundsThreads;
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBoundsThreads), 0, st, h->d_shapes, n, misc_partials(h));
+    hipLaunchKernelGGL(k_bounds, dim3(blocks), dim3(kBoundsThreads), 0, st, (const Shape *)h->d_shapes, n,
+                       misc_partials(h), h->boxes.as<ShapeBox>());
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(kBoundsThreads), 0, st, (const float *)misc_partials(h), blocks,
                        misc_scene6(h));
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_morton, dim3((n + 255) / 256), dim3(256), 0, st, h->d_shapes, n, misc_scene6(h),
-                       h->keysA.as<unsigned>());
+                       h->keysA.as<unsigned>(), (const ShapeBox *)h->boxes.as<ShapeBox>());
     PT_HIP(hipGetLastError());
     PT_HIP(hipEventRecord(h->ev[1], st));
     h->timed[0] = true;
@@ -233,20 +237,20 @@ static int clbvh_refit_levels(ptgpu_clbvh *h) {
         const bool tn = h->trav_topology == 0;
         if (s0 == 0) {
             if (tn)
-                hipLaunchKernelGGL((k_refit_levels<true, true>), dim3(blocks), dim3(256), 0, st, h->d_shapes, h->sorted,
+                hipLaunchKernelGGL((k_refit_levels<true, true>), dim3(blocks), dim3(256), 0, st, h->boxes.as<ShapeBox>(), h->sorted,
                                    n, 0, levels, (const float *)nullptr, out, h->btn.as<BVHNodeGPU>(),
                                    h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(), h->prims.as<Prim>());
             else
-                hipLaunchKernelGGL((k_refit_levels<true, false>), dim3(blocks), dim3(256), 0, st, h->d_shapes,
+                hipLaunchKernelGGL((k_refit_levels<true, false>), dim3(blocks), dim3(256), 0, st, h->boxes.as<ShapeBox>(),
                                    h->sorted, n, 0, levels, (const float *)nullptr, out, h->btn.as<BVHNodeGPU>(),
                                    h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(), h->prims.as<Prim>());
         } else {
             if (tn)
-                hipLaunchKernelGGL((k_refit_levels<false, true>), dim3(blocks), dim3(256), 0, st, h->d_shapes,
+                hipLaunchKernelGGL((k_refit_levels<false, true>), dim3(blocks), dim3(256), 0, st, h->boxes.as<ShapeBox>(),
                                    h->sorted, n, s0, levels, in, out, h->btn.as<BVHNodeGPU>(), h->btl.as<BVHNodeGPU>(),
                                    h->tnodes.as<TNode>(), h->prims.as<Prim>());
             else
-                hipLaunchKernelGGL((k_refit_levels<false, false>), dim3(blocks), dim3(256), 0, st, h->d_shapes,
+                hipLaunchKernelGGL((k_refit_levels<false, false>), dim3(blocks), dim3(256), 0, st, h->boxes.as<ShapeBox>(),
                                    h->sorted, n, s0, levels, in, out, h->btn.as<BVHNodeGPU>(), h->btl.as<BVHNodeGPU>(),
                                    h->tnodes.as<TNode>(), h->prims.as<Prim>());
         }
@@ -277,7 +281,7 @@ static int clbvh_traversal_morton(ptgpu_clbvh *h) {
         hipLaunchKernelGGL(k_karras_morton, dim3((n - 1 + 255) / 256), dim3(256), 0, st, h->keysA.as<unsigned>(), n,
                            h->mnode.as<int4>(), mpar_int, mpar_leaf);
         PT_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_mrefit_chunk, dim3((n + kMChunk - 1) / kMChunk), dim3(256), 0, st, n,
+        hipLaunchKernelGGL(k_mrefit_chunk, dim3((n + kMChunk - 1) / kMChunk), dim3(kMChunk), 0, st, n,
                            h->mnode.as<int4>(), mpar_int, mpar_leaf, h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(),
                            gbi, gh, xlist, ms.xcnt, ctr + 11, ctr + 9);
         PT_HIP(hipGetLastError());
@@ -519,9 +523,9 @@ int ptgpu_clbvh_build_radix_tree(ptgpu_clbvh *h) {
     hipStream_t st = h->ctx->stream;
     const int n = h->n;
     PT_HIP(hipEventRecord(h->ev[2], st));
-    // CLBVH.cpp:83-88: internal nodes zeroed, leaves filled with 0xFF bytes
-    if (n > 1) PT_TRY(fill_u32(st, h->btn.p, 0u, (size_t)(n - 1) * sizeof(BVHNodeGPU) / 4));
-    PT_TRY(fill_u32(st, h->btl.p, 0xFFFFFFFFu, (size_t)n * sizeof(BVHNodeGPU) / 4));
+    // CLBVH.cpp:83-88: internal nodes zeroed, leaves filled with 0xFF bytes (k_karras writes both
+    // alongside the topology; a single leaf has no k_karras thread)
+    if (n == 1) PT_TRY(fill_u32(st, h->btl.p, 0xFFFFFFFFu, sizeof(BVHNodeGPU) / 4));
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, h->btn.as<BVHNodeGPU>(),
                            h->btl.as<BVHNodeGPU>());
@@ -595,7 +599,8 @@ int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes) {
     PT_CHECK(h && bytes, PTGPU_ERR_INVALID, "null argument");
     const DevBuf *bufs[] = {&h->own_shapes, &h->keysA, &h->keysB, &h->valsA, &h->valsB, &h->btn, &h->btl,
                             &h->tnodes, &h->prims, &h->hist, &h->status, &h->misc, &h->lights, &h->scan,
-                            &h->opt_tmp, &h->mnode, &h->mpar, &h->gbi, &h->xlist, &h->tnodes4, &h->treelet};
+                            &h->opt_tmp, &h->mnode, &h->mpar, &h->gbi, &h->xlist, &h->tnodes4, &h->treelet,
+                            &h->boxes};
     size_t t = 0;
     for (const DevBuf *b : bufs) t += b->bytes;
     // the four-wide view is derived at the first traversal: count it before it exists
@@ -645,6 +650,7 @@ int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
     h->hist.release(); h->status.release(); h->misc.release(); h->own_shapes.release();
     h->lights.release(); h->scan.release(); h->opt_tmp.release();
     h->mnode.release(); h->mpar.release(); h->gbi.release(); h->xlist.release(); h->tnodes4.release();
+    h->boxes.release();
     h->treelet.release();
     for (auto &e : h->ev)
         if (e) (void)hipEventDestroy(e);

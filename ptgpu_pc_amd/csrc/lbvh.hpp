@@ -38,6 +38,7 @@ struct ptgpu_clbvh {
     bool optimized = false;
     Shape *d_shapes = nullptr;
     ptgpu::DevBuf own_shapes;
+    ptgpu::DevBuf boxes;                     // per-shape box records (input order) for the build's passes
     ptgpu::DevBuf keysA, keysB, valsA, valsB;
     ptgpu::DevBuf btn, btl, tnodes, prims;
     ptgpu::DevBuf hist, status, misc;

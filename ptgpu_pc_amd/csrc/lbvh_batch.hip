@@ -24,6 +24,7 @@ namespace {
 // One scene's buffers and sizes, as the batched kernels see them.
 struct BuildSeg {
     Shape *shapes;
+    ShapeBox *boxes;
     unsigned *keysA, *keysB, *valsA, *valsB;
     unsigned *hist, *status;
     unsigned *ctr;          // counters of the handle's misc block (see ptgpu_clbvh::counters)
@@ -78,7 +79,7 @@ __global__ __launch_bounds__(256) void kb_fill(const FillSeg *f, const int *star
 __global__ __launch_bounds__(kBoundsThreads) void kb_bounds(const BuildSeg *g, const int *starts, int ns) {
     const int s = seg_find(starts, ns, blockIdx.x);
     const BuildSeg &S = g[s];
-    bounds_body(S.shapes, S.n, S.partials, blockIdx.x - starts[s], starts[s + 1] - starts[s]);
+    bounds_body(S.shapes, S.n, S.partials, S.boxes, blockIdx.x - starts[s], starts[s + 1] - starts[s]);
 }
 
 // one block per scene
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(kBoundsThreads) void kb_bounds_final(const BuildSeg
 __global__ __launch_bounds__(256) void kb_morton(const BuildSeg *g, const int *starts, int ns) {
     const int s = seg_find(starts, ns, blockIdx.x);
     const BuildSeg &S = g[s];
-    morton_body(S.shapes, S.n, S.scene6, S.keysA, blockIdx.x - starts[s]);
+    morton_body(S.shapes, S.n, S.scene6, S.keysA, S.boxes, blockIdx.x - starts[s]);
 }
 
 __global__ __launch_bounds__(256) void kb_karras(const BuildSeg *g, const int *starts, int ns) {
@@ -141,7 +142,7 @@ __global__ __launch_bounds__(256) void kb_refit(const BuildSeg *g, const int *st
     const int levels = (L - s0) < kRefitLog ? (L - s0) : kRefitLog;
     const float *in = p == 0 ? nullptr : S.lvl + ((p - 1) & 1) * S.lvl_per;
     float *out = (s0 + levels < L) ? S.lvl + (p & 1) * S.lvl_per : nullptr;
-    refit_levels_body<LEAF>(S.shapes, S.valsA, S.n, s0, levels, in, out, S.btn, S.btl, S.tnodes, S.prims, S.topo == 0,
+    refit_levels_body<LEAF>(S.boxes, S.valsA, S.n, s0, levels, in, out, S.btn, S.btl, S.tnodes, S.prims, S.topo == 0,
                             blockIdx.x - starts[s]);
 }
 
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(256) void kb_karras_morton(const BuildSeg *g, const
     karras_morton_body(S.keysA, S.n, S.mnode, S.mpar_int, S.mpar_leaf, blockIdx.x - starts[s]);
 }
 
-__global__ __launch_bounds__(256) void kb_mrefit_chunk(const BuildSeg *g, const int *starts, int ns) {
+__global__ __launch_bounds__(kMChunk) void kb_mrefit_chunk(const BuildSeg *g, const int *starts, int ns) {
     const int s = seg_find(starts, ns, blockIdx.x);
     const BuildSeg &S = g[s];
     mrefit_chunk_body(S.n, S.mnode, S.mpar_int, S.mpar_leaf, S.btl, S.tnodes, S.gbi, S.gh, S.xlist, S.xcnt,
@@ -197,6 +198,7 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         BuildSeg &S = segs[s];
         std::memset(&S, 0, sizeof(S));
         S.shapes = h->d_shapes;
+        S.boxes = h->boxes.as<ShapeBox>();
         S.keysA = h->keysA.as<unsigned>();
         S.keysB = h->keysB.as<unsigned>();
         S.valsA = h->valsA.as<unsigned>();
@@ -253,9 +255,9 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         blocks(kStKMorton, s) = mtree ? (n - 1 + 255) / 256 : 0;
         blocks(kStMChunk, s) = mtree ? (n + kMChunk - 1) / kMChunk : 0;
         blocks(kStMSuper, s) = mtree && n > kMChunk ? (int)((n + kMSuper - 1) / kMSuper) : 0;
-        // clears done by the single-scene path (CLBVH.cpp:83-88 for btn/btl)
-        if (n > 1) fills.push_back({(unsigned *)S.btn, (unsigned long long)(n - 1) * sizeof(BVHNodeGPU) / 4, 0u, 0u});
-        fills.push_back({(unsigned *)S.btl, (unsigned long long)n * sizeof(BVHNodeGPU) / 4, 0xFFFFFFFFu, 0u});
+        // clears done by the single-scene path (CLBVH.cpp:83-88 for btn/btl: kb_karras writes them,
+        // except the lone leaf of a one-shape scene)
+        if (n == 1) fills.push_back({(unsigned *)S.btl, (unsigned long long)sizeof(BVHNodeGPU) / 4, 0xFFFFFFFFu, 0u});
         fills.push_back({S.hist, (unsigned long long)C::kPasses * C::kRadix, 0u, 0u});
         fills.push_back({S.status, (unsigned long long)C::kPasses * S.tiles * C::kRadix, 0u, 0u});
         fills.push_back({S.ctr + 9, 3ull, 0u, 0u});
@@ -327,7 +329,7 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         PT_BATCH_LAUNCH(kb_refit<false>, kStRefit0 + p, 256, p);
     }
     PT_BATCH_LAUNCH(kb_karras_morton, kStKMorton, 256);
-    PT_BATCH_LAUNCH(kb_mrefit_chunk, kStMChunk, 256);
+    PT_BATCH_LAUNCH(kb_mrefit_chunk, kStMChunk, kMChunk);
     PT_BATCH_LAUNCH(kb_mrefit_mid, kStMSuper, 1024);
 #undef PT_BATCH_LAUNCH
     if (total(kStMChunk) > 0) {

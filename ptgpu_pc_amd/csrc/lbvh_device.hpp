@@ -61,18 +61,31 @@ __device__ __forceinline__ void reduce6(float *v, float *sh /*6*kBoundsThreads*/
 // Per shape: bound -> Shape.b, index -> Shape.index, centroid; block partial of the centroid
 // min/max (max starts at FLT_MIN as CLBVH.cpp:92-96 does) -> partials[bid].
 // (block `bid` of `nblocks` working on one scene; the kernels map blocks to scenes)
-__device__ __forceinline__ void bounds_body(Shape *shapes, int n, float *partials, int bid, int nblocks) {
+// A shape's box record, 64 aligned bytes in input order: words 0-9 of the Shape (type, then the
+// triangle's vertices or the sphere's radius and centre) and its AABB (min_x, max_x, min_y, max_y,
+// min_z, max_z).  The Morton pass reads it instead of the 104-byte Shape and the leaf refit gathers
+// it (one aligned 64-byte record per leaf instead of a whole Shape).
+struct alignas(16) ShapeBox {
+    float4 q[4];
+};
+static_assert(sizeof(ShapeBox) == 64, "ShapeBox");
+__device__ __forceinline__ const float *box_words(const ShapeBox &r) { return reinterpret_cast<const float *>(&r); }
+
+__device__ __forceinline__ void bounds_body(const Shape *shapes, int n, float *partials, ShapeBox *boxes, int bid,
+                                            int nblocks) {
     __shared__ float sh[6 * kBoundsThreads];
     float acc[6] = {FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN, FLT_MAX, FLT_MIN};
     for (int i = bid * blockDim.x + threadIdx.x; i < n; i += nblocks * blockDim.x) {
-        Shape *s = shapes + i;
+        const Shape *s = shapes + i;
         int type;
         Bound b = shape_bound(s, &type);
-        int *wi = reinterpret_cast<int *>(s);
-        float *wf = reinterpret_cast<float *>(s);
-        wi[17] = i;                                    // index @68
-        wf[19] = b.min_x; wf[20] = b.max_x; wf[21] = b.min_y;   // b @76
-        wf[22] = b.max_y; wf[23] = b.min_z; wf[24] = b.max_z;
+        const float *wf = reinterpret_cast<const float *>(s);
+        ShapeBox r;
+        r.q[0] = make_float4(wf[0], wf[1], wf[2], wf[3]);
+        r.q[1] = make_float4(wf[4], wf[5], wf[6], wf[7]);
+        r.q[2] = make_float4(wf[8], wf[9], b.min_x, b.max_x);
+        r.q[3] = make_float4(b.min_y, b.max_y, b.min_z, b.max_z);
+        boxes[i] = r;
         float px, py, pz;
         if (type == SPHERE) {
             px = wf[2]; py = wf[3]; pz = wf[4];
@@ -121,27 +134,33 @@ __device__ __forceinline__ unsigned expand10(unsigned x) {
 // CLBVH.cpp:125-164: quantise each centroid to 10 bits per axis (1024 wraps to 0 through the
 // low-10-bit interleave) and interleave x above y above z.  Also emits the sort key ~code (30 bits)
 // so that an ascending stable sort yields the reference's stable descending order.
-__device__ __forceinline__ void morton_body(Shape *shapes, int n, const float *scene6, unsigned *keys, int bid) {
+// Also stores the reference's per-shape outputs: index @68, morton_code @72, b @76 (one 32-byte run).
+__device__ __forceinline__ void morton_body(Shape *shapes, int n, const float *scene6, unsigned *keys,
+                                            const ShapeBox *boxes, int bid) {
     int i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float mnx = scene6[0], mxx = scene6[1], mny = scene6[2], mxy = scene6[3], mnz = scene6[4],
                 mxz = scene6[5];
-    Shape *s = shapes + i;
-    float *wf = reinterpret_cast<float *>(s);
-    int type = reinterpret_cast<int *>(s)[0];
+    const ShapeBox r = boxes[i];
+    const float *w = box_words(r);   // w[10..15]: the AABB
+    const int type = __float_as_int(w[0]);
     float px, py, pz;
     if (type == SPHERE) {
-        px = wf[2]; py = wf[3]; pz = wf[4];
+        px = w[2]; py = w[3]; pz = w[4];
     } else {
-        px = (wf[19] + wf[20]) / 2.0f;
-        py = (wf[21] + wf[22]) / 2.0f;
-        pz = (wf[23] + wf[24]) / 2.0f;
+        px = (w[10] + w[11]) / 2.0f;
+        py = (w[12] + w[13]) / 2.0f;
+        pz = (w[14] + w[15]) / 2.0f;
     }
+    Shape *s = shapes + i;
     unsigned a = f2u_code(((px - mnx) / (mxx - mnx)) * 1024.f);
     unsigned b = f2u_code(((py - mny) / (mxy - mny)) * 1024.f);
     unsigned c = f2u_code(((pz - mnz) / (mxz - mnz)) * 1024.f);
     unsigned code = (expand10(a) << 2) | (expand10(b) << 1) | expand10(c);
-    reinterpret_cast<unsigned *>(s)[18] = code;   // morton_code @72
+    unsigned *ws = reinterpret_cast<unsigned *>(s);
+    ws[17] = (unsigned)i;   // index @68
+    ws[18] = code;          // morton_code @72
+    for (int f = 0; f < 6; f++) ws[19 + f] = __float_as_uint(w[10 + f]);   // b @76
     keys[i] = (~code) & 0x3FFFFFFFu;
 }
 
@@ -345,9 +364,21 @@ __device__ __forceinline__ int delta_idx(int i, int j, int n) {
     return (j >= 0 && j < n) ? __clz(i ^ j) : -1;
 }
 
+// Each thread also writes the reference's initial contents (CLBVH.cpp:83-88: internal nodes zeroed,
+// leaves filled with 0xFF bytes) into every field of internal node i and leaf i but nParent, which
+// the parent's thread writes (the root's is 0); thread n-2 fills leaf n-1 as well.  One pass of
+// whole records instead of two fills followed by scattered field stores.
+__device__ __forceinline__ void fill_leaf_record(BVHNodeGPU *l) {
+    int *w = reinterpret_cast<int *>(l);
+    for (int f = 0; f < 15; f++)
+        if (f != 6) w[f] = -1;   // nParent (field 6) comes from the parent
+}
+
 __device__ __forceinline__ void karras_body(int n, BVHNodeGPU *btn, BVHNodeGPU *btl, int bid) {
     const int i = bid * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
+    fill_leaf_record(btl + i);
+    if (i == n - 2) fill_leaf_record(btl + n - 1);
     const int d = (delta_idx(i, i + 1, n) - delta_idx(i, i - 1, n)) > 0 ? 1 : -1;
     const int dmin = delta_idx(i, i - d, n);
     int lmax = 2;
@@ -364,17 +395,20 @@ __device__ __forceinline__ void karras_body(int n, BVHNodeGPU *btn, BVHNodeGPU *
     } while (t > 1);
     const int gamma = i + s * d + (d < 0 ? d : 0);
     const int lo = i < j ? i : j, hi = i < j ? j : i;
-    BVHNodeGPU *cur = btn + i;
     int leafflag = 0;
-    cur->nLeft = gamma;
     if (lo == gamma) { leafflag = 2; btl[gamma].nParent = i; }
     else btn[gamma].nParent = i;
-    cur->nRight = gamma + 1;
     if (hi == gamma + 1) { leafflag = 2; btl[gamma + 1].nParent = i; }
     else btn[gamma + 1].nParent = i;
-    cur->leaf = leafflag;
-    cur->min = lo;
-    cur->max = hi;
+    int *w = reinterpret_cast<int *>(btn + i);
+    w[0] = lo;           // min
+    w[1] = hi;           // max
+    w[2] = 0;            // nShape
+    w[3] = leafflag;     // leaf
+    w[4] = gamma;        // nLeft
+    w[5] = gamma + 1;    // nRight
+    if (i == 0) w[6] = 0;   // the root's nParent
+    for (int f = 7; f < 15; f++) w[f] = 0;   // bound, cost, area
 }
 
 // -------------------------------------------------------------------------------------------
@@ -383,9 +417,9 @@ __device__ __forceinline__ void karras_body(int n, BVHNodeGPU *btn, BVHNodeGPU *
 // leaf iff nLeft == min / nRight == max.  Also writes the traversal layout (TNode per internal
 // node, Prim per leaf).
 // -------------------------------------------------------------------------------------------
-__device__ __forceinline__ Prim make_prim(const Shape *s, int id) {
-    const float *w = reinterpret_cast<const float *>(s);
-    const int type = reinterpret_cast<const int *>(s)[0] | kLastInLeaf;   // one primitive per leaf
+// w: words 0-9 of a Shape (a ShapeBox's first ten)
+__device__ __forceinline__ Prim make_prim(const float *w, int id) {
+    const int type = __float_as_int(w[0]) | kLastInLeaf;   // one primitive per leaf
     Prim p;
     if ((type & 0xff) == SPHERE) {
         p.p0 = make_float4(w[2], w[3], w[4], __int_as_float(id));
@@ -439,24 +473,40 @@ __device__ __forceinline__ int child_ref(long long lo, long long hi, int n) {
 }
 
 template <bool LEAF>
-__device__ __forceinline__ void refit_levels_body(const Shape *shapes, const unsigned *sorted, int n, int s0, int levels,
+__device__ __forceinline__ void refit_levels_body(const ShapeBox *boxes, const unsigned *sorted, int n, int s0, int levels,
                                                   const float *in_bounds, float *out_bounds, BVHNodeGPU *btn,
                                                   BVHNodeGPU *btl, TNode *tnodes, Prim *prims, bool tnodes_on, int bid) {
     __shared__ float sb[6][kRefitItems];
     const long long item_base = (long long)bid * kRefitItems;
     const long long m = ((long long)n + (1ll << s0) - 1) >> s0;   // items at level s0
-    for (int q = 0; q < kRefitItems / 256; q++) {
+    // leaf level: all of this thread's gathers are issued before any store (a store between them
+    // would make each gather wait for the stores before it)
+    constexpr int kPer = kRefitItems / 256;
+    int sids[kPer];
+    ShapeBox recs[kPer];
+    if (LEAF) {
+#pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const long long it = item_base + threadIdx.x + q * 256;
+            sids[q] = it < m ? (int)sorted[it] : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; q++)
+            if (sids[q] >= 0) recs[q] = boxes[sids[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; q++) {
         const int j = threadIdx.x + q * 256;
         const long long it = item_base + j;
         if (it >= m) continue;
         Bound B;
         if (LEAF) {
-            const int sid = (int)sorted[it];
-            const Shape *sh = shapes + sid;
-            B = load_bound(&sh->b);
+            const int sid = sids[q];
+            const float *w = box_words(recs[q]);
+            B.min_x = w[10]; B.max_x = w[11]; B.min_y = w[12]; B.max_y = w[13]; B.min_z = w[14]; B.max_z = w[15];
             store_bound(btl + it, B);
             btl[it].nShape = sid;
-            prims[it] = make_prim(sh, sid);
+            prims[it] = make_prim(w, sid);
         } else {
             const float *f = in_bounds + it * 6;
             B.min_x = f[0]; B.max_x = f[1]; B.min_y = f[2]; B.max_y = f[3]; B.min_z = f[4]; B.max_z = f[5];
@@ -559,38 +609,54 @@ __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, cons
     __shared__ float nb[6][kMChunk];
     __shared__ int nh[kMChunk];
     __shared__ unsigned cnt[kMChunk];
+    // The chunk's internal nodes, their parents and the leaves' parents, loaded once with coalesced
+    // reads: the climbs below then touch global memory only for their tnodes stores and, at their
+    // end, for a parent spanning chunks.  (On gfx950 a wait for a global load also waits for every
+    // earlier store of the thread, so a load inside the climb would serialise it on those stores.)
+    __shared__ int4 smn[kMChunk];
+    __shared__ int spar[kMChunk];
+    __shared__ int slp[kMChunk];
     const int base = bid * kMChunk;
     const int cend = (n - base) < kMChunk ? n : base + kMChunk;
-    for (int q = threadIdx.x; q < kMChunk; q += 256) {
+    for (int q = threadIdx.x; q < kMChunk; q += (int)blockDim.x) {
         cnt[q] = 0;
         const int k = base + q;
         if (k < cend) {
             const Bound b = load_bound(&btl[k].bound);
             lb[0][q] = b.min_x; lb[1][q] = b.max_x; lb[2][q] = b.min_y;
             lb[3][q] = b.max_y; lb[4][q] = b.min_z; lb[5][q] = b.max_z;
+            slp[q] = n > 1 ? mpar_leaf[k] : -1;
+        }
+        if (k < n - 1) {
+            smn[q] = mnode[k];
+            spar[q] = mpar_int[k];
         }
     }
     __syncthreads();
-    for (int q = threadIdx.x; q < kMChunk; q += 256) {
+    for (int q = threadIdx.x; q < kMChunk; q += (int)blockDim.x) {
         const int k = base + q;
         if (k >= cend) break;
         int child = ~k, ch = 0;
-        int p = n > 1 ? mpar_leaf[k] : -1;
+        int p = slp[q];
+        bool spans = false;
         while (p >= 0) {
-            const int4 P = mnode[p];
-            if (P.z < base || P.w >= base + kMChunk) {
-                if (child >= 0) {
-                    const int c = child - base;
-                    for (int f = 0; f < 6; f++) gbi[(size_t)child * 6 + f] = nb[f][c];
-                    gh[child] = ch;
-                }
-                const unsigned sc = (unsigned)P.z / kMSuper;
-                const unsigned slot = atomicAdd(xcnt + sc, 1u);
-                if (slot < kXCap) xlist[(size_t)sc * kXCap + slot] = make_int2(p, child);
-                else atomicOr(err, 2u);
+            // a node whose leaf range lies inside the chunk has its index there too (Karras numbering:
+            // the index is an end of the range), so an index outside the chunk means a spanning node
+            if (p < base || p >= base + kMChunk) {
+                spans = true;
                 break;
             }
-            const unsigned old = __hip_atomic_fetch_add(&cnt[p - base], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int4 P = smn[p - base];
+            if (P.z < base || P.w >= base + kMChunk) {
+                spans = true;
+                break;
+            }
+            // The sibling's bound and height are in LDS, so the counter orders LDS only: an LDS-scope
+            // release / acquire, which (unlike a workgroup-scope one over all memory) does not wait
+            // for this thread's earlier tnodes stores to reach memory.
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+            const unsigned old = __hip_atomic_fetch_add(&cnt[p - base], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
             if (old == 0) break;
             Bound L, R;
             int hl, hr;
@@ -616,9 +682,23 @@ __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, cons
             nh[c] = ch;
             tnodes[p] = make_tnode(L, R, P.x, P.y, P.x < 0 ? 1 : 0, P.y < 0 ? 1 : 0);
             child = p;
-            p = mpar_int[p];
+            p = spar[c];
         }
-        if (p < 0 && child >= 0) *root_height = (unsigned)ch;   // the root closed inside this chunk
+        if (spans) {
+            // publish the child's bound and height, queue the arrival at its spanning parent
+            const int pz = (p >= base && p < base + kMChunk) ? smn[p - base].z : mnode[p].z;
+            if (child >= 0) {
+                const int c = child - base;
+                for (int f = 0; f < 6; f++) gbi[(size_t)child * 6 + f] = nb[f][c];
+                gh[child] = ch;
+            }
+            const unsigned sc = (unsigned)pz / kMSuper;
+            const unsigned slot = atomicAdd(xcnt + sc, 1u);
+            if (slot < kXCap) xlist[(size_t)sc * kXCap + slot] = make_int2(p, child);
+            else atomicOr(err, 2u);
+        } else if (p < 0 && child >= 0) {
+            *root_height = (unsigned)ch;   // the root closed inside this chunk
+        }
     }
 }
 
