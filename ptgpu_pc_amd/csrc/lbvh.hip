@@ -378,6 +378,7 @@ int ptgpu::clbvh_new(ptgpu_ctx *ctx, int n, ptgpu_clbvh **out) {
 }
 
 void ptgpu::clbvh_mark_built(ptgpu_clbvh *h) {
+    h->radix_pending = false;
     const int n = h->n;
     h->state = 3;
     h->timed[4] = false;   // an optimisation timed before this build no longer describes the tree
@@ -523,14 +524,10 @@ int ptgpu_clbvh_build_radix_tree(ptgpu_clbvh *h) {
     hipStream_t st = h->ctx->stream;
     const int n = h->n;
     PT_HIP(hipEventRecord(h->ev[2], st));
-    // CLBVH.cpp:83-88: internal nodes zeroed, leaves filled with 0xFF bytes (k_karras writes both
-    // alongside the topology; a single leaf has no k_karras thread)
-    if (n == 1) PT_TRY(fill_u32(st, h->btl.p, 0xFFFFFFFFu, sizeof(BVHNodeGPU) / 4));
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, h->btn.as<BVHNodeGPU>(),
-                           h->btl.as<BVHNodeGPU>());
-        PT_HIP(hipGetLastError());
-    }
+    // The topology is a closed form of N (LCP = clz(i ^ j)): the refit writes whole records,
+    // topology and bounds together, so the records are written here only when they are read
+    // before a refit (radix_pending, clbvh_materialize_radix).
+    h->radix_pending = true;
     PT_HIP(hipEventRecord(h->ev[3], st));
     h->timed[2] = true;
     h->state = 2;
@@ -548,6 +545,7 @@ int ptgpu_clbvh_build_bvh_tree(ptgpu_clbvh *h) {
     else PT_TRY(clbvh_sort<8>(h));
     PT_HIP(hipEventRecord(h->ev[5], st));
     PT_TRY(clbvh_refit_levels(h));
+    h->radix_pending = false;
     PT_HIP(hipEventRecord(h->ev[9], st));
     if (h->trav_topology == 1) PT_TRY(clbvh_traversal_morton(h));
     PT_HIP(hipEventRecord(h->ev[6], st));
@@ -564,9 +562,26 @@ int ptgpu_clbvh_set_digit_bits(ptgpu_clbvh *h, int bits) {
     return PTGPU_OK;
 }
 
+// The exported arrays of a tree built only to the radix-tree stage (CLBVH.cpp:83-88 fills, then
+// BVH.cl:127-208): k_karras writes them on demand.
+static int clbvh_materialize_radix(ptgpu_clbvh *h) {
+    if (!h->radix_pending || h->state != 2) return PTGPU_OK;
+    hipStream_t st = h->ctx->stream;
+    const int n = h->n;
+    if (n == 1) PT_TRY(fill_u32(st, h->btl.p, 0xFFFFFFFFu, sizeof(BVHNodeGPU) / 4));
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3((n - 1 + 255) / 256), dim3(256), 0, st, n, h->btn.as<BVHNodeGPU>(),
+                           h->btl.as<BVHNodeGPU>());
+        PT_HIP(hipGetLastError());
+    }
+    h->radix_pending = false;
+    return PTGPU_OK;
+}
+
 int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl) {
     PT_CHECK(h && h_btn && h_btl, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(h->state >= 2, PTGPU_ERR_INVALID, "no tree built");
+    PT_TRY(clbvh_materialize_radix(h));
     PT_TRY(clbvh_export_optimized(h));
     const int n = h->n;
     h->h_btn.resize(n > 1 ? n - 1 : 0);
@@ -589,6 +604,7 @@ int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl
 int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVHNodeGPU **d_btl) {
     PT_CHECK(h && d_btn && d_btl, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(h->state >= 2, PTGPU_ERR_INVALID, "no tree built");
+    PT_TRY(clbvh_materialize_radix(h));
     PT_TRY(clbvh_export_optimized(h));
     *d_btn = h->btn.as<BVHNodeGPU>();
     *d_btl = h->btl.as<BVHNodeGPU>();

@@ -39,6 +39,7 @@ struct ptgpu_clbvh {
     Shape *d_shapes = nullptr;
     ptgpu::DevBuf own_shapes;
     ptgpu::DevBuf boxes;                     // per-shape box records (input order) for the build's passes
+    bool radix_pending = false;              // btn/btl topology not yet written (the refit writes it)
     ptgpu::DevBuf keysA, keysB, valsA, valsB;
     ptgpu::DevBuf btn, btl, tnodes, prims;
     ptgpu::DevBuf hist, status, misc;

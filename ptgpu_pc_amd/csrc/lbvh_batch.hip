@@ -242,7 +242,7 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         int bb = (n + kBoundsThreads - 1) / kBoundsThreads;
         blocks(kStBounds, s) = bb > 4096 ? 4096 : bb < 1 ? 1 : bb;
         blocks(kStMorton, s) = (n + 255) / 256;
-        blocks(kStKarras, s) = n > 1 ? (n - 1 + 255) / 256 : 0;
+        blocks(kStKarras, s) = 0;   // the refit writes whole btn/btl records (topology included)
         int hb = (int)((n + kSortThreads * 8 - 1) / (kSortThreads * 8));
         blocks(kStHist, s) = hb > 2048 ? 2048 : hb < 1 ? 1 : hb;
         blocks(kStTiles, s) = S.tiles;

@@ -471,6 +471,39 @@ __device__ __forceinline__ int karras_index(long long lo, long long hi, int n) {
 __device__ __forceinline__ int child_ref(long long lo, long long hi, int n) {
     return lo == hi ? ~(int)lo : karras_index(lo, hi, n);
 }
+// Parent of the node (or leaf, s = 0) that covers the aligned block of level s starting at lo: the
+// first enclosing block whose right half is not empty (a block with an empty right half is the
+// same node one level up).  The root's nParent is 0 (the reference zeroes internal nodes), a lone
+// leaf's stays 0xFF-filled.
+__device__ __forceinline__ int index_tree_parent(long long lo, int s, int n) {
+    if (n < 2) return -1;
+    for (int t = s + 1; t < 40; t++) {
+        const long long plo = lo & ~((1ll << t) - 1);
+        const long long pmid = plo + (1ll << (t - 1));
+        if (pmid <= (long long)n - 1) {
+            const long long phi_full = plo + (1ll << t) - 1;
+            return karras_index(plo, phi_full < n - 1 ? phi_full : (long long)n - 1, n);
+        }
+    }
+    return 0;
+}
+// Whole reference records (BVHNodeGPU.h:4-16, 15 words) in one pass: a leaf keeps the 0xFF fill
+// (CLBVH.cpp:83-88) in every field but nShape, nParent and bound; an internal node gets its Karras
+// fields (min, max, leaf = 2 when a child is a leaf, nLeft = split, nRight = split + 1, nParent)
+// with nShape, cost and area zero.
+__device__ __forceinline__ void write_leaf_record(BVHNodeGPU *l, int sid, int parent, const Bound &B) {
+    int *w = reinterpret_cast<int *>(l);
+    w[0] = -1; w[1] = -1; w[2] = sid; w[3] = -1; w[4] = -1; w[5] = -1; w[6] = parent;
+    store_bound(l, B);
+    w[13] = -1; w[14] = -1;
+}
+__device__ __forceinline__ void write_node_record(BVHNodeGPU *d, int lo, int hi, int mid, int parent, const Bound &B) {
+    int *w = reinterpret_cast<int *>(d);
+    w[0] = lo; w[1] = hi; w[2] = 0; w[3] = (lo == mid - 1 || hi == mid) ? 2 : 0;
+    w[4] = mid - 1; w[5] = mid; w[6] = parent;
+    store_bound(d, B);
+    w[13] = 0; w[14] = 0;
+}
 
 template <bool LEAF>
 __device__ __forceinline__ void refit_levels_body(const ShapeBox *boxes, const unsigned *sorted, int n, int s0, int levels,
@@ -504,8 +537,7 @@ __device__ __forceinline__ void refit_levels_body(const ShapeBox *boxes, const u
             const int sid = sids[q];
             const float *w = box_words(recs[q]);
             B.min_x = w[10]; B.max_x = w[11]; B.min_y = w[12]; B.max_y = w[13]; B.min_z = w[14]; B.max_z = w[15];
-            store_bound(btl + it, B);
-            btl[it].nShape = sid;
+            write_leaf_record(btl + it, sid, index_tree_parent(it, 0, n), B);
             prims[it] = make_prim(w, sid);
         } else {
             const float *f = in_bounds + it * 6;
@@ -534,7 +566,7 @@ __device__ __forceinline__ void refit_levels_body(const ShapeBox *boxes, const u
             sb[0][ls] = M.min_x; sb[1][ls] = M.max_x; sb[2][ls] = M.min_y;
             sb[3][ls] = M.max_y; sb[4][ls] = M.min_z; sb[5][ls] = M.max_z;
             const int idx = karras_index(lo, hi, n);
-            store_bound(btn + idx, M);
+            write_node_record(btn + idx, (int)lo, (int)hi, (int)mid, index_tree_parent(lo, s, n), M);
             if (tnodes_on) {
                 const int rl = child_ref(lo, mid - 1, n), rr = child_ref(mid, hi, n);
                 tnodes[idx] = make_tnode(L, R, rl, rr, rl < 0 ? 1 : 0, rr < 0 ? 1 : 0);

@@ -81,6 +81,36 @@ def test_lbvh_bitexact_config_sizes(ctx, which):
     assert (np.diff(keys.astype(np.int64)) <= 0).all()
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 5000])
+def test_radix_tree_only_export(ctx, n):
+    """After buildRadixTree alone the arrays hold the reference's state at that point (CLBVH.cpp:83-88
+    fills, then BVH.cl:127-208's topology): internal nodes zeroed but for the Karras fields, leaves
+    0xFF but for nParent.  Derived from the oracle's full build by clearing what the refit adds."""
+    say("case radix-only export", n)
+    shapes = scenes.soup(max(n - 2, 1), seed=8)[0][:n].copy()
+    ref_s, ref_sorted, ref_btn, ref_btl = O.build_lbvh(shapes, literal_sort=n <= 4096)
+    s = shapes.copy()
+    bvh = pt.CLBVH(ctx, s)
+    bvh.buildRadixTree()
+    btn, btl = bvh.getTrees()
+    exp_btn = ref_btn.copy()
+    if len(exp_btn):
+        exp_btn["bound"] = 0
+    exp_btl = ref_btl.copy()
+    raw = exp_btl.view(np.uint8).reshape(len(exp_btl), 60)
+    raw[:, 8:12] = 0xFF     # nShape
+    raw[:, 28:52] = 0xFF    # bound
+    if n > 1:
+        np.testing.assert_array_equal(btn.view(np.uint8), exp_btn.view(np.uint8))
+    np.testing.assert_array_equal(btl.view(np.uint8), exp_btl.view(np.uint8))
+    bvh.buildBVHTree()
+    btn, btl = bvh.getTrees()
+    if n > 1:
+        np.testing.assert_array_equal(btn.view(np.uint8), ref_btn.view(np.uint8))
+    np.testing.assert_array_equal(btl.view(np.uint8), ref_btl.view(np.uint8))
+    bvh.close()
+
+
 def check_traversal_tree(bvh, s):
     """Invariants of the traversal tree: a binary tree over all leaves, each child box the exact
     union (select-merge) of its leaf range, children partition the parent's range."""
