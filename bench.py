@@ -84,8 +84,8 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
 
 
 RENDER_KERNEL = "k_render_lanes"
-# the timed instance: single frame, no statistics (<stack, STATS, WIDE, MULTI>)
-RENDER_INSTANCE = "k_render_lanes<%, false, %, false>"
+# the timed instance: single frame, no statistics (<stack, STATS, WIDE, MULTI, HALF>)
+RENDER_INSTANCE = "k_render_lanes<%, false, %, false, %>"
 WORKLOAD = ("LBVH build + BVH traversal: 65,536-triangle soup (seed 1) + emissive quad, 512x512, 8 spp, depth 5, "
             "30-bit Morton, one leaf per primitive")
 TRAFFIC_FILE = ROOT / "profiles" / "k_render_lanes_c2_traffic.json"
@@ -132,7 +132,7 @@ def profile_summary(d: Path, out_md: Path) -> None:
         for name, val in _db_rows(dbp, "select counter_name, avg(value) from counters_collection where kernel_name "
                                        f"like '%{RENDER_INSTANCE}%' group by counter_name"):
             sq[name] = val
-    timed = re.compile(r"k_render_lanes<\d+, false, (true|false), false>")
+    timed = re.compile(r"k_render_lanes<\d+, false, (true|false), false, (true|false)>")
     render_avg = next((r[2] / 1e3 for r in rows if timed.search(r[0])), None)
     cmds = json.loads((d / "commands.json").read_text())
     lines = [f"# {RENDER_KERNEL} — C2 (65,538-shape soup, 512x512, 8 spp, depth 5), MI355X", "",
