@@ -237,6 +237,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     f3 nl = col, Ld = col, contrib = col, inv = col;
     float tb = kInf;
     int bid = -1, node = 0, sp = 0;
+    int pl = -1;   // binary16 walk: first primitive of a leaf run waiting to be tested (-1 none)
     unsigned nv = 0;
     unsigned long long n_closest = 0, n_any = 0;
     unsigned st_nodes = 0, st_prims = 0;
@@ -293,6 +294,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             t_root = A.root;
         }
         node = t_root;
+        pl = -1;
         sp = 0;
         nv = 0;
         st = kTrav;
@@ -335,6 +337,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 q_inv.x = __shfl(inv.x, owner); q_inv.y = __shfl(inv.y, owner); q_inv.z = __shfl(inv.z, owner);
                 float q_tb = __shfl(tb, owner);
                 int q_bid = __shfl(bid, owner), q_node = __shfl(node, owner), q_sp = __shfl(sp, owner);
+                int q_pl = __shfl(pl, owner);
                 unsigned q_nv = (unsigned)__shfl((int)nv, owner);
                 const bool q_any = __shfl((int)qany, owner) != 0;
                 const TNode4H *q_nodes = t_nodes4h;
@@ -362,20 +365,30 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     const unsigned long long fin = __ballot(inq && q_end && qc == 0);
                     if (ready0 + (unsigned)__popcll(fin) >= regroup || (unsigned)__popcll(fin) >= (unsigned)nq) break;
                     if (!q_done) {
-                        if ((unsigned)q_node >= (unsigned)q_nnodes || ++q_nv > kMaxVisits) {
+                        const bool qhn = q_node >= 0;
+                        if ((qhn && q_node >= q_nnodes) || ++q_nv > kMaxVisits) {
                             q_done = q_end = true;   // malformed caller tree
                         } else {
-                            const uint4 w = q_nodes[q_node].c[qc];
+                            uint4 w = make_uint4(0x7C007C00u, 0x7C007C00u, 0x7C007C00u, (unsigned)kEmptyRef);
+                            if (qhn) w = q_nodes[q_node].c[qc];
                             const int ref = (int)w.w;
                             float tc = slab(half_lo(w.x), half_lo(w.y), half_lo(w.z), half_hi(w.x), half_hi(w.y),
                                             half_hi(w.z), q_o, q_inv, q_tb);
                             if (STATS) st_nodes += ref != kEmptyRef;
-                            // this lane's leaf child: its primitive run against the step's tb
+                            // this lane's leaf child, and on lane 0 the owner's pending run, against the
+                            // step's tb
                             float bt = q_tb;
                             int bb = q_bid;
                             bool occl = false;
-                            if (ref < 0 && tc != inf) {
-                                const int first = ~ref;
+                            for (int pass = 0; pass < 2; pass++) {
+                                int first;
+                                if (pass == 0) {
+                                    if (!(qc == 0 && q_pl >= 0)) continue;
+                                    first = q_pl;
+                                } else {
+                                    if (!(ref < 0 && tc != inf)) continue;
+                                    first = ~ref;
+                                }
                                 if (first >= 0 && first < q_nprims) {
                                     for (int q = first; q < q_nprims; q++) {
                                         int id;
@@ -391,7 +404,9 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                         if (last) break;
                                     }
                                 }
+                                if (occl) break;
                             }
+                            q_pl = -1;
                             if (ref < 0) tc = inf;   // only internal children from here on
                             // the quad's hit: occlusion by any lane; closest by (t, then higher id)
                             occl = (qxor<1>((int)occl) | qxor<2>((int)occl) | qxor<1>(qxor<2>((int)occl)) |
@@ -440,7 +455,9 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                 } else if (q_sp == 0) {
                                     q_done = q_end = true;
                                 } else {
-                                    q_node = q_stk[(--q_sp) * 256];
+                                    const int r = q_stk[(--q_sp) * 256];   // a node, or a leaf to test next
+                                    q_node = r >= 0 ? r : -1;
+                                    q_pl = r >= 0 ? -1 : ~r;
                                 }
                             }
                         }
@@ -452,8 +469,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 const float b_tb = __shfl(q_tb, src);
                 const unsigned b_nv = (unsigned)__shfl((int)q_nv, src);
                 const bool b_end = __shfl((int)q_end, src) != 0;
+                const int b_pl = __shfl(q_pl, src);
                 if (st == kTrav) {
                     node = b_node;
+                    pl = b_pl;
                     sp = b_sp;
                     bid = b_bid;
                     tb = b_tb;
@@ -500,7 +519,108 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 }
                 if (done) st = qany ? kADone : kCDone;
             }
-            if (WIDE && st == kTrav) {
+            if (HALF && st == kTrav) {
+                // One step = the current node (if any) and the pending leaf run (if any), loaded
+                // together: a hit leaf is not tested in the step that finds it but in the next one,
+                // beside the next node's loads, so a step costs one memory round trip, not two.
+                bool done = false;
+                constexpr float inf = __builtin_inff();
+                const bool hn = node >= 0;
+                if ((hn && node >= t_nnodes) || ++nv > kMaxVisits) {
+                    done = true;   // malformed caller tree
+                } else {
+                    uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0, q2 = q0, q3 = q0;
+                    if (hn) {
+                        const TNode4H *np = t_nodes4h + node;
+                        q0 = np->c[0]; q1 = np->c[1]; q2 = np->c[2]; q3 = np->c[3];
+                    }
+                    const bool hl = pl >= 0 && pl < t_nprims;
+                    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
+                    if (hl) {
+                        a0 = t_prims[pl].p0; a1 = t_prims[pl].p1; a2 = t_prims[pl].p2;
+                    }
+                    // the pending run (one primitive per LBVH leaf), before the boxes so tb is tight
+                    if (hl) {
+                        int q = pl;
+                        for (;;) {
+                            const int id = __float_as_int(a0.w);
+                            const float t = (__float_as_int(a2.w) & 0xff) == SPHERE ? sph_isect(a0, a1.x, ro, rd)
+                                                                                     : tri_isect(a0, a1, a2, ro, rd);
+                            if (STATS) st_prims++;
+                            if (qany) {
+                                if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; done = true; break; }
+                            } else if (t != 0.f && (t < tb || (t == tb && id > bid))) {
+                                tb = t;
+                                bid = id;
+                            }
+                            if ((__float_as_int(a1.w) & kLastInLeaf) || ++q >= t_nprims) break;
+                            a0 = t_prims[q].p0; a1 = t_prims[q].p1; a2 = t_prims[q].p2;
+                        }
+                    }
+                    pl = -1;
+                    int nn = -1;
+                    if (!done && hn) {
+                        int r0 = (int)q0.w, r1 = (int)q1.w, r2 = (int)q2.w, r3 = (int)q3.w;
+                        float t0 = slab(half_lo(q0.x), half_lo(q0.y), half_lo(q0.z), half_hi(q0.x), half_hi(q0.y),
+                                        half_hi(q0.z), ro, inv, tb);
+                        float t1 = slab(half_lo(q1.x), half_lo(q1.y), half_lo(q1.z), half_hi(q1.x), half_hi(q1.y),
+                                        half_hi(q1.z), ro, inv, tb);
+                        float t2 = slab(half_lo(q2.x), half_lo(q2.y), half_lo(q2.z), half_hi(q2.x), half_hi(q2.y),
+                                        half_hi(q2.z), ro, inv, tb);
+                        float t3 = slab(half_lo(q3.x), half_lo(q3.y), half_lo(q3.z), half_hi(q3.x), half_hi(q3.y),
+                                        half_hi(q3.z), ro, inv, tb);
+                        if (STATS) st_nodes += (r0 != kEmptyRef) + (r1 != kEmptyRef) + (r2 != kEmptyRef) + (r3 != kEmptyRef);
+                        auto cswap = [](float &ta, int &ra, float &tb_, int &rb) {
+                            const bool sw = tb_ < ta;
+                            const float tt = sw ? tb_ : ta;
+                            tb_ = sw ? ta : tb_;
+                            ta = tt;
+                            const int rr = sw ? rb : ra;
+                            rb = sw ? ra : rb;
+                            ra = rr;
+                        };
+                        cswap(t0, r0, t1, r1);
+                        cswap(t2, r2, t3, r3);
+                        cswap(t0, r0, t2, r2);
+                        cswap(t1, r1, t3, r3);
+                        cswap(t1, r1, t2, r2);
+                        // nearest internal child next, nearest leaf pending, the rest pushed farthest first
+                        bool c0 = false, c1 = false, c2 = false, c3 = false;
+                        auto choose = [&](float t, int r, bool &c) {
+                            if (t == inf) return;
+                            if (r >= 0) {
+                                if (nn < 0) { nn = r; c = true; }
+                            } else if (pl < 0) {
+                                pl = ~r;
+                                c = true;
+                            }
+                        };
+                        choose(t0, r0, c0);
+                        choose(t1, r1, c1);
+                        choose(t2, r2, c2);
+                        choose(t3, r3, c3);
+                        if (t3 != inf && !c3 && sp < STACK) stk[(sp++) * 256] = r3;
+                        if (t2 != inf && !c2 && sp < STACK) stk[(sp++) * 256] = r2;
+                        if (t1 != inf && !c1 && sp < STACK) stk[(sp++) * 256] = r1;
+                        if (t0 != inf && !c0 && sp < STACK) stk[(sp++) * 256] = r0;
+                    }
+                    node = nn;
+                    if (!done) {
+                        // refill from the stack: a node for the next step, and a leaf to load beside it
+                        if (node < 0 && sp > 0) {
+                            const int r = stk[(sp - 1) * 256];
+                            if (r >= 0) { node = r; --sp; }
+                        }
+                        if (pl < 0 && sp > 0) {
+                            const int r = stk[(sp - 1) * 256];
+                            if (r < 0) { pl = ~r; --sp; }
+                        }
+                        if (node < 0 && pl < 0) done = true;   // the stack is empty
+                    }
+                }
+                if (done) st = qany ? kADone : kCDone;
+            }
+            if (WIDE && !HALF && st == kTrav) {
                 bool done = false;
                 if ((unsigned)node >= (unsigned)t_nnodes || ++nv > kMaxVisits) {
                     done = true;   // malformed caller tree
