@@ -179,6 +179,7 @@ constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortThreads * kSortItems;
 constexpr unsigned kFlagAgg = 1u << 30, kFlagPrefix = 2u << 30, kValMask = (1u << 30) - 1;
 constexpr unsigned kSpinLimit = 1u << 26;
+constexpr int kLookback = 8;   // predecessor status words per look-back round
 
 template <int BITS> struct SortCfg {
     static constexpr int kRadix = 1 << BITS;
@@ -315,18 +316,34 @@ __device__ __forceinline__ void sort_pass_body(const unsigned *__restrict__ keys
                 st_relaxed(my, kFlagPrefix | cnt);
             } else {
                 st_relaxed(my, kFlagAgg | cnt);
+                // Windowed look-back: the status words of kLookback predecessors are loaded together
+                // and summed nearest first up to the first inclusive prefix, so a walk over tiles
+                // that have published only their counts costs one round trip per window instead of
+                // one per tile.  A word not yet published stops the window; the walk resumes there.
                 long long t = (long long)tile - 1;
                 unsigned spins = 0;
                 while (t >= 0) {
-                    unsigned s = ld_relaxed(status + (size_t)t * C::kRadix + d);
-                    unsigned flag = s & ~kValMask;
-                    if (flag == 0) {
-                        if (++spins > kSpinLimit) { atomicOr(err, 1u); break; }
-                        continue;
+                    unsigned w[kLookback];
+#pragma unroll
+                    for (int k = 0; k < kLookback; k++)
+                        w[k] = t - k >= 0 ? ld_relaxed(status + (size_t)(t - k) * C::kRadix + d) : kFlagPrefix;
+                    bool stop = false, found = false;
+                    int adv = 0;
+#pragma unroll
+                    for (int k = 0; k < kLookback; k++) {
+                        if (stop) continue;
+                        const unsigned flag = w[k] & ~kValMask;
+                        if (flag == 0) {
+                            stop = true;
+                        } else {
+                            prefix += w[k] & kValMask;
+                            adv++;
+                            if (flag == kFlagPrefix) stop = found = true;
+                        }
                     }
-                    prefix += s & kValMask;
-                    if (flag == kFlagPrefix) break;
-                    t--;
+                    if (found) break;
+                    t -= adv;
+                    if (adv < kLookback && ++spins > kSpinLimit) { atomicOr(err, 1u); break; }
                 }
                 st_relaxed(my, kFlagPrefix | ((prefix + cnt) & kValMask));
             }
