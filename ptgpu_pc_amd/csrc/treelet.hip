@@ -79,7 +79,9 @@ __global__ __launch_bounds__(256) void k_opt_init(const TNode *__restrict__ tn, 
 }
 
 // Depth of every internal node (walk to the root) and the deepest one (one atomic per wave).
-__global__ __launch_bounds__(256) void k_opt_depth(OptNode *on, int m, unsigned *max_depth) {
+// depth of every node (and, with hist, the number of nodes per depth)
+constexpr int kLocalDepths = 64;   // depths counted per block in LDS (deeper ones: global atomics)
+__global__ __launch_bounds__(256) void k_opt_depth(OptNode *on, int m, unsigned *max_depth, unsigned *hist) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned d = 0;
     if (i < m) {
@@ -89,6 +91,19 @@ __global__ __launch_bounds__(256) void k_opt_depth(OptNode *on, int m, unsigned 
             p = on[p].parent;
         }
         on[i].depth = (int)d;
+    }
+    if (hist) {
+        // block-local counts first: tens of depths, so one global add per depth per block instead of
+        // one per node on a handful of contended words
+        __shared__ unsigned lh[kLocalDepths];
+        if (threadIdx.x < kLocalDepths) lh[threadIdx.x] = 0;
+        __syncthreads();
+        if (i < m) {
+            if (d < (unsigned)kLocalDepths) atomicAdd(lh + d, 1u);
+            else if (d < (unsigned)kMaxLevels) atomicAdd(hist + d, 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x < kLocalDepths && lh[threadIdx.x]) atomicAdd(hist + threadIdx.x, lh[threadIdx.x]);
     }
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned v = __shfl_down(d, o, kWave);
@@ -100,23 +115,57 @@ __global__ __launch_bounds__(256) void k_opt_depth(OptNode *on, int m, unsigned 
 // tree height after optimisation: deepest internal depth + 1 (the deepest leaf), for the lazy readback
 __global__ void k_opt_height(const unsigned *max_depth, unsigned *height) { *height = *max_depth + 1u; }
 
-// Treelets of one depth.  A workgroup is one wave of 64 lanes in 8 groups of 8: it scans 64-node
-// chunks for the level's roots and restructures them 8 at a time, one per group.  The group leader
-// grows the treelet (a chain of dependent loads); the group's lanes share the subset unions and the
-// DP by subset size; the leader rebuilds.  Barriers stay wave-uniform: every group walks the same
-// subset-size loop whatever its treelet's size.
+// Per-depth node lists: dstart = exclusive prefix sums of the depth histogram (one thread; a tree has
+// tens of depths), then each node takes a slot of its depth's range.  Order within a depth is free:
+// the treelets rooted at one depth are disjoint.
+__global__ void k_opt_scan(const unsigned *hist, int levels, unsigned *dstart, unsigned *cursor) {
+    unsigned run = 0;
+    for (int d = 0; d < levels; d++) {
+        dstart[d] = run;
+        cursor[d] = 0;
+        run += hist[d];
+    }
+    dstart[levels] = run;
+}
+__global__ __launch_bounds__(256) void k_opt_scatter(const OptNode *on, int m, int levels, const unsigned *dstart,
+                                                     unsigned *cursor, int *order) {
+    // block-local ranks per depth, then one global reservation per depth per block
+    __shared__ unsigned lc[kLocalDepths], lbase[kLocalDepths];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < kLocalDepths) lc[threadIdx.x] = 0;
+    __syncthreads();
+    const int d = i < m ? on[i].depth : -1;
+    const bool listed = d >= 0 && d < levels;   // depths >= levels: not listed
+    unsigned r = 0;
+    if (listed && d < kLocalDepths) r = atomicAdd(lc + d, 1u);
+    __syncthreads();
+    if (threadIdx.x < kLocalDepths && lc[threadIdx.x]) lbase[threadIdx.x] = atomicAdd(cursor + threadIdx.x, lc[threadIdx.x]);
+    __syncthreads();
+    if (listed) order[dstart[d] + (d < kLocalDepths ? lbase[d] + r : atomicAdd(cursor + d, 1u))] = i;
+}
+
+// Treelets of depths d_hi down to d_lo, from the per-depth node lists.  A workgroup of WAVES waves is
+// 8 * WAVES groups of 8 lanes, one treelet per group at a time.  The group leader grows the treelet
+// (a chain of dependent loads); the group's lanes share the subset unions and the DP by subset size;
+// the leader rebuilds.  Barriers stay block-uniform: every group walks the same subset-size loop
+// whatever its treelet's size.  A launch over several depths is one workgroup (the top of the tree,
+// at most 2^d nodes per depth d): its barrier orders one depth's results before the next.
 constexpr int kGroup = 8;
 constexpr int kGroups = 64 / kGroup;
 
-__global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m, int depth, int tsize, OptNode *on) {
-    __shared__ float s_area[kGroups][1 << kMaxTreelet];
-    __shared__ float s_cost[kGroups][1 << kMaxTreelet];
-    __shared__ unsigned char s_best[kGroups][1 << kMaxTreelet];
-    __shared__ float s_box[kGroups][kMaxTreelet][6];
-    __shared__ int s_ref[kGroups][kMaxTreelet];
-    __shared__ int s_int[kGroups][kMaxTreelet];
-    __shared__ int s_nl[kGroups];
-    __shared__ float s_cur[kGroups];
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_treelet(const int *__restrict__ order,
+                                                        const unsigned *__restrict__ dstart, int d_hi, int d_lo,
+                                                        int tsize, OptNode *on) {
+    constexpr int NG = kGroups * WAVES;
+    __shared__ float s_area[NG][1 << kMaxTreelet];
+    __shared__ float s_cost[NG][1 << kMaxTreelet];
+    __shared__ unsigned char s_best[NG][1 << kMaxTreelet];
+    __shared__ float s_box[NG][kMaxTreelet][6];
+    __shared__ int s_ref[NG][kMaxTreelet];
+    __shared__ int s_int[NG][kMaxTreelet];
+    __shared__ int s_nl[NG];
+    __shared__ float s_cur[NG];
     const int lane = threadIdx.x, grp = lane / kGroup, sub = lane % kGroup;
     // a treelet leaf j: ref, box and subtree cost, taken from the record of its parent
     auto put_leaf = [&](int j, int ref, const float *box, float cost) {
@@ -125,21 +174,11 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
         s_cost[grp][1 << j] = cost;
     };
 
-    for (int base = blockIdx.x * 64; base < m; base += gridDim.x * 64) {
-        const int i = base + lane;
-        unsigned long long hit = __ballot(i < m && depth_src[i].depth == depth);
-        while (hit) {
-            // the next (up to) 8 roots of this chunk, one per group
-            int root = -1;
-            {
-                unsigned long long h = hit;
-                for (int g = 0; g < kGroups && h; g++) {
-                    const int j = __ffsll((long long)h) - 1;
-                    h &= h - 1;
-                    if (g == grp) root = base + j;
-                }
-                hit = h;
-            }
+    for (int d = d_hi; d >= d_lo; d--) {
+        const int lo = (int)dstart[d], cnt = (int)dstart[d + 1] - lo;
+        for (int base = blockIdx.x * NG; base < cnt; base += gridDim.x * NG) {
+            const int root = base + grp < cnt ? order[lo + base + grp] : -1;
+            int par = -1, pslot = 0;   // the root's parent and the slot of the root in it (leader only)
             if (root >= 0 && sub == 0) {
                 // grow: replace the treelet leaf of largest area that is an internal node (one fetch:
                 // its record carries its children's boxes and costs)
@@ -147,6 +186,10 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
                 float ar[kMaxTreelet];
                 inl[0] = root;
                 const OptNode R = on[root];
+                par = R.parent;
+                // the parent sits one depth up and is not rewritten at this depth: its child slot can
+                // be read now, beside the growth loads, instead of after the rebuild
+                if (par >= 0) pslot = on[par].c[0] == root ? 0 : 1;
                 for (int j = 0; j < 2; j++) {
                     put_leaf(j, R.c[j], R.cb[j], R.cc[j]);
                     ar[j] = R.c[j] >= 0 ? surface_area(R.cb[j], R.cb[j] + 3) : -1.f;
@@ -179,34 +222,63 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
             __syncthreads();
             const int nl = s_nl[grp];
             const int full = nl >= 3 ? (1 << nl) - 1 : 0;
-            // areas of every subset's union box
-            for (int s = sub + 1; s <= full; s += kGroup) {
-                float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-                float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-                for (int j = 0; j < nl; j++) {
-                    if (!((s >> j) & 1)) continue;
+            // the treelet's leaf boxes in registers (every lane): subset unions and the rebuild read them
+            float bx[kMaxTreelet][6];
+#pragma unroll
+            for (int j = 0; j < kMaxTreelet; j++)
+#pragma unroll
+                for (int f = 0; f < 6; f++) bx[j][f] = j < nl ? s_box[grp][j][f] : 0.f;
+            auto union_of = [&](int X, float *lo, float *hi) {
+#pragma unroll
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = __builtin_inff();
+                    hi[a] = -__builtin_inff();
+                }
+#pragma unroll
+                for (int j = 0; j < kMaxTreelet; j++) {
+                    if (!((X >> j) & 1)) continue;
+#pragma unroll
                     for (int a = 0; a < 3; a++) {
-                        lo[a] = fminf(lo[a], s_box[grp][j][a]);
-                        hi[a] = fmaxf(hi[a], s_box[grp][j][3 + a]);
+                        lo[a] = fminf(lo[a], bx[j][a]);
+                        hi[a] = fmaxf(hi[a], bx[j][3 + a]);
                     }
                 }
+            };
+            // areas of every subset's union box
+            for (int s = sub + 1; s <= full; s += kGroup) {
+                float lo[3], hi[3];
+                union_of(s, lo, hi);
                 s_area[grp][s] = surface_area(lo, hi);
             }
             __syncthreads();
-            // optimal cost by subset size; a split is named by the side holding the lowest member
-            for (int k = 2; k <= kMaxTreelet; k++) {
+            // optimal cost by subset size; a split is named by the side holding the lowest member.
+            // The splits of s are p = low | q for q a proper subset of the rest, visited in
+            // descending order, two at a time so that their cost loads overlap; the first minimum
+            // wins, as in a one-at-a-time scan.
+            for (int k = 2; k <= tsize; k++) {
                 for (int s = sub + 1; s <= full; s += kGroup) {
                     if (__popc((unsigned)s) != k) continue;
-                    const int low = s & -s;
+                    const int low = s & -s, rest = s ^ low;
                     float best = __builtin_inff();
                     int bp = low;
-                    for (int p = (s - 1) & s; p; p = (p - 1) & s) {
-                        if (!(p & low)) continue;
-                        const float c = s_cost[grp][p] + s_cost[grp][s ^ p];
-                        if (c < best) {
-                            best = c;
-                            bp = p;
+                    int q = (rest - 1) & rest;   // rest itself would be p == s
+                    bool more = rest != 0;
+                    while (more) {
+                        const int qa = q, qb = (q - 1) & rest;
+                        const bool hb = qa != 0;
+                        const int pa = low | qa, pb = low | qb;
+                        const float ca = s_cost[grp][pa] + s_cost[grp][s ^ pa];
+                        const float cb = hb ? s_cost[grp][pb] + s_cost[grp][s ^ pb] : __builtin_inff();
+                        if (ca < best) {
+                            best = ca;
+                            bp = pa;
                         }
+                        if (hb && cb < best) {
+                            best = cb;
+                            bp = pb;
+                        }
+                        more = hb && qb != 0;
+                        q = (qb - 1) & rest;
                     }
                     s_cost[grp][s] = kCi * s_area[grp][s] + best;
                     s_best[grp][s] = (unsigned char)bp;
@@ -237,15 +309,8 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
                                 st_node[sp++] = cref;
                             }
                             if (cref >= 0) on[cref].parent = node;
-                            float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
-                            float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-                            for (int j = 0; j < nl; j++) {
-                                if (!((X >> j) & 1)) continue;
-                                for (int a = 0; a < 3; a++) {
-                                    lo[a] = fminf(lo[a], s_box[grp][j][a]);
-                                    hi[a] = fmaxf(hi[a], s_box[grp][j][3 + a]);
-                                }
-                            }
+                            float lo[3], hi[3];
+                            union_of(X, lo, hi);
                             for (int a = 0; a < 3; a++) {
                                 o.cb[side][a] = lo[a];
                                 o.cb[side][3 + a] = hi[a];
@@ -256,8 +321,7 @@ __global__ __launch_bounds__(64) void k_treelet(const OptNode *depth_src, int m,
                     }
                 }
                 // hand the subtree's cost to the parent's record (siblings write different slots)
-                const int P = on[root].parent;
-                if (P >= 0) on[P].cc[on[P].c[0] == root ? 0 : 1] = root_cost;
+                if (par >= 0) on[par].cc[pslot] = root_cost;
             }
             __syncthreads();
         }
@@ -283,14 +347,23 @@ namespace {
 struct OptLayout {
     OptNode *nodes;
     unsigned *maxd;
+    unsigned *hist, *dstart, *cursor;   // per-depth counts, list starts (levels + 1), fill cursors
+    int *order;                         // node indices grouped by depth
 };
 
 int opt_layout(ptgpu_clbvh *h, OptLayout *L) {
     const size_t n = (size_t)h->n, m = n - 1;
-    PT_TRY(h->treelet.ensure(m * sizeof(OptNode) + 256));
+    const size_t lv = (size_t)kMaxLevels + 1;
+    PT_TRY(h->treelet.ensure(m * sizeof(OptNode) + 256 + 3 * lv * 4 + m * 4));
     char *p = static_cast<char *>(h->treelet.p);
     L->nodes = reinterpret_cast<OptNode *>(p);
-    L->maxd = reinterpret_cast<unsigned *>(p + m * sizeof(OptNode));
+    p += m * sizeof(OptNode);
+    L->maxd = reinterpret_cast<unsigned *>(p);
+    p += 256;
+    L->hist = reinterpret_cast<unsigned *>(p);
+    L->dstart = L->hist + lv;
+    L->cursor = L->dstart + lv;
+    L->order = reinterpret_cast<int *>(L->cursor + lv);
     return PTGPU_OK;
 }
 
@@ -308,17 +381,26 @@ int clbvh_optimize(ptgpu_clbvh *h, int tsize) {
     const dim3 g((m + 255) / 256), b(256);
     hipLaunchKernelGGL(k_opt_init, g, b, 0, st, h->tnodes.as<TNode>(), m, L.nodes);
     PT_TRY(fill_u32(st, L.maxd, 0u, 1));
-    hipLaunchKernelGGL(k_opt_depth, g, b, 0, st, L.nodes, m, L.maxd);
+    PT_TRY(fill_u32(st, L.hist, 0u, (size_t)levels));
+    hipLaunchKernelGGL(k_opt_depth, g, b, 0, st, L.nodes, m, L.maxd, L.hist);
+    hipLaunchKernelGGL(k_opt_scan, dim3(1), dim3(1), 0, st, (const unsigned *)L.hist, levels, L.dstart, L.cursor);
+    hipLaunchKernelGGL(k_opt_scatter, g, b, 0, st, (const OptNode *)L.nodes, m, levels, (const unsigned *)L.dstart,
+                       L.cursor, L.order);
     PT_HIP(hipGetLastError());
-    // persistent: each workgroup scans node chunks for the level's roots
-    int grid = h->ctx->cu_count * 32;
-    if (grid > (m + 63) / 64) grid = (m + 63) / 64;
-    for (int d = levels - 1; d >= 0; d--)
-        hipLaunchKernelGGL(k_treelet, dim3(grid), dim3(64), 0, st, L.nodes, m, d, tsize, L.nodes);
+    // deepest first, one launch per depth over that depth's list; depths 0..7 (at most 2^d nodes
+    // each) in one workgroup launch with a barrier between depths
+    const int dtop = levels < 8 ? levels : 8;
+    int grid = h->ctx->cu_count * 16;
+    if (grid > (m + kGroups - 1) / kGroups) grid = (m + kGroups - 1) / kGroups;
+    for (int d = levels - 1; d >= dtop; d--)
+        hipLaunchKernelGGL(k_treelet<1>, dim3(grid), dim3(64), 0, st, (const int *)L.order,
+                           (const unsigned *)L.dstart, d, d, tsize, L.nodes);
+    hipLaunchKernelGGL(k_treelet<8>, dim3(1), dim3(512), 0, st, (const int *)L.order, (const unsigned *)L.dstart,
+                       dtop - 1, 0, tsize, L.nodes);
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_opt_emit, g, b, 0, st, L.nodes, m, h->tnodes.as<TNode>());
     PT_TRY(fill_u32(st, L.maxd, 0u, 1));
-    hipLaunchKernelGGL(k_opt_depth, g, b, 0, st, L.nodes, m, L.maxd);
+    hipLaunchKernelGGL(k_opt_depth, g, b, 0, st, L.nodes, m, L.maxd, (unsigned *)nullptr);
     hipLaunchKernelGGL(k_opt_height, dim3(1), dim3(1), 0, st, L.maxd, h->height_word());
     PT_HIP(hipGetLastError());
     h->height_pending = true;   // new height read back before the next traversal
