@@ -157,7 +157,7 @@ int autotune_frame(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *a, i
     ctx->wide = bw;
 
     // 3. treelet size: rebuild (+ optimise) once, then frames_per_build renders on that tree; a
-    // restructuring has to beat the plain tree by 2% (trial timings scatter by about 1%)
+    // restructuring has to beat the plain tree by 1% (about the scatter of trial timings)
     float best_cost = 1e30f;
     int best_treelet = 0;
     float best_opt_render = best_render, best_opt_build = best_build;
@@ -168,7 +168,7 @@ int autotune_frame(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *a, i
         PT_TRY(render());
         PT_TRY(best_ms(ctx, ev, trials, restore, render, &rms));
         const float cost = bms + (float)frames_per_build * rms;
-        if (t == 0 ? cost < best_cost : cost < best_cost * (best_treelet == 0 ? 0.98f : 1.f)) {
+        if (t == 0 ? cost < best_cost : cost < best_cost * (best_treelet == 0 ? 0.99f : 1.f)) {
             best_cost = cost;
             best_treelet = t;
             best_opt_render = rms;
