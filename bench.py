@@ -719,6 +719,9 @@ def main():
     ap.add_argument("--graph", action="store_true", help="only the recorded-graph vs eager per-frame comparison")
     ap.add_argument("--c3", action="store_true", help="only the 327K-triangle icosphere: LBVH (+treelet 7) vs KD trees")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph comparison in the default line")
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="build and render each frame in turn on one stream (default: the next frame's tree is "
+                         "built on a second stream while the current frame renders)")
     ap.add_argument("--tuning", default=None, help="JSON file with the tuning object of an earlier run (no autotune)")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=500.0)
@@ -813,6 +816,56 @@ def main():
                   ray_counter=ray_counter, trav_stats=trav_stats)
         ctx.record(2)
 
+    # Two frames in flight: frame k's tree is built on a second context's stream (its own handle and
+    # shape array, one of two used in turn) while frame k-1 renders.  A build waits for the render
+    # that last used its handle; the render of frame k waits for its build (the library inserts
+    # that wait when the handle's context differs from the render's).  Event slots: build stream
+    # 2i/2i+1 = build begin/end of parity i; render stream 6+2i/7+2i = render begin/end.
+    pipeline = not args.no_pipeline
+    if pipeline:
+        cb = pt.Context(0)
+        bits = int(tuning["digit_bits"]) if tuning else 8
+        d_shapes_b = ctx.to_device(shapes)
+        slots = [(pt.CLBVH(cb, d_shapes, n, digit_bits=bits), d_shapes),
+                 (pt.CLBVH(cb, d_shapes_b, n, digit_bits=bits), d_shapes_b)]
+        ctx.record(7)
+        ctx.record(9)
+        frame = [0]
+        last = [None]
+
+        def pstep(ray_counter=None, trav_stats=None):
+            """Enqueue frame k; returns (build_ms, render_ms) of frame k-1 (None for the first), read
+            after frame k's render is queued so the host's wait never idles the device."""
+            i = frame[0] & 1
+            frame[0] += 1
+            b, s_i = slots[i]
+            cb.wait_event(ctx, 7 + 2 * i)
+            cb.record(2 * i)
+            b.reset()
+            b.buildRadixTree()
+            b.buildBVHTree()
+            for _ in range(passes if treelet else 0):
+                b.optimize(treelet)
+            cb.record(2 * i + 1)
+            d_seeds.copy_from(seeds0)
+            ctx.wait_event(cb, 2 * i + 1)
+            ctx.record(6 + 2 * i)
+            pt.render(ctx, b, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=s_i, shapeCnt=n,
+                      lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=d_pix,
+                      ray_counter=ray_counter, trav_stats=trav_stats)
+            ctx.record(7 + 2 * i)
+            prev, last[0] = last[0], i
+            if prev is None:
+                return None
+            return cb.elapsed_ms(2 * prev, 2 * prev + 1), ctx.elapsed_ms(6 + 2 * prev, 7 + 2 * prev)
+
+        def pdrain():
+            """Timings of the last frame enqueued; leaves both streams idle."""
+            prev, last[0] = last[0], None
+            ctx.synchronize()
+            cb.synchronize()
+            return cb.elapsed_ms(2 * prev, 2 * prev + 1), ctx.elapsed_ms(6 + 2 * prev, 7 + 2 * prev)
+
     # counter calibration: one plain copy of a known 256 MiB (beyond the 256 MB MALL)
     cal_src = ctx.alloc(256 << 20, np.uint8)
     cal_dst = ctx.alloc(256 << 20, np.uint8)
@@ -821,27 +874,62 @@ def main():
     cal_src.free()
     cal_dst.free()
 
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    # one counting frame (untimed) for the roofline's algorithmic bytes
-    step(trav_stats=d_stats)
-    ctx.synchronize()
+    if pipeline:
+        for _ in range(args.warmup):
+            pstep()
+        pdrain()
+        # one counting frame (untimed) for the roofline's algorithmic bytes
+        pstep(trav_stats=d_stats)
+        pdrain()
+    else:
+        for _ in range(args.warmup):
+            step()
+        ctx.synchronize()
+        step(trav_stats=d_stats)
+        ctx.synchronize()
     box_tests, prim_tests = (int(x) for x in d_stats.download(np.uint64)[:2])
 
     d_rays.zero()
     ctx.synchronize()
     render_ms, build_ms = [], []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(ray_counter=d_rays)
-        build_ms.append(ctx.elapsed_ms(0, 1))
-        render_ms.append(ctx.elapsed_ms(1, 2))
-    ctx.synchronize()
+    if pipeline:
+        for _ in range(args.steps):
+            r = pstep(ray_counter=d_rays)
+            if r is not None:
+                build_ms.append(r[0])
+                render_ms.append(r[1])
+        r = pdrain()
+        build_ms.append(r[0])
+        render_ms.append(r[1])
+    else:
+        for _ in range(args.steps):
+            step(ray_counter=d_rays)
+            build_ms.append(ctx.elapsed_ms(0, 1))
+            render_ms.append(ctx.elapsed_ms(1, 2))
+        ctx.synchronize()
     wall = time.perf_counter() - t0
     closest, anyhit = (int(x) for x in d_rays.download(np.uint64)[:2])
     rays = closest + anyhit
-    stage = bvh.stage_times()
+    overlapped_ms = None
+    if pipeline:
+        # the builds above ran beside a render, on what it left of the device; the build rate is
+        # measured on builds alone (after the timed region), as the one-stream step runs them
+        overlapped_ms = float(np.mean(build_ms))
+        b = slots[0][0]
+        build_ms = []
+        for _ in range(5):
+            cb.record(0)
+            b.reset()
+            b.buildRadixTree()
+            b.buildBVHTree()
+            for _ in range(passes if treelet else 0):
+                b.optimize(treelet)
+            cb.record(1)
+            build_ms.append(cb.elapsed_ms(0, 1))
+        stage = b.stage_times()
+    else:
+        stage = bvh.stage_times()
     mrays = rays / wall / 1e6
     render_avg = float(np.mean(render_ms))
     build_avg = float(np.mean(build_ms))
@@ -891,7 +979,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": WORKLOAD,
-                   "shapes": n, "width": W, "height": H, "spp": SPP, "max_depth": DEPTH},
+                   "shapes": n, "width": W, "height": H, "spp": SPP, "max_depth": DEPTH,
+                   "frames_in_flight": 2 if pipeline else 1},
         "rays_per_step": rays // args.steps,
         "closest_per_step": closest // args.steps,
         "anyhit_per_step": anyhit // args.steps,
@@ -900,6 +989,9 @@ def main():
         # LBVH construction alone (Morton, sort, topology, refit stages of the last step);
         # build_with_optimize_mtri_s: primitives over build_ms
         "build_ms": round(build_avg, 4),
+        # with two frames in flight: the mean span of a build on its own stream beside the previous
+        # frame's render (hidden behind it)
+        "build_overlapped_ms": round(overlapped_ms, 4) if overlapped_ms is not None else None,
         "build_mtri_s": round(n / ((stage["morton"] + stage["sort"] + stage["topology"] + stage["refit"]) * 1e-3)
                               / 1e6, 1),
         "build_with_optimize_mtri_s": round(n / (build_avg * 1e-3) / 1e6, 1),
@@ -913,6 +1005,10 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if pipeline:
+        for b, _ in slots:
+            b.close()
+        cb.close()
     bvh.close()
     ctx.close()
 

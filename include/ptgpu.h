@@ -140,6 +140,12 @@ int ptgpu_copy_calibration(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_
 /* Stream-ordered timing: record a named event slot (0..15) and read the elapsed ms between two. */
 int ptgpu_event_record(ptgpu_ctx *ctx, int slot);
 int ptgpu_event_elapsed_ms(ptgpu_ctx *ctx, int slot_begin, int slot_end, float *ms);
+/* No reference counterpart.  Work enqueued on `waiter` after this call waits for event `slot` last
+ * recorded on `signaller` (two contexts on one device run on two streams; e.g. build the next frame's
+ * tree on one while the other renders).  A render on one context of a handle built on another
+ * waits for the handle's build by itself; reusing a handle for a rebuild while another context
+ * still renders with it needs this wait. */
+int ptgpu_ctx_wait_event(ptgpu_ctx *waiter, ptgpu_ctx *signaller, int slot);
 
 /* ------------------------------------------------------------------------------------------ */
 /* LBVH builder -- replaces class CLBVH (CLBVH.h:39-47, CLBVH.cpp:80-366)                      */

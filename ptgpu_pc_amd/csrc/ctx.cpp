@@ -237,6 +237,12 @@ int ptgpu_event_record(ptgpu_ctx *ctx, int slot) {
     return PTGPU_OK;
 }
 
+int ptgpu_ctx_wait_event(ptgpu_ctx *waiter, ptgpu_ctx *signaller, int slot) {
+    PT_CHECK(waiter && signaller && slot >= 0 && slot < 16, PTGPU_ERR_INVALID, "bad argument");
+    PT_HIP(hipStreamWaitEvent(waiter->stream, signaller->ev[slot], 0));
+    return PTGPU_OK;
+}
+
 int ptgpu_event_elapsed_ms(ptgpu_ctx *ctx, int a, int b, float *ms) {
     PT_CHECK(ctx && ms && a >= 0 && a < 16 && b >= 0 && b < 16, PTGPU_ERR_INVALID, "bad event slots");
     PT_HIP(hipEventSynchronize(ctx->ev[b]));

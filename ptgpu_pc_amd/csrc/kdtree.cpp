@@ -368,6 +368,8 @@ int upload_device(ptgpu_kdtree *T) {
     A.nodes4 = nullptr;
     A.nodes4h = nullptr;
     PT_TRY(build_wide(T->ctx, &A, T->d_nodes4));
+    // the tree is complete when the build returns: a render on any context's stream may use it
+    PT_HIP(hipStreamSynchronize(st));
     return PTGPU_OK;
 }
 

@@ -64,6 +64,6 @@ struct ptgpu_clbvh {
     // device word holding the traversal tree's height (read back lazily by clbvh_prepare_traversal)
     unsigned *height_word() const { return counters() + 11; }
     // events: 0/1 morton, 2/3 topology, 4/5 sort, 5/6 refit, 7/8 optimize
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[11] = {};   // 0-9 stage timing; 10: the traversal layout is ready (cross-stream use)
     bool timed[5] = {false, false, false, false, false};
 };

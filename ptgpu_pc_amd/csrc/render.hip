@@ -1323,6 +1323,14 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
         PT_CHECK(bvh->state >= 3, PTGPU_ERR_INVALID, "BVH not built (call build_bvh_tree)");
         PT_CHECK((unsigned)bvh->n == n, PTGPU_ERR_INVALID, "shapeCnt %u differs from the BVH's %d", n, bvh->n);
         PT_TRY(clbvh_prepare_traversal(bvh));
+        PT_CHECK(bvh->ctx->device == ctx->device, PTGPU_ERR_INVALID, "BVH built on device %d, render on device %d",
+                 bvh->ctx->device, ctx->device);
+        if (bvh->ctx != ctx && bvh->ctx->stream != ctx->stream) {
+            // a handle built on another context's stream: this stream waits for that stream's work on
+            // the handle (the build and the derived traversal layout) before it traverses
+            PT_HIP(hipEventRecord(bvh->ev[10], bvh->ctx->stream));
+            PT_HIP(hipStreamWaitEvent(ctx->stream, bvh->ev[10], 0));
+        }
         *acc = bvh->acc;
         return PTGPU_OK;
     }

@@ -90,6 +90,7 @@ _SIGS = {
     "ptgpu_memcpy_d2d": [vp, vp, vp, C.c_size_t],
     "ptgpu_host_free": [vp],
     "ptgpu_event_record": [vp, i32],
+    "ptgpu_ctx_wait_event": [vp, vp, i32],
     "ptgpu_copy_calibration": [vp, vp, vp, C.c_size_t],
     "ptgpu_event_elapsed_ms": [vp, i32, i32, C.POINTER(f32)],
     "ptgpu_clbvh_create": [vp, vp, i32, C.POINTER(vp)],
@@ -349,6 +350,10 @@ class Context:
 
     def record(self, slot: int) -> None:
         _check(self.lib.ptgpu_event_record(self.handle, slot))
+
+    def wait_event(self, other: "Context", slot: int) -> None:
+        """Later work on this context waits for event `slot` last recorded on `other`."""
+        _check(self.lib.ptgpu_ctx_wait_event(self.handle, other.handle, slot))
 
     def elapsed_ms(self, a: int, b: int) -> float:
         ms = f32()

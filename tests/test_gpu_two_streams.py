@@ -1,0 +1,111 @@
+"""Trees built on one context's stream and rendered on another's.
+
+A render whose CLBVH handle belongs to another context waits for that context's work on the handle
+(the build and the traversal layout derived at the first render after it), so a caller can build
+frame k+1 on one stream while frame k renders on another.  Checked bit for bit against the same
+frames built and rendered on one stream: a handle rendered on a second context straight after its
+build, and six frames in flight over two handles whose shape arrays are refilled between uses
+(three scenes in turn, so a render that read the other slot's tree or a refill that overtook the
+render still using it changes the image), with and without treelet restructuring.
+"""
+import numpy as np
+import pytest
+
+import ptgpu_pc_amd as pt
+from ptgpu_pc_amd import scenes
+from ptgpu_pc_amd.layout import default_seeds
+
+from test_gpu_parity import render_gpu, say
+
+pytestmark = pytest.mark.gpu
+
+W, H, SPP, DEPTH = 64, 48, 2, 5
+F32 = np.float32
+
+
+def _scenes():
+    out = [scenes.soup(20000, seed=s, width=W, height=H) for s in (31, 32, 33)]
+    assert len({len(x[0]) for x in out}) == 1 and len({x[1] for x in out}) == 1
+    return out
+
+
+def _serial(ctx, shapes, lc, cam, treelet):
+    bvh = pt.CLBVH(ctx, shapes)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    if treelet:
+        bvh.optimize(treelet)
+    d_sh = ctx.to_device(shapes)
+    img = render_gpu(ctx, shapes, lc, cam, W, H, SPP, DEPTH, None, bvh=bvh, d_shapes=d_sh)[:3]
+    bvh.close()
+    return img
+
+
+def test_render_on_other_context(ctx):
+    say("case build on context B, render on context A")
+    shapes, lc, cam = scenes.soup(20000, seed=31, width=W, height=H)
+    pt.update_camera(cam)
+    ref = _serial(ctx, shapes, lc, cam, 0)
+    cb = pt.Context(0)
+    try:
+        d_sh = ctx.to_device(shapes)
+        ctx.synchronize()
+        bvh = pt.CLBVH(cb, d_sh, len(shapes))
+        for rebuild in range(2):
+            if rebuild:
+                bvh.reset()
+            bvh.buildRadixTree()
+            bvh.buildBVHTree()
+            got = render_gpu(ctx, shapes, lc, cam, W, H, SPP, DEPTH, None, bvh=bvh, d_shapes=d_sh)[:3]
+            for x, y in zip(got, ref):
+                assert x.tobytes() == y.tobytes()
+        bvh.close()
+    finally:
+        cb.close()
+
+
+@pytest.mark.parametrize("treelet", [0, 5])
+def test_two_frames_in_flight(ctx, treelet):
+    say("case two frames in flight, treelet", treelet)
+    sc = _scenes()
+    n, lc = len(sc[0][0]), sc[0][1]
+    cam = sc[0][2]
+    pt.update_camera(cam)
+    refs = [_serial(ctx, s, lc, cam, treelet) for s, _, _ in sc]
+    cb = pt.Context(0)
+    try:
+        src = [ctx.to_device(s) for s, _, _ in sc]
+        arr = [ctx.alloc(src[0].nbytes) for _ in range(2)]
+        hs = [pt.CLBVH(cb, arr[i], n) for i in range(2)]
+        d_cam = ctx.to_device(cam)
+        seeds0 = ctx.to_device(default_seeds(W, H))
+        outs = []
+        for k in range(6):
+            i, j = k & 1, k % 3
+            # refill slot i on the render stream (after the render that last read it), then build on
+            # the other stream once the refill is done
+            arr[i].copy_from(src[j])
+            ctx.record(10 + i)
+            cb.wait_event(ctx, 10 + i)
+            hs[i].reset()
+            hs[i].buildRadixTree()
+            hs[i].buildBVHTree()
+            if treelet:
+                hs[i].optimize(treelet)
+            col = ctx.alloc(W * H * 12, F32).zero()
+            seeds = ctx.alloc(W * H * 8, np.uint32).copy_from(seeds0)
+            pix = ctx.alloc(W * H * 4, np.uint32)
+            pt.render(ctx, hs[i], None, colors=col, seeds=seeds, camera=d_cam, shapes=arr[i], shapeCnt=n,
+                      lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=pix)
+            outs.append((col, seeds, pix))
+        ctx.synchronize()
+        cb.synchronize()
+        for k, (col, seeds, pix) in enumerate(outs):
+            got = (col.download(F32), seeds.download(np.uint32), pix.download(np.uint32))
+            for x, y in zip(got, refs[k % 3]):
+                assert x.tobytes() == y.tobytes(), k
+        assert refs[0][0].tobytes() != refs[1][0].tobytes()
+        for h in hs:
+            h.close()
+    finally:
+        cb.close()
