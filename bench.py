@@ -549,6 +549,81 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: boo
                         "frame rendered per step"}
 
 
+def frames_compare(tuning: dict | None, counts=(1, 2, 4), steps: int = 5, warmup: int = 1):
+    """The C2 frame rendered one per launch against 2 and 4 frames of it per launch (ptgpu_render_batch).
+    Every frame has its own tree, rebuilt each step (LBVH + the tuned treelet pass, one after another
+    on one stream), its own buffers and the same seeds, so every frame's pixels must equal the
+    one-per-launch frame's.  A single frame is bound by its longest per-pixel chain of samples (the
+    RNG state runs through a pixel's samples in order); more frames per launch fill the device
+    while those chains run."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import default_seeds
+    shapes, lc, cam = scenes.soup(N_TRI, seed=1, width=W, height=H)
+    pt.update_camera(cam)
+    n = len(shapes)
+    ctx = pt.Context(0)
+    tun = tuning or {}
+    for k in ("blocks_per_cu", "regroup", "wide", "batch_blocks_per_cu", "batch_regroup"):
+        if k in tun:
+            ctx.set_tuning(k, int(tun[k]))
+    treelet = int(tun.get("treelet") or 0)
+    fmax = max(counts)
+    d_sh = [ctx.to_device(shapes) for _ in range(fmax)]
+    hs = [pt.CLBVH(ctx, d_sh[i], n, digit_bits=int(tun.get("digit_bits", 8))) for i in range(fmax)]
+    d_cam = ctx.to_device(cam)
+    seeds0 = ctx.to_device(default_seeds(W, H))
+    bufs = [dict(colors=ctx.alloc(W * H * 12, np.float32).zero(), seeds=ctx.alloc(W * H * 8, np.uint32),
+                 pixels=ctx.alloc(W * H * 4, np.uint32)) for _ in range(fmax)]
+    d_rays = ctx.alloc(16, np.uint64).zero()
+
+    def job(i):
+        return dict(bvh=hs[i], camera=d_cam, shapes=d_sh[i], shapeCnt=n, lightCnt=lc, width=W, height=H,
+                    currentSample=0, spp=SPP, max_depth=DEPTH, **bufs[i])
+
+    def step(f, counter=None):
+        for i in range(f):
+            hs[i].reset()
+            hs[i].buildRadixTree()
+            hs[i].buildBVHTree()
+            if treelet:
+                hs[i].optimize(treelet)
+            bufs[i]["seeds"].copy_from(seeds0)
+        ctx.record(0)
+        if f == 1:
+            pt.render(ctx, hs[0], None, ray_counter=counter, **{a: b for a, b in job(0).items() if a != "bvh"})
+        else:
+            pt.render_batch(ctx, [job(i) for i in range(f)], ray_counter=counter)
+        ctx.record(1)
+
+    rows, ref, same = [], None, True
+    for f in counts:
+        for _ in range(warmup):
+            step(f)
+        ctx.synchronize()
+        pix = [bufs[i]["pixels"].download(np.uint32).tobytes() for i in range(f)]
+        if ref is None:
+            ref = pix[0]
+        same = same and all(p == ref for p in pix)
+        d_rays.zero()
+        ctx.synchronize()
+        r_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(f, d_rays)
+            r_ms.append(ctx.elapsed_ms(0, 1))
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        rays = int(d_rays.download(np.uint64)[:2].sum())
+        rows.append({"frames_per_launch": f, "render_ms_per_frame": round(float(np.mean(r_ms)) / f, 3),
+                     "ms_per_frame": round(wall * 1e3 / (steps * f), 3), "mrays_s": round(rays / wall / 1e6, 1)})
+    for h in hs:
+        h.close()
+    ctx.close()
+    return {"workload": WORKLOAD, "treelet": treelet, "rows": rows, "pixels_identical": same,
+            "note": "each step rebuilds every frame's tree, then renders the frames in one launch"}
+
+
 def graph_compare(frames: int = 200, sizes=(128, 256), spp: int = 1):
     """Per-frame sequence of an interactive viewer on the C2 scene (camera upload from pinned memory,
     seed reset, colour clear, render, pixel readback, wait), issued call by call against replayed as
@@ -719,6 +794,9 @@ def main():
     ap.add_argument("--graph", action="store_true", help="only the recorded-graph vs eager per-frame comparison")
     ap.add_argument("--c3", action="store_true", help="only the 327K-triangle icosphere: LBVH (+treelet 7) vs KD trees")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph comparison in the default line")
+    ap.add_argument("--frames", action="store_true",
+                    help="only the C2 frame rendered 1, 2 and 4 per launch (settings of --tuning FILE, else the defaults)")
+    ap.add_argument("--no-frames", action="store_true", help="skip the frames-per-launch comparison in the default line")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="build and render each frame in turn on one stream (default: the next frame's tree is "
                          "built on a second stream while the current frame renders)")
@@ -738,6 +816,10 @@ def main():
         return
     if args.c3:
         print(json.dumps(c3_compare()), flush=True)
+        return
+    if args.frames:
+        tun = json.loads(Path(args.tuning).read_text()) if args.tuning else None
+        print(json.dumps(frames_compare(tun)), flush=True)
         return
     if args.batch:
         print(json.dumps(batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)), flush=True)
@@ -955,6 +1037,10 @@ def main():
     if not args.no_graph:
         graphed = graph_compare()
 
+    per_launch = None
+    if not args.no_frames:
+        per_launch = frames_compare(tuning)
+
     batched = None
     if not args.no_batch:
         batched = batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)
@@ -1001,6 +1087,7 @@ def main():
         "tuning": tuning,
         "served": served,
         "batched": batched,
+        "frames_per_launch": per_launch,
         "graph": graphed,
         "cpu_baseline": cpu,
     }
