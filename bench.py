@@ -874,11 +874,12 @@ def main():
         if not args.treelet and tuning["treelet"]:
             treelet, passes = int(tuning["treelet"]), 1
     elif not args.no_tune:
-        # autotuner: digit width, launch shape, treelet size for this scene and frame (one frame per build)
+        # autotuner: digit width, launch shape, treelet size for this scene and frame (one frame per
+        # build; frames_per_build 0: each build runs beside the previous frame's render)
         bvh.buildRadixTree()
         bvh.buildBVHTree()
         d_seeds.copy_from(seeds0)
-        tuning = pt.autotune(ctx, bvh, frames_per_build=1, trials=2, colors=d_col, seeds=d_seeds, camera=d_cam,
+        tuning = pt.autotune(ctx, bvh, frames_per_build=1 if args.no_pipeline else 0, trials=2, colors=d_col, seeds=d_seeds, camera=d_cam,
                              shapes=d_shapes, shapeCnt=n, lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP,
                              max_depth=DEPTH, pixels=d_pix)
         if not args.treelet and tuning["treelet"]:

@@ -350,8 +350,11 @@ typedef struct {
 } ptgpu_tuning;
 /* Tunes the digit width, the single-frame launch shape (blocks per CU 1..4, regroup 8/16/32, node
  * format: wide 2, 1 or 0) and the treelet size (none, 5, 7; minimising rebuild + optimise +
- * frames_per_build renders).  Applies the result to ctx and h and leaves h holding a tree built
- * (and optimised) with it.  args: one ptgpu_render call's arguments on h (h: >= 2 shapes). */
+ * frames_per_build renders).  frames_per_build 0: every frame rebuilds its tree on another stream
+ * while the previous frame renders (two frames in flight), so the render time alone is minimised,
+ * over treelet sizes whose rebuild + optimise takes at most 1/8 of the render.  Applies the result
+ * to ctx and h and leaves h holding a tree built (and optimised) with it.  args: one ptgpu_render
+ * call's arguments on h (h: >= 2 shapes). */
 int ptgpu_autotune(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *args, int frames_per_build,
                    int trials, ptgpu_tuning *out);
 /* Tunes batch_blocks_per_cu (2..4) and batch_regroup (16/32/64) on a ptgpu_render_batch job list

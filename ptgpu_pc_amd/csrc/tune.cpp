@@ -7,7 +7,9 @@
 //   * single-frame render: workgroups per CU (1..4), shading-pass threshold `regroup` (8, 16, 32),
 //     node format (four-wide binary16, four-wide fp32 or binary);
 //   * treelet restructuring (none, 5 or 7 leaves): build + optimise time plus `frames_per_build`
-//     renders on the resulting tree, the cost a caller pays per rebuild;
+//     renders on the resulting tree, the cost a caller pays per rebuild.  frames_per_build 0 means
+//     each rebuild runs on another stream beside the previous frame's render: the cost is the render
+//     alone, for a build short enough to hide there (kOverlapBuildShare);
 //   * batched renders: workgroups per CU (2..4) and `batch_regroup` (16, 32, 64) on the caller's job list.
 // Trial renders write to scratch copies of the caller's frame buffers, so the caller's colours,
 // seeds and pixels are untouched.
@@ -19,6 +21,10 @@
 using namespace ptgpu;
 
 namespace {
+
+// frames_per_build 0: the largest rebuild (+ optimise) time, as a share of the render, that the
+// tuner takes to be hidden behind the previous frame's render
+constexpr float kOverlapBuildShare = 0.125f;
 
 struct Events {
     hipEvent_t a = nullptr, b = nullptr;
@@ -157,7 +163,10 @@ int autotune_frame(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *a, i
     ctx->wide = bw;
 
     // 3. treelet size: rebuild (+ optimise) once, then frames_per_build renders on that tree; a
-    // restructuring has to beat the plain tree by 1% (about the scatter of trial timings)
+    // restructuring has to beat the plain tree by 1% (about the scatter of trial timings).  A build
+    // beside a render gets the device only as the render's waves retire, so it hides only when it
+    // is short against the render: at 65K triangles and an 8.2 ms render, a 0.75 ms build +
+    // optimise completed inside the render and a 1.7 ms one (treelet 7) held up the next frame.
     float best_cost = 1e30f;
     int best_treelet = 0;
     float best_opt_render = best_render, best_opt_build = best_build;
@@ -167,7 +176,8 @@ int autotune_frame(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *a, i
         PT_TRY(restore());
         PT_TRY(render());
         PT_TRY(best_ms(ctx, ev, trials, restore, render, &rms));
-        const float cost = bms + (float)frames_per_build * rms;
+        if (frames_per_build == 0 && t != 0 && bms > kOverlapBuildShare * rms) continue;
+        const float cost = frames_per_build == 0 ? rms : bms + (float)frames_per_build * rms;
         if (t == 0 ? cost < best_cost : cost < best_cost * (best_treelet == 0 ? 0.99f : 1.f)) {
             best_cost = cost;
             best_treelet = t;
@@ -204,7 +214,7 @@ int ptgpu_autotune(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *args
     PT_CHECK(h->n >= 2, PTGPU_ERR_INVALID, "a scene of at least two shapes is needed to tune");
     PT_CHECK(args->colors && args->seeds && args->camera && args->shapes, PTGPU_ERR_INVALID, "null buffer");
     PT_CHECK(args->width > 0 && args->height > 0 && args->spp >= 1, PTGPU_ERR_INVALID, "bad frame");
-    PT_CHECK(frames_per_build >= 1 && trials >= 1 && trials <= 16, PTGPU_ERR_INVALID,
+    PT_CHECK(frames_per_build >= 0 && trials >= 1 && trials <= 16, PTGPU_ERR_INVALID,
              "frames_per_build %d / trials %d out of range", frames_per_build, trials);
     PT_HIP(hipSetDevice(ctx->device));
     const SavedTuning saved(ctx);

@@ -64,6 +64,32 @@ def test_autotune_frame(ctx):
         bvh.close()
 
 
+def test_autotune_overlapped_builds(ctx):
+    """frames_per_build 0 (each rebuild beside the previous frame's render): the render alone is
+    minimised, over treelet sizes whose rebuild + optimise is at most 1/8 of the render."""
+    w, h = 128, 96
+    shapes, lc, cam = scenes.soup(20000, seed=32, width=w, height=h)
+    pt.update_camera(cam)
+    say("case autotune overlapped builds")
+    s, bvh = gpu_lbvh(ctx, shapes)
+    d_sh = ctx.to_device(s)
+    img0 = render_gpu(ctx, s, lc, cam, w, h, 4, 5, None, bvh=bvh, d_shapes=d_sh)[:3]
+    try:
+        t = pt.autotune(ctx, bvh, frames_per_build=0, trials=1, colors=ctx.alloc(w * h * 12, F32).zero(),
+                        seeds=ctx.to_device(default_seeds(w, h)), camera=ctx.to_device(cam), shapes=d_sh,
+                        shapeCnt=len(s), lightCnt=lc, width=w, height=h, currentSample=0, spp=4, max_depth=5,
+                        pixels=ctx.alloc(w * h * 4, np.uint32).zero())
+        say(f"  {t}")
+        assert t["treelet"] in (0, 5, 7)
+        assert t["treelet"] == 0 or t["build_ms"] <= 0.125 * t["render_ms"] * (1 + 1e-6)
+        img1 = render_gpu(ctx, s, lc, cam, w, h, 4, 5, None, bvh=bvh, d_shapes=d_sh)[:3]
+        for a, b in zip(img0, img1):
+            assert a.tobytes() == b.tobytes()
+    finally:
+        _restore_defaults(ctx)
+        bvh.close()
+
+
 def test_autotune_batch(ctx):
     say("case autotune batch")
     jobs, keep = [], []
