@@ -141,6 +141,10 @@ def profile_summary(d: Path, out_md: Path) -> None:
     lines += ["", "## Kernel trace (rocprofv3 --kernel-trace --stats)", "",
               "| kernel | calls | average ms | % of GPU time |", "|---|---|---|---|"]
     lines += [f"| `{r[0][:90]}` | {r[1]} | {r[2] / 1e3:.4f} | {r[3]:.2f} |" for r in rows]   # rocpd: µs
+    lines += ["", "The timed frames build the next frame's tree on a second stream while the current frame "
+              "renders, so build kernels overlap the render and their durations are spans on the device: "
+              "`k_mrefit_chunk` (1024-thread workgroups) starts only as render waves retire. Summed "
+              "kernel time exceeds the wall time, and the build kernels' shares overstate their cost."]
     lines += ["", "## HBM traffic per render launch", "",
               f"* calibration: `k_copy_calib` reads and writes {CALIB_BYTES >> 20} MiB; FETCH_SIZE reports {f_c:.0f} KiB "
               f"(true/reported = {f_scale:.4f}), WRITE_SIZE reports {w_c:.0f} KiB (true/reported = {w_scale:.4f})",
