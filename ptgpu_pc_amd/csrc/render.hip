@@ -529,16 +529,19 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 if ((hn && node >= t_nnodes) || ++nv > kMaxVisits) {
                     done = true;   // malformed caller tree
                 } else {
-                    uint4 q0 = make_uint4(0u, 0u, 0u, 0u), q1 = q0, q2 = q0, q3 = q0;
-                    if (hn) {
-                        const TNode4H *np = t_nodes4h + node;
-                        q0 = np->c[0]; q1 = np->c[1]; q2 = np->c[2]; q3 = np->c[3];
-                    }
+                    // Every lane issues both loads, the run's primitive first: a lane with no run reads
+                    // primitive 0 and one with no node reads node 0 (both exist in a tree with binary16
+                    // nodes), and ignores them.  With no branch around the loads the wait for the
+                    // primitive leaves the node's four loads in flight, so the triangle test overlaps
+                    // the node's latency.
                     const bool hl = pl >= 0 && pl < t_nprims;
-                    float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, a2 = a0;
-                    if (hl) {
-                        a0 = t_prims[pl].p0; a1 = t_prims[pl].p1; a2 = t_prims[pl].p2;
-                    }
+                    const Prim *pp = t_prims + (hl ? pl : 0);
+                    float4 a0 = pp->p0, a1 = pp->p1, a2 = pp->p2;
+                    // compiler-only barrier (no operands, no instruction): the primitive's loads stay
+                    // here, ahead of the node's, instead of sinking into the branch that tests them
+                    asm volatile("" ::: "memory");
+                    const TNode4H *np = t_nodes4h + (hn ? node : 0);
+                    const uint4 q0 = np->c[0], q1 = np->c[1], q2 = np->c[2], q3 = np->c[3];
                     // the pending run (one primitive per LBVH leaf), before the boxes so tb is tight
                     if (hl) {
                         int q = pl;
