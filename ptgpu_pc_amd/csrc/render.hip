@@ -544,20 +544,27 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     const uint4 q0 = np->c[0], q1 = np->c[1], q2 = np->c[2], q3 = np->c[3];
                     // the pending run (one primitive per LBVH leaf), before the boxes so tb is tight
                     if (hl) {
-                        int q = pl;
-                        for (;;) {
-                            const int id = __float_as_int(a0.w);
-                            const float t = (__float_as_int(a2.w) & 0xff) == SPHERE ? sph_isect(a0, a1.x, ro, rd)
-                                                                                     : tri_isect(a0, a1, a2, ro, rd);
+                        // true: the run ends here (an occluder found, or its last primitive)
+                        auto test_prim = [&](const float4 &b0, const float4 &b1, const float4 &b2, int q) -> bool {
+                            const int id = __float_as_int(b0.w);
+                            const float t = (__float_as_int(b2.w) & 0xff) == SPHERE ? sph_isect(b0, b1.x, ro, rd)
+                                                                                     : tri_isect(b0, b1, b2, ro, rd);
                             if (STATS) st_prims++;
                             if (qany) {
-                                if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; done = true; break; }
+                                if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; done = true; return true; }
                             } else if (t != 0.f && (t < tb || (t == tb && id > bid))) {
                                 tb = t;
                                 bid = id;
                             }
-                            if ((__float_as_int(a1.w) & kLastInLeaf) || ++q >= t_nprims) break;
-                            a0 = t_prims[q].p0; a1 = t_prims[q].p1; a2 = t_prims[q].p2;
+                            return (__float_as_int(b1.w) & kLastInLeaf) || q + 1 >= t_nprims;
+                        };
+                        // the first primitive outside the loop: the loop's own loads issue in another
+                        // order, and a shared loop head would make this test wait for the node too
+                        if (!test_prim(a0, a1, a2, pl)) {
+                            for (int q = pl + 1;; q++) {
+                                const float4 b0 = t_prims[q].p0, b1 = t_prims[q].p1, b2 = t_prims[q].p2;
+                                if (test_prim(b0, b1, b2, q)) break;
+                            }
                         }
                     }
                     pl = -1;
