@@ -616,14 +616,19 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     }
                     node = nn;
                     if (!done) {
-                        // refill from the stack: a node for the next step, and a leaf to load beside it
-                        if (node < 0 && sp > 0) {
-                            const int r = stk[(sp - 1) * 256];
-                            if (r >= 0) { node = r; --sp; }
+                        // refill from the stack: a node for the next step, and a leaf to load beside it.
+                        // The top two entries are read together (one LDS round trip, not two).
+                        const int e1 = sp > 0 ? stk[(sp - 1) * 256] : 0;
+                        const int e2 = sp > 1 ? stk[(sp - 2) * 256] : 0;
+                        int top = e1;
+                        if (node < 0 && sp > 0 && e1 >= 0) {
+                            node = e1;
+                            --sp;
+                            top = e2;
                         }
-                        if (pl < 0 && sp > 0) {
-                            const int r = stk[(sp - 1) * 256];
-                            if (r < 0) { pl = ~r; --sp; }
+                        if (pl < 0 && sp > 0 && top < 0) {
+                            pl = ~top;
+                            --sp;
                         }
                         if (node < 0 && pl < 0) done = true;   // the stack is empty
                     }
