@@ -369,42 +369,51 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         if ((qhn && q_node >= q_nnodes) || ++q_nv > kMaxVisits) {
                             q_done = q_end = true;   // malformed caller tree
                         } else {
-                            uint4 w = make_uint4(0x7C007C00u, 0x7C007C00u, 0x7C007C00u, (unsigned)kEmptyRef);
-                            if (qhn) w = q_nodes[q_node].c[qc];
-                            const int ref = (int)w.w;
-                            float tc = slab(half_lo(w.x), half_lo(w.y), half_lo(w.z), half_hi(w.x), half_hi(w.y),
-                                            half_hi(w.z), q_o, q_inv, q_tb);
-                            if (STATS) st_nodes += ref != kEmptyRef;
-                            // this lane's leaf child, and on lane 0 the owner's pending run, against the
-                            // step's tb
+                            // On lane 0 the owner's pending run, then on every lane its leaf child, against
+                            // the step's tb.  The run's first primitive is loaded before the child (every
+                            // lane issues both loads; the others read primitive 0 and ignore it), so its
+                            // test overlaps the child's latency.
+                            const bool qhl = qc == 0 && q_pl >= 0 && q_pl < q_nprims;
+                            const Prim *qpp = q_prims + (qhl ? q_pl : 0);
+                            const float4 m0 = qpp->p0, m1 = qpp->p1, m2 = qpp->p2;
+                            asm volatile("" ::: "memory");   // compiler-only: keeps the order above
+                            const uint4 w = q_nodes[qhn ? q_node : 0].c[qc];   // ignored with no node
                             float bt = q_tb;
                             int bb = q_bid;
                             bool occl = false;
-                            for (int pass = 0; pass < 2; pass++) {
-                                int first;
-                                if (pass == 0) {
-                                    if (!(qc == 0 && q_pl >= 0)) continue;
-                                    first = q_pl;
-                                } else {
-                                    if (!(ref < 0 && tc != inf)) continue;
-                                    first = ~ref;
+                            auto take = [&](float t, int id) -> bool {   // true: occluded, stop
+                                if (STATS) st_prims++;
+                                if (q_any) {
+                                    if (t != 0.f && t < q_tb && id != q_nshapes - 1) { occl = true; return true; }
+                                } else if (t != 0.f && (t < bt || (t == bt && id > bb))) {
+                                    bt = t;
+                                    bb = id;
                                 }
-                                if (first >= 0 && first < q_nprims) {
-                                    for (int q = first; q < q_nprims; q++) {
-                                        int id;
-                                        bool last;
-                                        const float t = prim_isect_run(q_prims, q, q_o, q_d, &id, &last);
-                                        if (STATS) st_prims++;
-                                        if (q_any) {
-                                            if (t != 0.f && t < q_tb && id != q_nshapes - 1) { occl = true; break; }
-                                        } else if (t != 0.f && (t < bt || (t == bt && id > bb))) {
-                                            bt = t;
-                                            bb = id;
-                                        }
-                                        if (last) break;
+                                return false;
+                            };
+                            if (qhl) {
+                                int id;
+                                bool last;
+                                const float t = prim_isect_vals(m0, m1, m2, q_o, q_d, &id, &last);
+                                if (!take(t, id) && !last) {
+                                    for (int q = q_pl + 1; q < q_nprims; q++) {
+                                        const float t2 = prim_isect_run(q_prims, q, q_o, q_d, &id, &last);
+                                        if (take(t2, id) || last) break;
                                     }
                                 }
-                                if (occl) break;
+                            }
+                            const int ref = qhn ? (int)w.w : kEmptyRef;
+                            float tc = slab(half_lo(w.x), half_lo(w.y), half_lo(w.z), half_hi(w.x), half_hi(w.y),
+                                            half_hi(w.z), q_o, q_inv, q_tb);
+                            if (!qhn) tc = inf;
+                            if (STATS) st_nodes += ref != kEmptyRef;
+                            if (!occl && ref < 0 && tc != inf && ~ref < q_nprims) {
+                                for (int q = ~ref; q < q_nprims; q++) {
+                                    int id;
+                                    bool last;
+                                    const float t = prim_isect_run(q_prims, q, q_o, q_d, &id, &last);
+                                    if (take(t, id) || last) break;
+                                }
                             }
                             q_pl = -1;
                             if (ref < 0) tc = inf;   // only internal children from here on

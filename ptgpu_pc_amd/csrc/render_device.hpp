@@ -74,13 +74,17 @@ __device__ __forceinline__ float prim_isect(const Prim *__restrict__ prims, int 
 }
 
 // prim_isect that also reports whether primitive k ends its leaf run (kLastInLeaf)
-__device__ __forceinline__ float prim_isect_run(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id,
-                                                bool *last) {
-    const float4 p0 = prims[k].p0, p1 = prims[k].p1, p2 = prims[k].p2;
+__device__ __forceinline__ float prim_isect_vals(const float4 &p0, const float4 &p1, const float4 &p2, f3 o, f3 d,
+                                                 int *id, bool *last) {
     *id = __float_as_int(p0.w);
     *last = (__float_as_int(p1.w) & kLastInLeaf) != 0;
     if ((__float_as_int(p2.w) & 0xff) == SPHERE) return sph_isect(p0, p1.x, o, d);
     return tri_isect(p0, p1, p2, o, d);
+}
+__device__ __forceinline__ float prim_isect_run(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id,
+                                                bool *last) {
+    const float4 p0 = prims[k].p0, p1 = prims[k].p1, p2 = prims[k].p2;
+    return prim_isect_vals(p0, p1, p2, o, d, id, last);
 }
 
 // the two binary16 halves of a TNode4H word, widened exactly to fp32
