@@ -218,6 +218,21 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
     const unsigned long long lt = lanemask_lt();
+    // A single frame's first kLdsLights lights, 16 words each (words 0-12 of the light's Shape, then
+    // its area), copied once per workgroup: SampleLights then reads LDS instead of two dependent
+    // global loads (the light's index, then its Shape) per light in every shading pass.
+    constexpr int kLdsLights = 32;
+    __shared__ float s_lights[MULTI ? 16 : kLdsLights * 16];
+    int n_lds_lights = 0;
+    if (!MULTI) {
+        n_lds_lights = P.nlights < kLdsLights ? P.nlights : kLdsLights;
+        for (int i = threadIdx.x; i < n_lds_lights * 16; i += blockDim.x) {
+            const int l = i >> 4, w = i & 15;
+            const float *sw = reinterpret_cast<const float *>(P.shapes + P.lights[l]);
+            s_lights[i] = w <= 12 ? sw[w] : (w == 13 ? sw[25] : 0.f);
+        }
+        __syncthreads();
+    }
     // frame fields: the launch's own, or (batch) the lane's job's
 #define JF(field) (MULTI ? P.jobs[jb].field : P.field)
     int jb = 0;
@@ -799,7 +814,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 bool shoot = false;
                 const int nlights = JF(nlights);
                 for (; li < nlights; li++) {
-                    if (sample_light(JF(shapes) + JF(lights)[li], s0, s1, ro, nl, &rd, &len, &contrib)) {
+                    const bool sh = !MULTI && li < n_lds_lights
+                                        ? sample_light_words(s_lights + 16 * li, s_lights[16 * li + 13], s0, s1, ro,
+                                                             nl, &rd, &len, &contrib)
+                                        : sample_light(JF(shapes) + JF(lights)[li], s0, s1, ro, nl, &rd, &len,
+                                                       &contrib);
+                    if (sh) {
                         shoot = true;
                         break;
                     }

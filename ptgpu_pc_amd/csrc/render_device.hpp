@@ -300,10 +300,10 @@ __device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
 
 // ---- one light of SampleLights (rendering_kernel.cl:421-490): draws two randoms (u1 first),
 // returns true when a shadow ray must decide the contribution s * e; sd/len describe that ray.
-__device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsigned &s1, f3 hp, f3 nl, f3 *sd_out,
-                                             float *len_out, f3 *contrib) {
-    const float *lw = reinterpret_cast<const float *>(L);
-    const int ltype = reinterpret_cast<const int *>(L)[0];
+// lw: words 0-12 of the light's Shape (type, geometry, emission); area: its word 25
+__device__ __forceinline__ bool sample_light_words(const float *lw, float area, unsigned &s0, unsigned &s1, f3 hp,
+                                                   f3 nl, f3 *sd_out, float *len_out, f3 *contrib) {
+    const int ltype = __float_as_int(lw[0]);
     f3 lp, ln;
     if (ltype == TRIANGLE) {
         const float u1 = get_random(s0, s1);
@@ -336,11 +336,16 @@ __device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsig
     wo = fminf(fmaxf(wo, 0.f), 1.f);
     const float wi = dot3(sd, nl);
     if (!(wi > 0.f)) return false;
-    const float s = lw[25] * wi * wo / (len * len);
+    const float s = area * wi * wo / (len * len);
     *sd_out = sd;
     *len_out = len;
     *contrib = smul(s, mk(lw[10], lw[11], lw[12]));
     return true;
+}
+__device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsigned &s1, f3 hp, f3 nl, f3 *sd_out,
+                                             float *len_out, f3 *contrib) {
+    const float *lw = reinterpret_cast<const float *>(L);
+    return sample_light_words(lw, lw[25], s0, s1, hp, nl, sd_out, len_out, contrib);
 }
 
 // ---- cosine-weighted diffuse bounce (rendering_kernel.cl:629-661) ----
