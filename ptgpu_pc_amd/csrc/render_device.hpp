@@ -277,8 +277,14 @@ struct Surface {
     int refl;
 };
 __device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
-    const float *w = reinterpret_cast<const float *>(sh);
-    const int type = reinterpret_cast<const int *>(sh)[0];
+    // words 0-16 loaded together before the type is looked at: one memory round trip, where loads
+    // placed in the type's branches would wait for the type word first
+    float w[17];
+    const float *src = reinterpret_cast<const float *>(sh);
+#pragma unroll
+    for (int i = 0; i < 17; i++) w[i] = src[i];
+    asm volatile("" ::: "memory");   // compiler-only: keeps the loads here, ahead of the branches
+    const int type = __float_as_int(w[0]);
     Surface S;
     f3 nrm;
     if (type == SPHERE) {
@@ -294,7 +300,7 @@ __device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
     S.nl = smul(-1.f * sgn, nrm);
     S.e = mk(w[10], w[11], w[12]);
     S.c = mk(w[13], w[14], w[15]);
-    S.refl = reinterpret_cast<const int *>(sh)[16];
+    S.refl = __float_as_int(w[16]);
     return S;
 }
 
