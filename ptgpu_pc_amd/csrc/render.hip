@@ -221,8 +221,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     // A single frame's first kLdsLights lights, 16 words each (words 0-12 of the light's Shape, then
     // its area), copied once per workgroup: SampleLights then reads LDS instead of two dependent
     // global loads (the light's index, then its Shape) per light in every shading pass.
+    // The camera likewise: every new sample reads it.
     constexpr int kLdsLights = 32;
     __shared__ float s_lights[MULTI ? 16 : kLdsLights * 16];
+    __shared__ Camera s_cam;
     int n_lds_lights = 0;
     if (!MULTI) {
         n_lds_lights = P.nlights < kLdsLights ? P.nlights : kLdsLights;
@@ -231,6 +233,9 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             const float *sw = reinterpret_cast<const float *>(P.shapes + P.lights[l]);
             s_lights[i] = w <= 12 ? sw[w] : (w == 13 ? sw[25] : 0.f);
         }
+        constexpr int kCamWords = (int)(sizeof(Camera) / sizeof(float));
+        if ((int)threadIdx.x < kCamWords)
+            reinterpret_cast<float *>(&s_cam)[threadIdx.x] = reinterpret_cast<const float *>(P.camera)[threadIdx.x];
         __syncthreads();
     }
     // frame fields: the launch's own, or (batch) the lane's job's
@@ -767,7 +772,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     s1 = JF(seeds)[2 * gid + 1];
                     k = 0;
                     if (JF(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
-                    camera_ray(JF(camera), s0, s1, W, H, x, y, ro, rd);
+                    camera_ray(MULTI ? JF(camera) : &s_cam, s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
@@ -842,7 +847,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 const int W = JF(width), H = JF(height);
                 const int x = gid % W, y = gid / W;
                 if (k < JF(spp)) {
-                    camera_ray(JF(camera), s0, s1, W, H, x, y, ro, rd);
+                    camera_ray(MULTI ? JF(camera) : &s_cam, s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
