@@ -350,8 +350,15 @@ __device__ __forceinline__ bool sample_light_words(const float *lw, float area, 
 }
 __device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsigned &s1, f3 hp, f3 nl, f3 *sd_out,
                                              float *len_out, f3 *contrib) {
-    const float *lw = reinterpret_cast<const float *>(L);
-    return sample_light_words(lw, lw[25], s0, s1, hp, nl, sd_out, len_out, contrib);
+    // the words the sample reads, loaded together ahead of the branch on the light's type (one
+    // memory round trip instead of the type word first, then the geometry)
+    const float *src = reinterpret_cast<const float *>(L);
+    float lw[13];
+#pragma unroll
+    for (int i = 0; i < 13; i++) lw[i] = src[i];
+    const float area = src[25];
+    asm volatile("" ::: "memory");   // compiler-only: keeps the loads here, ahead of the branches
+    return sample_light_words(lw, area, s0, s1, hp, nl, sd_out, len_out, contrib);
 }
 
 // ---- cosine-weighted diffuse bounce (rendering_kernel.cl:629-661) ----
