@@ -238,6 +238,16 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             reinterpret_cast<float *>(&s_cam)[threadIdx.x] = reinterpret_cast<const float *>(P.camera)[threadIdx.x];
         __syncthreads();
     }
+    // A batch's first pixel per job (prefix sums of the jobs' pixel counts) for up to kLdsJobs jobs:
+    // the search for a claimed pixel's job then reads LDS instead of one dependent global load per
+    // halving.
+    constexpr int kLdsJobs = 256;
+    __shared__ unsigned s_job_begin[MULTI ? kLdsJobs : 1];
+    if (MULTI) {
+        const int nj = P.njobs < kLdsJobs ? P.njobs : kLdsJobs;
+        for (int i = threadIdx.x; i < nj; i += blockDim.x) s_job_begin[i] = P.job_begin[i];
+        __syncthreads();
+    }
     // frame fields: the launch's own, or (batch) the lane's job's
 #define JF(field) (MULTI ? P.jobs[jb].field : P.field)
     int jb = 0;
@@ -756,13 +766,25 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 const unsigned my = base + (unsigned)__popcll(need & lt);
                 if (my < (unsigned)P.total) {
                     if (MULTI) {   // the job holding work item `my` (prefix sums of the jobs' pixel counts)
+                        // two loops rather than one over a selected pointer, so the LDS search
+                        // compiles to LDS reads (a pointer that may be either would be a flat one)
                         int lo = 0, hi = P.njobs - 1;
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (P.job_begin[mid] <= my) lo = mid; else hi = mid - 1;
+                        unsigned first;
+                        if (P.njobs <= kLdsJobs) {
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (s_job_begin[mid] <= my) lo = mid; else hi = mid - 1;
+                            }
+                            first = s_job_begin[lo];
+                        } else {
+                            while (lo < hi) {
+                                const int mid = (lo + hi + 1) >> 1;
+                                if (P.job_begin[mid] <= my) lo = mid; else hi = mid - 1;
+                            }
+                            first = P.job_begin[lo];
                         }
                         jb = lo;
-                        gid = (int)(my - P.job_begin[jb]);
+                        gid = (int)(my - first);
                     } else {
                         gid = (int)my;
                     }
