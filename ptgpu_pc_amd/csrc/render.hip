@@ -312,17 +312,6 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         tb = any ? maxt : kInf;
         bid = -1;
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
-        if (MULTI) {
-            const DevAccel &A = P.jobs[jb].acc;
-            t_nodes = A.nodes;
-            t_nodes4 = A.nodes4;
-            t_nodes4h = A.nodes4h;
-            t_prims = A.prims;
-            t_nnodes = A.nnodes;
-            t_nprims = A.nprims;
-            t_nshapes = A.nshapes;
-            t_root = A.root;
-        }
         node = t_root;
         pl = -1;
         sp = 0;
@@ -785,6 +774,16 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         }
                         jb = lo;
                         gid = (int)(my - first);
+                        // the job's tree, for every query of this pixel
+                        const DevAccel &A = P.jobs[jb].acc;
+                        t_nodes = A.nodes;
+                        t_nodes4 = A.nodes4;
+                        t_nodes4h = A.nodes4h;
+                        t_prims = A.prims;
+                        t_nnodes = A.nnodes;
+                        t_nprims = A.nprims;
+                        t_nshapes = A.nshapes;
+                        t_root = A.root;
                     } else {
                         gid = (int)my;
                     }
