@@ -102,3 +102,36 @@ def test_half_nodes_batch(ctx):
                 assert x.tobytes() == y.tobytes()
     for _, bvh, kd, _ in cache.values():
         (bvh if bvh is not None else kd).close()
+
+
+@pytest.mark.parametrize("n_tri", [0, 1, 3, 30])
+def test_smallest_trees(ctx, n_tri):
+    """Trees of 2 to 32 shapes (a single internal node upward).  A binary16 walk step has every lane
+    load a primitive and a node, lanes with none reading primitive 0 and node 0; those exist in any
+    tree with binary16 nodes.  All node formats give the same bits, and the oracle's image."""
+    from test_gpu_parity import image_checks, render_oracle
+    import pyoracle as O
+    w, h = 40, 32
+    shapes, lc, cam = scenes.soup(n_tri, seed=70 + n_tri, width=w, height=h)
+    pt.update_camera(cam)
+    say("case smallest tree", len(shapes), "shapes")
+    bvh = pt.CLBVH(ctx, shapes)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    d_sh = ctx.to_device(shapes)
+    out = []
+    try:
+        for wide, quad in ((2, 16), (2, 0), (1, 16), (0, 16)):
+            ctx.set_tuning("wide", wide)
+            ctx.set_tuning("quad", quad)
+            out.append(render_gpu(ctx, shapes, lc, cam, w, h, 4, 5, None, bvh=bvh, d_shapes=d_sh)[:3])
+    finally:
+        ctx.set_tuning("wide", 2)
+        ctx.set_tuning("quad", 16)
+        bvh.close()
+    for o in out[1:]:
+        for x, y in zip(out[0], o):
+            assert x.tobytes() == y.tobytes()
+    _, _, btn, btl = O.build_lbvh(shapes)
+    ref = render_oracle(shapes, lc, cam, w, h, 4, 5, O.accel_bvh(btn, btl))
+    image_checks(*out[0], *ref)
