@@ -251,6 +251,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     // frame fields: the launch's own, or (batch) the lane's job's
 #define JF(field) (MULTI ? P.jobs[jb].field : P.field)
     int jb = 0;
+    // batch: the lane's job's fields that every shading pass reads, kept in registers from the pixel
+    // claim on (a field read through the job table is one more dependent load)
+    const Shape *j_shapes = P.shapes;
+    const int *j_lights = P.lights;
+    int j_nlights = P.nlights, j_max_depth = P.max_depth;
+#define JSHAPES (MULTI ? j_shapes : P.shapes)
     // the tree of the lane's current query (uniform for a single frame)
     const TNode *t_nodes = P.acc.nodes;
     const TNode4 *t_nodes4 = P.acc.nodes4;
@@ -784,6 +790,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         t_nprims = A.nprims;
                         t_nshapes = A.nshapes;
                         t_root = A.root;
+                        j_shapes = P.jobs[jb].shapes;
+                        j_lights = P.jobs[jb].lights;
+                        j_nlights = P.jobs[jb].nlights;
+                        j_max_depth = P.jobs[jb].max_depth;
                     } else {
                         gid = (int)my;
                     }
@@ -808,7 +818,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     st = kEnded;
                 } else {
                     const f3 hp = add3(ro, smul(tb, rd));
-                    const Surface S = surface_at(JF(shapes) + bid, hp, rd);
+                    const Surface S = surface_at(JSHAPES + bid, hp, rd);
                     if (!iszero3(S.e)) {
                         if (spec) rad = add3(rad, mul3(thr, smul(fabsf(S.dp), S.e)));
                         st = kEnded;
@@ -838,12 +848,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 // lights whose sample needs no shadow ray only draw their random numbers
                 float len = 0.f;
                 bool shoot = false;
-                const int nlights = JF(nlights);
+                const int nlights = MULTI ? j_nlights : P.nlights;
                 for (; li < nlights; li++) {
                     const bool sh = !MULTI && li < n_lds_lights
                                         ? sample_light_words(s_lights + 16 * li, s_lights[16 * li + 13], s0, s1, ro,
                                                              nl, &rd, &len, &contrib)
-                                        : sample_light(JF(shapes) + JF(lights)[li], s0, s1, ro, nl, &rd, &len,
+                                        : sample_light(JSHAPES + (MULTI ? j_lights : P.lights)[li], s0, s1, ro, nl, &rd, &len,
                                                        &contrib);
                     if (sh) {
                         shoot = true;
@@ -861,7 +871,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 depth++;
                 st = kSeg;
             }
-            if (st == kSeg && depth > JF(max_depth)) st = kEnded;
+            if (st == kSeg && depth > (MULTI ? j_max_depth : P.max_depth)) st = kEnded;
             if (st == kEnded) {
                 col = running_mean(col, rad, JF(currentSample) + k);
                 k++;
@@ -911,6 +921,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         }
     }
 #undef JF
+#undef JSHAPES
 }
 
 // Batch queries (Intersect / IntersectP on arbitrary rays).
