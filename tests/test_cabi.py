@@ -38,6 +38,7 @@ def test_null_arguments_are_rejected():
     assert lib.ptgpu_render(None, None, None, None) == 1
     assert lib.ptgpu_kdtree_build(None, None, 5, None) == 1
     assert b"null" in lib.ptgpu_last_error()
+    assert lib.ptgpu_ctx_wait_event(None, None, 0) == 1
     h = C.c_void_p()
     shapes = pt.empty_shapes(4)
     assert lib.ptgpu_kdtree_build(None, shapes.ctypes.data, 0, C.byref(h)) == 1
