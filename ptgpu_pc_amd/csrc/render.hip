@@ -256,7 +256,11 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     const Shape *j_shapes = P.shapes;
     const int *j_lights = P.lights;
     int j_nlights = P.nlights, j_max_depth = P.max_depth;
+    // and those every new sample reads
+    const Camera *j_camera = P.camera;
+    int j_width = P.width, j_height = P.height, j_spp = P.spp, j_currentSample = P.currentSample;
 #define JSHAPES (MULTI ? j_shapes : P.shapes)
+#define JFR(field) (MULTI ? j_##field : P.field)
     // the tree of the lane's current query (uniform for a single frame)
     const TNode *t_nodes = P.acc.nodes;
     const TNode4 *t_nodes4 = P.acc.nodes4;
@@ -794,16 +798,21 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         j_lights = P.jobs[jb].lights;
                         j_nlights = P.jobs[jb].nlights;
                         j_max_depth = P.jobs[jb].max_depth;
+                        j_camera = P.jobs[jb].camera;
+                        j_width = P.jobs[jb].width;
+                        j_height = P.jobs[jb].height;
+                        j_spp = P.jobs[jb].spp;
+                        j_currentSample = P.jobs[jb].currentSample;
                     } else {
                         gid = (int)my;
                     }
-                    const int W = JF(width), H = JF(height);
+                    const int W = JFR(width), H = JFR(height);
                     const int x = gid % W, y = gid / W;
                     s0 = JF(seeds)[2 * gid];
                     s1 = JF(seeds)[2 * gid + 1];
                     k = 0;
-                    if (JF(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
-                    camera_ray(MULTI ? JF(camera) : &s_cam, s0, s1, W, H, x, y, ro, rd);
+                    if (JFR(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
+                    camera_ray(MULTI ? j_camera : &s_cam, s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
@@ -873,12 +882,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             }
             if (st == kSeg && depth > (MULTI ? j_max_depth : P.max_depth)) st = kEnded;
             if (st == kEnded) {
-                col = running_mean(col, rad, JF(currentSample) + k);
+                col = running_mean(col, rad, JFR(currentSample) + k);
                 k++;
-                const int W = JF(width), H = JF(height);
+                const int W = JFR(width), H = JFR(height);
                 const int x = gid % W, y = gid / W;
-                if (k < JF(spp)) {
-                    camera_ray(MULTI ? JF(camera) : &s_cam, s0, s1, W, H, x, y, ro, rd);
+                if (k < JFR(spp)) {
+                    camera_ray(MULTI ? j_camera : &s_cam, s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
@@ -922,6 +931,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     }
 #undef JF
 #undef JSHAPES
+#undef JFR
 }
 
 // Batch queries (Intersect / IntersectP on arbitrary rays).
