@@ -589,6 +589,9 @@ typedef struct {
     const Bound *kd_box;
     int quirks;                /* 1: reproduce the reference literally (Q2 both-leaf read, 18-entry
                                   stack that gives up on overflow, first-found wins ties) */
+    int shadow_accel;          /* 1 (mode 1, quirks 0): answer IntersectP through the BVH instead of
+                                  the reference's brute-force scan.  The answer is the same predicate
+                                  over the same shapes (0..shapeCnt-2), so only the time differs. */
 } OrAccel;
 
 static inline void or_take(float d, unsigned int id, float *t, unsigned int *best, int quirks) {
@@ -718,13 +721,80 @@ int or_intersect_p(const Shape *shapes, unsigned int shapeCnt, const Ray *r, flo
     return 0;
 }
 
+/* IntersectP's predicate (a shape i < shapeCnt-1 hit at 0 != t < maxt) answered by a walk of the
+ * reference-layout BVH with the closest-hit walk's box test and child rule.  Every shape the ray hits
+ * lies in a box the walk visits, so the answer equals or_intersect_p's (checked against it on random
+ * rays in the tests). */
+static int or_intersect_p_bvh(const Shape *shapes, unsigned int shapeCnt, const OrAccel *acc, const Ray *r,
+                              float maxt, int sem) {
+    if (shapeCnt <= 1) return 0;
+    auto test = [&](int leaf) {
+        const unsigned int sid = (unsigned)acc->btl[leaf].nShape;
+        if (sid == shapeCnt - 1) return 0;
+        const float d = or_shape_intersect(&shapes[sid], r, sem);
+        return (d != 0.f && d < maxt) ? 1 : 0;
+    };
+    std::vector<int> stack;
+    stack.reserve(64);
+    stack.push_back(0);
+    while (!stack.empty()) {
+        const int n = stack.back();
+        stack.pop_back();
+        const BVHNodeGPU &nd = acc->btn[n];
+        if (!or_bound_test(r, &nd.bound)) continue;
+        if (nd.nRight == nd.max) { if (test(nd.nRight)) return 1; } else stack.push_back(nd.nRight);
+        if (nd.nLeft == nd.min) { if (test(nd.nLeft)) return 1; } else stack.push_back(nd.nLeft);
+    }
+    return 0;
+}
+
+/* sin and cos of a sampling angle.  Kernel semantics (sem 0): OpenCL leaves sin/cos to the
+ * implementation within 4 ulp; the kernel semantics fix one evaluation -- x reduced modulo pi/2 with
+ * pi/2 = hi + lo (hi with 33 significant bits), the Taylor series of sin to r^11 and of cos to r^12
+ * on [-pi/4, pi/4], all in binary64, rounded once to binary32.  Host-twin semantics (sem 1): the C
+ * library's sinf/cosf, as the reference's host build calls them (geomfunc.h). */
+void or_sincos(float xf, float *s, float *c, int sem) {
+    if (sem != 0) {
+        *s = sinf(xf);
+        *c = cosf(xf);
+        return;
+    }
+    const double two_over_pi = 0.6366197723675814, pio2_hi = 1.5707963267341256, pio2_lo = 6.077100506506192e-11;
+    const double x = xf;
+    const double k = rint(x * two_over_pi);
+    const double r = (x - k * pio2_hi) - k * pio2_lo;
+    const double r2 = r * r;
+    /* sin r = r - r^3/3! + r^5/5! - ... - r^11/11!;  cos r = 1 - r^2/2! + ... + r^12/12! */
+    const double sin_r =
+        r + r * r2 *
+                (-0.16666666666666666 +
+                 r2 * (0.008333333333333333 +
+                       r2 * (-0.0001984126984126984 + r2 * (2.7557319223985893e-06 + r2 * -2.505210838544172e-08))));
+    const double cos_r =
+        1.0 + r2 * (-0.5 + r2 * (0.041666666666666664 +
+                                 r2 * (-0.001388888888888889 +
+                                       r2 * (2.48015873015873e-05 +
+                                             r2 * (-2.755731922398589e-07 + r2 * 2.08767569878681e-09)))));
+    double sv, cv;
+    switch ((int)k & 3) {             /* x = k pi/2 + r */
+        case 0: sv = sin_r; cv = cos_r; break;
+        case 1: sv = cos_r; cv = -sin_r; break;
+        case 2: sv = -sin_r; cv = -cos_r; break;
+        default: sv = -cos_r; cv = sin_r; break;
+    }
+    *s = (float)sv;
+    *c = (float)cv;
+}
+
 /* UniformSampleSphere rendering_kernel.cl:127-135 */
-static Vec or_uniform_sample_sphere(float u1, float u2) {
+static Vec or_uniform_sample_sphere(float u1, float u2, int sem) {
     const float zz = 1.f - 2.f * u1;
     float q = 1.f - zz * zz;
     const float r = sqrtf(0.f < q ? q : 0.f);   /* OpenCL max(0,q) / vec.h max macro agree off NaN */
     const float phi = 2.f * OR_PI * u2;
-    return v3(r * cosf(phi), r * sinf(phi), zz);
+    float sphi, cphi;
+    or_sincos(phi, &sphi, &cphi, sem);
+    return v3(r * cphi, r * sphi, zz);
 }
 
 static inline float or_clampf(float x, float a, float b, int sem) {
@@ -756,7 +826,8 @@ static inline void or_draw_pair(unsigned int *s0, unsigned int *s1, float *u1, f
 /* SampleLights rendering_kernel.cl:397-493 */
 static Vec or_sample_lights(const Shape *shapes, unsigned int shapeCnt, unsigned int lightCnt,
                             unsigned int *s0, unsigned int *s1, const Vec &hitPoint,
-                            const Vec &normal, int sem, OrStats *st) {
+                            const Vec &normal, int sem, OrStats *st, const OrAccel *acc) {
+    const bool bvh_shadow = acc && acc->shadow_accel && acc->mode == 1 && !acc->quirks && shapeCnt > 1;
     Vec result = v3(0.f, 0.f, 0.f);
     unsigned int i, visited;
     for (i = 0, visited = 0; i < shapeCnt && visited < lightCnt; i++) {
@@ -779,7 +850,7 @@ static Vec or_sample_lights(const Shape *shapes, unsigned int shapeCnt, unsigned
         } else {
             float u1, u2;
             or_draw_pair(s0, s1, &u1, &u2, sem);
-            Vec usp = or_uniform_sample_sphere(u1, u2);
+            Vec usp = or_uniform_sample_sphere(u1, u2, sem);
             lightPoint = vadd(vsmul(light->s.rad, usp), light->s.p);
             lightN = usp;
         }
@@ -793,7 +864,9 @@ static Vec or_sample_lights(const Shape *shapes, unsigned int shapeCnt, unsigned
         const float wi = vdot(shadowRay.d, normal);
         if (wi > 0.f) {
             if (st) st->anyhit++;
-            if (!or_intersect_p(shapes, shapeCnt, &shadowRay, len - OR_EPSILON, sem)) {
+            const int occluded = bvh_shadow ? or_intersect_p_bvh(shapes, shapeCnt, acc, &shadowRay, len - OR_EPSILON, sem)
+                                            : or_intersect_p(shapes, shapeCnt, &shadowRay, len - OR_EPSILON, sem);
+            if (!occluded) {
                 const float s = light->area * wi * wo / (len * len);
                 result = vadd(result, vsmul(s, light->e));
             }
@@ -836,7 +909,7 @@ static Vec or_radiance(const Shape *shapes, unsigned int shapeCnt, unsigned int 
         if (s.refl == DIFF) {
             specularBounce = 0;
             thr = vmul(thr, s.c);
-            Vec Ld = or_sample_lights(shapes, shapeCnt, lightCnt, s0, s1, hitPoint, nl, sem, st);
+            Vec Ld = or_sample_lights(shapes, shapeCnt, lightCnt, s0, s1, hitPoint, nl, sem, st, acc);
             Ld = vmul(thr, Ld);
             rad = vadd(rad, Ld);
             float r1 = 2.f * OR_PI * or_get_random(s0, s1);
@@ -846,8 +919,10 @@ static Vec or_radiance(const Shape *shapes, unsigned int shapeCnt, unsigned int 
             Vec a = (fabsf(w.x) > .1f) ? v3(0.f, 1.f, 0.f) : v3(1.f, 0.f, 0.f);
             Vec u = vnorm(vxcross(a, w));
             Vec v = vxcross(w, u);
-            u = vsmul(cosf(r1) * r2s, u);
-            v = vsmul(sinf(r1) * r2s, v);
+            float sr1, cr1;
+            or_sincos(r1, &sr1, &cr1, sem);
+            u = vsmul(cr1 * r2s, u);
+            v = vsmul(sr1 * r2s, v);
             Vec newDir = vadd(u, v);
             w = vsmul(sqrtf(1 - r2), w);
             newDir = vadd(newDir, w);
@@ -925,6 +1000,19 @@ static inline int or_to_int(float x, int sem) {
     return (int)(powf(or_clampf(x, 0.f, 1.f, sem), 1.f / 2.2f) * 255.f + .5f);
 }
 
+/* GenerateCameraRay for each listed pixel id (x = gid % width, y = gid / width), drawing from and
+ * advancing that pixel's seeds[2 gid], seeds[2 gid + 1] as RadianceGPU does (rendering_kernel.cl:1100-1110). */
+void or_camera_rays(const Camera *cam, unsigned int *seeds, int width, int height, const int *pix, int n,
+                    Ray *out) {
+    for (int q = 0; q < n; q++) {
+        const int gid = pix[q], x = gid % width, y = gid / width;
+        unsigned int s0 = seeds[2 * gid], s1 = seeds[2 * gid + 1];
+        out[q] = or_camera_ray(cam, &s0, &s1, width, height, x, y);
+        seeds[2 * gid] = s0;
+        seeds[2 * gid + 1] = s1;
+    }
+}
+
 /* One pixel of RadianceGPU rendering_kernel.cl:1100-1149, for `spp` successive launches starting
  * at currentSample.  colors/seeds/pixels are the full frame buffers. */
 static void or_pixel(int gid, Vec *colors, unsigned int *seeds, const Camera *cam,
@@ -987,6 +1075,11 @@ void or_trace_closest(const Shape *shapes, unsigned int shapeCnt, const OrAccel 
         else { t_out[i] = 0.f; id_out[i] = -1; }
     }
 }
+void or_trace_any_bvh(const Shape *shapes, unsigned int shapeCnt, const OrAccel *acc, const Ray *rays,
+                      const float *maxt, int n, int *hit_out, int sem, int nthreads) {
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int i = 0; i < n; i++) hit_out[i] = or_intersect_p_bvh(shapes, shapeCnt, acc, &rays[i], maxt[i], sem);
+}
 void or_trace_any(const Shape *shapes, unsigned int shapeCnt, const Ray *rays, const float *maxt,
                   int n, int *hit_out, int sem, int nthreads) {
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
@@ -1027,7 +1120,7 @@ extern "C" {
 /* SampleLights for one hit point (result3 = Ld); exposed for unit tests. */
 void or_sample_lights_at(const Shape *shapes, unsigned int shapeCnt, unsigned int lightCnt, unsigned int *s0,
                          unsigned int *s1, const Vec *hitPoint, const Vec *normal, int sem, float *result3) {
-    Vec r = or_sample_lights(shapes, shapeCnt, lightCnt, s0, s1, *hitPoint, *normal, sem, nullptr);
+    Vec r = or_sample_lights(shapes, shapeCnt, lightCnt, s0, s1, *hitPoint, *normal, sem, nullptr, nullptr);
     result3[0] = r.x; result3[1] = r.y; result3[2] = r.z;
 }
 }

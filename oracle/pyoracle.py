@@ -20,7 +20,7 @@ vp, i32, u32, f32 = C.c_void_p, C.c_int, C.c_uint, C.c_float
 
 class OrAccel(C.Structure):
     _fields_ = [("mode", i32), ("btn", vp), ("btl", vp), ("kng", vp), ("kn", vp), ("kd_node5", vp),
-                ("kd_box", vp), ("quirks", i32)]
+                ("kd_box", vp), ("quirks", i32), ("shadow_accel", i32)]
 
 
 def _p(a):
@@ -65,7 +65,10 @@ def lib():
                                          i32, i32, vp, i32, i32, vp]),
             "or_trace_closest": (None, [vp, u32, C.POINTER(OrAccel), vp, i32, vp, vp, i32, i32]),
             "or_trace_any": (None, [vp, u32, vp, vp, i32, vp, i32, i32]),
+            "or_trace_any_bvh": (None, [vp, u32, C.POINTER(OrAccel), vp, vp, i32, vp, i32, i32]),
             "or_update_camera": (None, [vp]),
+            "or_camera_rays": (None, [vp, vp, i32, i32, vp, i32, vp]),
+            "or_sincos": (None, [f32, vp, vp, i32]),
             "or_accel_size": (C.c_size_t, []),
         }
         for name, (res, args) in sig.items():
@@ -215,19 +218,21 @@ class KD:
 
 
 def accel_none():
-    return OrAccel(0, None, None, None, None, None, None, 0)
+    return OrAccel(0, None, None, None, None, None, None, 0, 0)
 
 
-def accel_bvh(btn, btl, quirks=0):
-    return OrAccel(1, _p(btn) if len(btn) else None, _p(btl), None, None, None, None, quirks)
+def accel_bvh(btn, btl, quirks=0, shadow_accel=False):
+    """BVH mode over reference-layout arrays.  shadow_accel: shadow rays (IntersectP) walk the BVH
+    instead of the reference's brute-force scan -- the same answers, for large scenes."""
+    return OrAccel(1, _p(btn) if len(btn) else None, _p(btl), None, None, None, None, quirks, int(shadow_accel))
 
 
 def accel_kd_heap(kng, kn, quirks=0):
-    return OrAccel(2, None, None, _p(kng), _p(kn), None, None, quirks)
+    return OrAccel(2, None, None, _p(kng), _p(kn), None, None, quirks, 0)
 
 
 def accel_kd_dfs(node5, box, kn):
-    return OrAccel(3, None, None, None, _p(kn), _p(node5), _p(box), 0)
+    return OrAccel(3, None, None, None, _p(kn), _p(node5), _p(box), 0, 0)
 
 
 def trace_closest(shapes, acc: OrAccel, rays, sem=0, threads=8):
@@ -244,6 +249,13 @@ def trace_any(shapes, rays, maxt, sem=0, threads=8):
     return hit
 
 
+def trace_any_bvh(shapes, acc: OrAccel, rays, maxt, sem=0, threads=8):
+    hit = np.empty(len(rays), dtype=np.int32)
+    lib().or_trace_any_bvh(_p(shapes), len(shapes), C.byref(acc), _p(rays),
+                           _p(np.ascontiguousarray(maxt, dtype=np.float32)), len(rays), _p(hit), sem, threads)
+    return hit
+
+
 def radiance_frame(colors, seeds, cam, shapes, light_cnt, acc: OrAccel, width, height, current_sample, spp,
                    pixels, max_depth, sem=0, pix_list=None, threads=8):
     """`spp` RadianceGPU passes (in place on colors/seeds/pixels).  Returns (closest, any-hit) counts."""
@@ -254,6 +266,22 @@ def radiance_frame(colors, seeds, cam, shapes, light_cnt, acc: OrAccel, width, h
                             max_depth, sem, _p(pl) if pl is not None else None, len(pl) if pl is not None else 0,
                             threads, _p(stats))
     return int(stats[0]), int(stats[1])
+
+
+def sincos(x: float, sem: int = 0):
+    """(sin x, cos x) as the kernel (sem 0) or host-twin (sem 1) semantics evaluate them."""
+    s, c = f32(), f32()
+    lib().or_sincos(float(x), C.byref(s), C.byref(c), sem)
+    return np.float32(s.value), np.float32(c.value)
+
+
+def camera_rays(cam, seeds, width, height, pix):
+    """GenerateCameraRay for pixel ids `pix`; advances `seeds` (2 per pixel) in place."""
+    from ptgpu_pc_amd.layout import RAY_DTYPE
+    pix = np.ascontiguousarray(pix, dtype=np.int32)
+    out = np.zeros(len(pix), dtype=RAY_DTYPE)
+    lib().or_camera_rays(_p(cam), _p(seeds), width, height, _p(pix), len(pix), _p(out))
+    return out
 
 
 def update_camera(cam: np.ndarray) -> np.ndarray:

@@ -178,6 +178,7 @@ struct ptgpu_ctx {
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
+    ptgpu::DevBuf tmp_parents;           // parent map and height of a caller's BVH (ptgpu_radiance_gpu)
     ptgpu::BatchChannel batch;            // ptgpu_render_batch launches (on the context's stream)
     ptgpu::BuildStaging build_staging;    // batched LBVH builds
     bool capturing = false;               // between ptgpu_graph_begin and ptgpu_graph_end

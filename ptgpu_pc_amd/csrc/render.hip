@@ -1146,6 +1146,43 @@ __global__ void k_convert_bvh(const Shape *shapes, int n, const BVHNodeGPU *btn,
     }
 }
 
+// Parents of a caller's reference-layout tree, taken from the child links the walk follows (the
+// nParent fields are not trusted): par[i] for internal node i, par[n-1+k] for leaf k, -1 where no
+// node points.  err[0] collects malformed links: 1 a child index out of range, 2 a node with two
+// parents or a link back to the root.
+__global__ void k_ref_parents(const BVHNodeGPU *btn, int n, int *par, unsigned *err) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int nl = btn[i].nLeft, nr = btn[i].nRight;
+    const bool lleaf = nl == btn[i].min, rleaf = nr == btn[i].max;
+    if ((unsigned)nl >= (unsigned)(lleaf ? n : n - 1) || (unsigned)nr >= (unsigned)(rleaf ? n : n - 1)) {
+        atomicOr(err, 1u);
+        return;
+    }
+    const int sl = lleaf ? n - 1 + nl : nl, sr = rleaf ? n - 1 + nr : nr;
+    if (sl == 0 || sr == 0 || sl == sr) atomicOr(err, 2u);
+    if (atomicExch(par + sl, i) != -1) atomicOr(err, 2u);
+    if (sl != sr && atomicExch(par + sr, i) != -1) atomicOr(err, 2u);
+}
+
+// Height of that tree: the longest walk from a leaf up to the root (node 0), in edges, into err[1].
+// A leaf no node points at sets err[0] bit 4; a walk longer than kMaxRefHeight (too deep for the
+// traversal stack, or a cycle) sets bit 8; a leaf whose nShape is not a shape index sets bit 16.
+constexpr int kMaxRefHeight = 62;
+__global__ void k_ref_height(const BVHNodeGPU *btl, const int *par, int n, unsigned *err) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if ((unsigned)btl[k].nShape >= (unsigned)n) atomicOr(err, 16u);
+    int p = par[n - 1 + k], d = 1;
+    while (p > 0 && d <= kMaxRefHeight) {
+        p = par[p];
+        d++;
+    }
+    if (p < 0) atomicOr(err, 4u);
+    else if (p != 0) atomicOr(err, 8u);
+    else atomicMax(err + 1, (unsigned)d);
+}
+
 // reference KD heap layout (1-based, children 2i/2i+1, leaves own kn[min..max)) -> TNode/Prim
 __global__ void k_convert_kd(const Shape *shapes, const KDNodeGPU *kng, int kngCnt, const int *kn, int knCnt,
                              TNode *nodes, Prim *prims) {
@@ -1227,6 +1264,38 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
 
 int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn, const BVHNodeGPU *d_btl,
                     DevBuf &nodes, DevBuf &prims, DevAccel *acc) {
+    acc->depth = 0;
+    if (n >= 2) {
+        // The caller's links are checked before any kernel follows them, and the stack is sized from
+        // the real height: a caller's tree (getTrees after optimize, or any other builder's) can be
+        // much deeper than ceil(log2 n).
+        PT_NO_CAPTURE(ctx, "measuring a caller's BVH");
+        PT_TRY(ctx->tmp_parents.ensure((size_t)(2 * n - 1) * 4 + 64));
+        int *par = ctx->tmp_parents.as<int>();
+        unsigned *err = reinterpret_cast<unsigned *>(ctx->tmp_parents.as<char>() + (size_t)(2 * n - 1) * 4);
+        PT_TRY(fill_u32(ctx->stream, par, 0xFFFFFFFFu, (size_t)(2 * n - 1)));
+        PT_TRY(fill_u32(ctx->stream, err, 0u, 2));
+        hipLaunchKernelGGL(k_ref_parents, dim3((n + 254) / 256), dim3(256), 0, ctx->stream, d_btn, n, par, err);
+        PT_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_ref_height, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_btl, par, n, err);
+        PT_HIP(hipGetLastError());
+        unsigned h_err[2] = {0, 0};
+        PT_HIP(hipMemcpyAsync(h_err, err, 8, hipMemcpyDeviceToHost, ctx->stream));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        PT_CHECK(!(h_err[0] & 3u), PTGPU_ERR_INVALID, "btn/btl: %s",
+                 (h_err[0] & 1u) ? "a child index is out of range" : "a node has two parents (not a tree)");
+        PT_CHECK(!(h_err[0] & 4u), PTGPU_ERR_INVALID, "btn/btl: a leaf is not reachable from the root");
+        PT_CHECK(!(h_err[0] & 16u), PTGPU_ERR_INVALID, "btl: a leaf's nShape is out of range");
+        PT_CHECK(!(h_err[0] & 8u), PTGPU_ERR_UNSUPPORTED, "btn/btl: tree deeper than %d levels (or cyclic)",
+                 kMaxRefHeight);
+        acc->depth = (int)h_err[1];
+    } else {
+        PT_CHECK(n == 1, PTGPU_ERR_INVALID, "shapeCnt %d < 1", n);
+        int ns = -1;
+        PT_HIP(hipMemcpyAsync(&ns, &d_btl[0].nShape, 4, hipMemcpyDeviceToHost, ctx->stream));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        PT_CHECK(ns == 0, PTGPU_ERR_INVALID, "btl: a leaf's nShape is out of range");
+    }
     PT_TRY(nodes.ensure((size_t)(n > 1 ? n - 1 : 1) * sizeof(TNode)));
     PT_TRY(prims.ensure((size_t)n * sizeof(Prim)));
     hipLaunchKernelGGL(k_convert_bvh, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_shapes, n, d_btn, d_btl,
@@ -1239,17 +1308,8 @@ int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeG
     acc->nshapes = n;
     acc->mode = PTGPU_ACCEL_BVH;
     acc->leaf_runs = 1;
-    if (n >= 2) {
-        acc->root = 0;
-        acc->root_cnt = 0;
-        int d = 0;
-        while ((1ll << d) < n) d++;
-        acc->depth = d;
-    } else {
-        acc->root = ~0;
-        acc->root_cnt = 1;
-        acc->depth = 0;
-    }
+    acc->root = n >= 2 ? 0 : ~0;
+    acc->root_cnt = n >= 2 ? 0 : 1;
     return PTGPU_OK;
 }
 
@@ -1451,6 +1511,9 @@ int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) 
     PT_CHECK(a->colors && a->seeds && a->camera && a->shapes && a->pixels, PTGPU_ERR_INVALID, "null buffer");
     PT_CHECK(a->shapeCnt >= 1, PTGPU_ERR_INVALID, "shapeCnt is 0");
     PT_CHECK(a->width > 0 && a->height > 0, PTGPU_ERR_INVALID, "bad frame size %dx%d", a->width, a->height);
+    PT_CHECK((unsigned long long)a->width * (unsigned long long)a->height < (1ull << 30), PTGPU_ERR_UNSUPPORTED,
+             "frame %dx%d: 2^30 pixels or more (seed indices 2*pixel must stay within int)", a->width,
+             a->height);
     PT_CHECK(a->currentSample >= 0, PTGPU_ERR_INVALID, "currentSample < 0");
     PT_HIP(hipSetDevice(ctx->device));
     const int n = (int)a->shapeCnt;
@@ -1494,6 +1557,9 @@ int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu
     PT_CHECK(a->colors && a->seeds && a->camera && a->shapes, PTGPU_ERR_INVALID, "null buffer");
     PT_CHECK(a->shapeCnt >= 1, PTGPU_ERR_INVALID, "shapeCnt is 0");
     PT_CHECK(a->width > 0 && a->height > 0, PTGPU_ERR_INVALID, "bad frame size %dx%d", a->width, a->height);
+    PT_CHECK((unsigned long long)a->width * (unsigned long long)a->height < (1ull << 30), PTGPU_ERR_UNSUPPORTED,
+             "frame %dx%d: 2^30 pixels or more (seed indices 2*pixel must stay within int)", a->width,
+             a->height);
     PT_CHECK(a->spp >= 1 && a->currentSample >= 0, PTGPU_ERR_INVALID, "bad spp/currentSample");
     PT_CHECK(a->max_depth >= 0 && a->max_depth < 1024, PTGPU_ERR_INVALID, "bad max_depth");
     PT_HIP(hipSetDevice(ctx->device));
@@ -1566,6 +1632,8 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         PT_CHECK(a->shapeCnt >= 2, PTGPU_ERR_UNSUPPORTED, "job %d: a batch needs at least two shapes per scene", j);
         PT_CHECK(a->width > 0 && a->height > 0, PTGPU_ERR_INVALID, "job %d: bad frame size %dx%d", j, a->width,
                  a->height);
+        PT_CHECK((unsigned long long)a->width * (unsigned long long)a->height < (1ull << 30), PTGPU_ERR_UNSUPPORTED,
+                 "job %d: frame %dx%d has 2^30 pixels or more", j, a->width, a->height);
         PT_CHECK(a->spp >= 1 && a->currentSample >= 0, PTGPU_ERR_INVALID, "job %d: bad spp/currentSample", j);
         PT_CHECK(a->max_depth >= 0 && a->max_depth < 1024, PTGPU_ERR_INVALID, "job %d: bad max_depth", j);
         RenderJob &R = table[(size_t)j];

@@ -30,6 +30,35 @@ __device__ __forceinline__ f3 ldvec(const Vec *p) {
     return mk(f[0], f[1], f[2]);
 }
 
+// ---- sin and cos of the sampling angles (rendering_kernel.cl:131-132, 652-654) ----
+// The reference calls OpenCL's sin/cos, whose results the OpenCL specification bounds (4 ulp) but
+// does not fix.  This library evaluates them one documented way: reduction of x modulo pi/2 with
+// pi/2 split in two parts (the high part has 33 significant bits, so k * hi is exact), then the
+// Taylor series of sin (to r^11) and cos (to r^12) on [-pi/4, pi/4], all in binary64, rounded once
+// to binary32.  The truncation error is below 1e-11 there, so each result is within a hair of
+// correct rounding, and the same IEEE binary64 operations give the same bits in any conforming
+// implementation of them (the CPU restatement uses this evaluation too).
+__device__ __forceinline__ void sincos_f32(float xf, float *s, float *c) {
+    const double x = xf;
+    const double k = rint(x * 0.6366197723675814);                 // 2/pi
+    const double r = (x - k * 1.5707963267341256) - k * 6.077100506506192e-11;
+    const double r2 = r * r;
+    const double sp =
+        r + r * r2 *
+                (-0.16666666666666666 +
+                 r2 * (0.008333333333333333 +
+                       r2 * (-0.0001984126984126984 + r2 * (2.7557319223985893e-06 + r2 * -2.505210838544172e-08))));
+    const double cp =
+        1.0 + r2 * (-0.5 + r2 * (0.041666666666666664 +
+                                 r2 * (-0.001388888888888889 +
+                                       r2 * (2.48015873015873e-05 +
+                                             r2 * (-2.755731922398589e-07 + r2 * 2.08767569878681e-09)))));
+    const int q = (int)k & 3;                                    // quadrant
+    const double sv = (q & 1) ? cp : sp, cv = (q & 1) ? sp : cp;
+    *s = (float)((q & 2) ? -sv : sv);
+    *c = (float)(((q + 1) & 2) ? -cv : cv);
+}
+
 // ---- GetRandom, rendering_kernel.cl:4-17 ----
 __device__ __forceinline__ float get_random(unsigned &s0, unsigned &s1) {
     s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
@@ -329,7 +358,9 @@ __device__ __forceinline__ bool sample_light_words(const float *lw, float area, 
         const float q = 1.f - zz * zz;
         const float rr = sqrtf(0.f < q ? q : 0.f);
         const float phi = 2.f * kPi * u2;
-        const f3 usp = mk(rr * cosf(phi), rr * sinf(phi), zz);
+        float sphi, cphi;
+        sincos_f32(phi, &sphi, &cphi);
+        const f3 usp = mk(rr * cphi, rr * sphi, zz);
         lp = add3(smul(lw[1], usp), mk(lw[2], lw[3], lw[4]));
         ln = usp;
     }
@@ -370,8 +401,10 @@ __device__ __forceinline__ f3 diffuse_dir(f3 nl, unsigned &s0, unsigned &s1) {
     const f3 aa = fabsf(ww.x) > .1f ? mk(0.f, 1.f, 0.f) : mk(1.f, 0.f, 0.f);
     f3 u = norm3(cross3(aa, ww));
     f3 v = cross3(ww, u);
-    u = smul(cosf(r1) * r2s, u);
-    v = smul(sinf(r1) * r2s, v);
+    float sr1, cr1;
+    sincos_f32(r1, &sr1, &cr1);
+    u = smul(cr1 * r2s, u);
+    v = smul(sr1 * r2s, v);
     f3 nd = add3(u, v);
     ww = smul(sqrtf(1 - r2), ww);
     return add3(nd, ww);

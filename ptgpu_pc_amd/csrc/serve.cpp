@@ -147,10 +147,13 @@ static bool take_pooled(ptgpu_server *s, size_t bytes, ptgpu::DevBuf *out) {
     return true;
 }
 
-static bool pool_has(const ptgpu_server *s, size_t bytes) {
-    for (const auto &b : s->pool)
-        if (b.bytes >= bytes) return true;
-    return false;
+// undoes the creation of a scene that admission brought in but could not use
+static void drop_new_scene(ptgpu_server *s, Scene *sc) {
+    s->unbuilt.erase(std::remove(s->unbuilt.begin(), s->unbuilt.end(), sc), s->unbuilt.end());
+    s->scenes.erase(sc->key);
+    s->used -= sc->bytes;
+    release_scene(sc);
+    delete sc;
 }
 
 static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
@@ -161,11 +164,33 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
     PT_CHECK(!sc || (sc->light_cnt == r.lightCnt && sc->n == (int)r.shapeCnt), PTGPU_ERR_INVALID,
              "request %d: scene key %llu is resident with other shapes or lightCnt", q.id, r.scene_key);
     const size_t fb = frame_bytes(r.width, r.height);
-    const size_t sb = sc ? 0 : (size_t)r.shapeCnt * kSceneBytesPerPrim;
-    const size_t fresh = pool_has(s, fb) ? 0 : fb;   // a pooled frame buffer needs no new memory
+    size_t sb = sc ? 0 : (size_t)r.shapeCnt * kSceneBytesPerPrim;
+    // A pooled frame buffer needs no new memory.  It leaves the pool before the eviction below, which
+    // frees pooled buffers first and must not free this one.
+    ptgpu::DevBuf pooled;
+    const bool have_pooled = take_pooled(s, fb, &pooled);
+    const size_t fresh = have_pooled ? 0 : fb;
+    // The request's own scene may be idle (refs 0) and the least recently used: it must survive.
+    if (sc) sc->refs++;
     evict_for(s, fresh + sb);
-    if (s->used + fresh + sb > s->budget) return PTGPU_OK;   // wait for memory
+    if (sc) sc->refs--;
+    if (sc && sc->refs == 0 && s->used + fresh > s->budget) {
+        // Only the request's own idle scene stands in the way (its built size can exceed the estimate
+        // that let the request in): drop it and bring the scene in anew, as for a first request.
+        s->used -= sc->bytes;
+        release_scene(sc);
+        s->scenes.erase(sc->key);
+        delete sc;
+        sc = nullptr;
+        sb = (size_t)r.shapeCnt * kSceneBytesPerPrim;
+        evict_for(s, fresh + sb);
+    }
+    if (s->used + fresh + sb > s->budget) {   // wait for memory
+        if (have_pooled) s->pool.push_back(pooled);
+        return PTGPU_OK;
+    }
     hipStream_t st = s->ctx->stream;
+    bool created = false;
     if (!sc) {
         sc = new Scene();
         sc->key = r.scene_key;
@@ -180,21 +205,26 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
         if (e != PTGPU_OK) {
             release_scene(sc);
             delete sc;
+            if (have_pooled) s->pool.push_back(pooled);
             return e;
         }
         sc->bytes = sb;   // the estimate until the build reports the real footprint
         s->used += sc->bytes;
         s->scenes[sc->key] = sc;
         s->unbuilt.push_back(sc);
+        created = true;
     }
     Slot *sl = new Slot();
     sl->id = q.id;
     sl->r = r;
     sl->scene = sc;
-    if (!take_pooled(s, fb, &sl->frame)) {
+    if (have_pooled) {
+        sl->frame = pooled;
+    } else {
         int e = sl->frame.ensure(fb);
         if (e != PTGPU_OK) {
             delete sl;
+            if (created) drop_new_scene(s, sc);   // refs 0: nothing else holds it yet
             return e;
         }
         s->used += sl->frame.bytes;
@@ -205,14 +235,21 @@ static int admit(ptgpu_server *s, const Request &q, bool *fitted) {
     const size_t oc = 0, os = (px * 12 + 255) & ~size_t(255), op = os + ((px * 8 + 255) & ~size_t(255));
     const size_t ocam = op + ((px * 4 + 255) & ~size_t(255));
     (void)oc;
+    hipError_t he;
     if (r.h_seeds) {
-        PT_HIP(hipMemcpyAsync(base + os, r.h_seeds, px * 8, hipMemcpyHostToDevice, st));
+        he = hipMemcpyAsync(base + os, r.h_seeds, px * 8, hipMemcpyHostToDevice, st);
     } else {
         hipLaunchKernelGGL(k_default_seeds, dim3((unsigned)((px + 255) / 256)), dim3(256), 0, st,
                            reinterpret_cast<unsigned *>(base + os), (unsigned)px);
-        PT_HIP(hipGetLastError());
+        he = hipGetLastError();
     }
-    PT_HIP(hipMemcpyAsync(base + ocam, &sl->r.camera, sizeof(Camera), hipMemcpyHostToDevice, st));
+    if (he == hipSuccess) he = hipMemcpyAsync(base + ocam, &sl->r.camera, sizeof(Camera), hipMemcpyHostToDevice, st);
+    if (he != hipSuccess) {   // the frame buffer stays counted in `used`, in the pool
+        s->pool.push_back(sl->frame);
+        delete sl;
+        if (created) drop_new_scene(s, sc);
+        return hip_fail(he, "admit: upload", __FILE__, __LINE__);
+    }
     sc->refs++;
     sc->last_use = s->tick_no;
     s->active.push_back(sl);

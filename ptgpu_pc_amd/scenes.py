@@ -224,3 +224,43 @@ def mixed_scene(seed: int, n: int, width: int = 256, height: int = 256):
     light = triangles_to_shapes(quad, e=(12.0, 12.0, 12.0), c=(0.0, 0.0, 0.0))
     shapes = concat_shapes([body, light])
     return shapes, 2, orbit_camera(yaw, width, height), kind
+
+
+def batch_set(n_scenes: int = 64, seed: int = 4000):
+    """The many-scene workload: independent scenes with triangle counts log-uniform in [1e3, 1e6],
+    soup or displaced icosphere by scene seed, a camera at a random orbit angle (UpdateCamera
+    applied), 256^2 / 512^2 / 1024^2 by seed; rendered at 4 spp, depth 5."""
+    from . import lib
+    out = []
+    for k in range(n_scenes):
+        ss = seed + k
+        r = np.random.default_rng([ss, 1])
+        n = int(np.exp(r.uniform(np.log(1e3), np.log(1e6))))
+        res = int(r.choice([256, 512, 1024]))
+        shapes, lc, cam, kind = mixed_scene(ss, n, res, res)
+        lib.update_camera(cam)
+        out.append({"shapes": shapes, "lc": lc, "cam": cam, "w": res, "h": res, "kind": kind, "seed": ss})
+    return out
+
+
+def serve_trace(n_req: int, rate: float, pool: int, seed: int = 2024):
+    """The served workload: Poisson arrivals at `rate`/s; each request picks one of `pool` scenes
+    (triangle count log-uniform in [1e3, 2e6] and mesh type from the scene seed), a camera at a
+    random orbit angle, a resolution in {256^2, 512^2, 1024^2} and spp in {1, 4, 16}; depth 5.
+    Returns (pool_scenes [(scene seed, shapes, lightCnt, kind)], requests)."""
+    rng = np.random.default_rng(seed)
+    pool_scenes = []
+    for k in range(pool):
+        ss = 1000 + k
+        n = int(np.exp(np.random.default_rng([ss, 1]).uniform(np.log(1e3), np.log(2e6))))
+        shapes, lc, _, kind = mixed_scene(ss, n, 64, 64)
+        pool_scenes.append((ss, shapes, lc, kind))
+    t, reqs = 0.0, []
+    for i in range(n_req):
+        t += rng.exponential(1.0 / rate)
+        sc = int(rng.integers(pool))
+        res = int(rng.choice([256, 512, 1024]))
+        spp = int(rng.choice([1, 4, 16]))
+        yaw = float(rng.uniform(0, 2 * np.pi))
+        reqs.append({"arrival": t, "scene": sc, "w": res, "h": res, "spp": spp, "yaw": yaw})
+    return pool_scenes, reqs

@@ -6,6 +6,7 @@ Bars (from the path's parity contract):
   Images: RMSE <= 1e-3 on clamped linear colours at matched seeds and spp.
 """
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -19,8 +20,17 @@ pytestmark = pytest.mark.gpu
 F32 = np.float32
 
 
+_T0 = time.perf_counter()
+
+
 def say(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print(f"[{time.perf_counter() - _T0:7.1f}s]", *a, file=sys.stderr, flush=True)
+
+
+def same(a, b):
+    """Bitwise equality of two arrays, as a bool: a failing assert then reports array summaries instead
+    of diffing megabytes of bytes."""
+    return a.tobytes() == b.tobytes()
 
 
 def gpu_lbvh(ctx, shapes, digit_bits=8):
@@ -286,11 +296,16 @@ def render_oracle(shapes, lc, cam, w, h, spp, depth, acc):
 
 def image_checks(c_g, s_g, p_g, c_o, s_o, p_o, tol=1e-3):
     rmse = float(np.sqrt(np.mean((clamp01(c_g) - clamp01(c_o)) ** 2)))
-    say(f"rmse {rmse:.3e}  seeds equal {np.mean(s_g == s_o):.5f}  pixels equal {np.mean(p_g == p_o):.5f}")
+    say(f"rmse {rmse:.3e}  colours bit-equal {np.mean(c_g == c_o):.5f}  seeds equal {np.mean(s_g == s_o):.5f}  "
+        f"pixels equal {np.mean(p_g == p_o):.5f}")
     assert rmse <= tol, rmse
-    assert np.mean(s_g == s_o) >= 0.999
+    # Beyond the RMSE bar: the kernel and the oracle's kernel semantics evaluate every FP32 expression
+    # in the same order with IEEE division / sqrt and one shared sin / cos, so colours and seeds are
+    # bit-equal; only the 8-bit pack's powf comes from two math libraries (within one step).
+    assert same(c_g, c_o), f"colours bit-equal on {np.mean(c_g == c_o):.6f} only"
+    assert same(s_g, s_o), f"seeds equal on {np.mean(s_g == s_o):.6f} only"
     dp = np.abs(p_g.view(np.uint8).astype(int) - p_o.view(np.uint8).astype(int))
-    assert np.mean(dp <= 1) >= 0.999
+    assert (dp <= 1).all()
 
 
 def test_render_cornell_none_vs_oracle(ctx):
@@ -303,25 +318,38 @@ def test_render_cornell_none_vs_oracle(ctx):
     assert c_g.mean() > 0.05
 
 
-def test_render_soup_bvh_vs_oracle(ctx):
+ENGINES = ["auto", "megakernel", "wavefront"]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_render_soup_bvh_vs_oracle(ctx, engine):
     shapes, lc, cam = scenes.soup(3000, seed=4, width=48, height=48)
     pt.update_camera(cam)
-    say("case soup3002 48x48x4 bvh")
+    say("case soup3002 48x48x4 bvh", engine)
     s, bvh = gpu_lbvh(ctx, shapes)
     d_sh = ctx.to_device(s)
     # the handle's traversal layout was built from `s`; render with the same shapes
-    c_g, s_g, p_g, _, _ = render_gpu(ctx, s, lc, cam, 48, 48, 4, 5, None, bvh=bvh, d_shapes=d_sh)
+    ctx.set_engine(engine)
+    try:
+        c_g, s_g, p_g, _, _ = render_gpu(ctx, s, lc, cam, 48, 48, 4, 5, None, bvh=bvh, d_shapes=d_sh)
+    finally:
+        ctx.set_engine("auto")
     _, _, btn, btl = O.build_lbvh(shapes)
     c_o, s_o, p_o = render_oracle(s, lc, cam, 48, 48, 4, 5, O.accel_bvh(btn, btl))
     image_checks(c_g, s_g, p_g, c_o, s_o, p_o)
 
 
-def test_render_icosphere_kd_vs_oracle(ctx):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_render_icosphere_kd_vs_oracle(ctx, engine):
     shapes, lc, cam = scenes.icosphere_in_box(3, seed=3, width=40, height=30)
     pt.update_camera(cam)
-    say("case icosphere l3 40x30x4 kd")
+    say("case icosphere l3 40x30x4 kd", engine)
     kd = pt.KDTree(ctx, shapes)
-    c_g, s_g, p_g, _, _ = render_gpu(ctx, shapes, lc, cam, 40, 30, 4, 6, None, kd=kd)
+    ctx.set_engine(engine)
+    try:
+        c_g, s_g, p_g, _, _ = render_gpu(ctx, shapes, lc, cam, 40, 30, 4, 6, None, kd=kd)
+    finally:
+        ctx.set_engine("auto")
     okd = O.KD(shapes)
     n5, box, kn = okd.flatten_dfs()
     c_o, s_o, p_o = render_oracle(shapes, lc, cam, 40, 30, 4, 6, O.accel_kd_dfs(n5, box, kn))
@@ -457,3 +485,4 @@ def test_render_c2_full_frame_properties(ctx):
     rmse = float(np.sqrt(np.mean((clamp01(cg) - clamp01(co)) ** 2)))
     say(f"subset rmse {rmse:.3e}, seeds {np.mean(s_g.reshape(-1, 2)[pix] == seeds.reshape(-1, 2)[pix]):.4f}")
     assert rmse <= 1e-3
+    assert same(cg, co) and same(s_g.reshape(-1, 2)[pix], seeds.reshape(-1, 2)[pix])

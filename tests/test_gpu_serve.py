@@ -11,7 +11,7 @@ import ptgpu_pc_amd as pt
 from ptgpu_pc_amd import scenes
 from ptgpu_pc_amd.layout import default_seeds
 
-from test_gpu_parity import say
+from test_gpu_parity import same, say
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -90,7 +90,7 @@ def test_served_equals_alone(ctx):
         sd = seeds if seeds is not None else default_seeds(w, h)
         a_col, a_pix, a_seed = _alone(ctx, shapes, lc, _camera(key, w, h), w, h, spp, sd)
         c, p, s = results[k]
-        assert c.tobytes() == a_col.tobytes() and p.tobytes() == a_pix.tobytes() and s.tobytes() == a_seed.tobytes(), k
+        assert same(c, a_col) and same(p, a_pix) and same(s, a_seed), k
 
 
 def test_served_small_budget_queues_and_evicts(ctx):
@@ -109,7 +109,7 @@ def test_served_small_budget_queues_and_evicts(ctx):
         shapes, lc, _ = _scene(key)
         sd = seeds if seeds is not None else default_seeds(w, h)
         a_col, _, _ = _alone(ctx, shapes, lc, _camera(key, w, h), w, h, spp, sd)
-        assert results[k][0].tobytes() == a_col.tobytes()
+        assert same(results[k][0], a_col)
 
 
 def test_server_rejects_oversized(ctx):
@@ -119,3 +119,50 @@ def test_server_rejects_oversized(ctx):
     with pytest.raises(pt.PtgpuError):
         srv.submit(1, shapes, lc, cam, 8, 8, 1)
     srv.close()
+
+
+def _frame_bytes(w, h):
+    """The server's per-request frame footprint: colors, seeds and pixels (each 256-byte aligned) + camera."""
+    px = w * h
+    r = lambda b: (b + 255) & ~255  # noqa: E731
+    return r(px * 12) + r(px * 8) + r(px * 4) + 256
+
+
+def _run_until_done(srv, rid, limit=10000):
+    for _ in range(limit):
+        srv.tick()
+        if rid in srv.finished():
+            return
+    raise AssertionError(f"request {rid} did not finish")
+
+
+def test_served_own_idle_scene_survives_eviction(ctx):
+    """Admission evicts idle scenes to fit a new frame.  When the only idle scene is the request's own
+    (its earlier requests taken) and the new frame is larger than every pooled buffer, the scene must
+    not be freed under the request: the server either keeps it or brings it in anew."""
+    say("case serve own idle scene under eviction")
+    key = ("soup", 20000)
+    shapes, lc, _ = _scene(key)
+    # the scene's built footprint, measured on a server with room to spare
+    srv = pt.Server(ctx, 4 << 30, 1)
+    rid = srv.submit(1, shapes, lc, _camera(key, 8, 8), 8, 8, 1)
+    _run_until_done(srv, rid)
+    real = srv.stats()["hbm_used"] - _frame_bytes(8, 8)
+    srv.result(rid)
+    srv.close()
+    estimate = len(shapes) * 600
+    assert real > estimate, "the case needs a built scene larger than the admission estimate"
+    # room for the scene at its estimate and the large frame, not for the built scene and the large frame
+    budget = estimate + _frame_bytes(64, 64)
+    srv = pt.Server(ctx, budget, 1)
+    a = srv.submit(1, shapes, lc, _camera(key, 8, 8), 8, 8, 1)
+    _run_until_done(srv, a)
+    srv.result(a)                              # the 8x8 frame buffer goes to the pool; the scene is idle
+    b = srv.submit(1, shapes, lc, _camera(key, 64, 64), 64, 64, 2)
+    _run_until_done(srv, b)
+    col, pix, _ = srv.result(b, colors=True, pixels=True)
+    st = srv.stats()
+    srv.close()
+    assert st["builds"] == 2                   # the scene was dropped and built anew, not used after free
+    a_col, a_pix, _ = _alone(ctx, shapes, lc, _camera(key, 64, 64), 64, 64, 2, default_seeds(64, 64))
+    assert same(col, a_col) and same(pix, a_pix)

@@ -496,3 +496,192 @@ def test_update_camera_library_matches_oracle():
         pt.update_camera(a)
         O.update_camera(b)
         assert a.tobytes() == b.tobytes()
+
+
+# ---------------------------------------------------------------------------------------------
+# The per-pixel wrapper: UpdateCamera, GenerateCameraRay, running mean, row flip, RGBA8 pack
+# ---------------------------------------------------------------------------------------------
+FOV = F32(F32(3.14159265358979323846) / F32(180.0)) * F32(45.0)   # displayfunc.cpp:742, camera.h:40
+
+
+def _both_update_camera(yaw, pitch, w, h):
+    import ptgpu_pc_amd as pt
+    a = make_camera((1, 2, 3), yaw=yaw, pitch=pitch, width=w, height=h)
+    b = a.copy()
+    pt.update_camera(a)
+    O.update_camera(b)
+    return a, b
+
+
+@pytest.mark.parametrize("yaw,pitch,w,h", [(0.0, 0.0, 256, 256), (math.pi / 2, 0.0, 1024, 768),
+                                           (0.0, 0.3, 640, 480)])
+def test_update_camera_closed_form(yaw, pitch, w, h):
+    """displayfunc.cpp:732-756: dir = -(sin yaw cos pitch, sin pitch, cos yaw cos pitch);
+    x = norm(dir x up) * W fov / H; y = norm(x x dir) * fov.  Closed forms per case:
+      yaw 0, pitch 0:  dir (0, 0, -1), x (W fov/H, 0, 0), y (0, fov, 0)
+      yaw 90 deg:      dir (-1, 0, 0), x (0, 0, -W fov/H), y (0, fov, 0)
+      pitch p:         dir (0, -sin p, -cos p), x (W fov/H, 0, 0), y (0, cos p, -sin p) * fov"""
+    k = float(FOV) * w / h
+    if pitch == 0.0 and yaw == 0.0:
+        want = [(0, 0, -1), (k, 0, 0), (0, float(FOV), 0)]
+    elif pitch == 0.0:
+        want = [(-1, 0, 0), (0, 0, -k), (0, float(FOV), 0)]
+    else:
+        sp, cp = math.sin(pitch), math.cos(pitch)
+        want = [(0, -sp, -cp), (k, 0, 0), (0, cp * float(FOV), -sp * float(FOV))]
+    for cam in _both_update_camera(yaw, pitch, w, h):
+        for name, v in zip(("dir", "x", "y"), want):
+            np.testing.assert_allclose(cam[name][0].astype(np.float64), v, rtol=0, atol=2e-6 * max(1.0, k), err_msg=name)
+
+
+def test_generate_camera_ray_hand_computed():
+    """rendering_kernel.cl:1042-1070 by hand in float32: r1 then r2 drawn from the pixel's seeds,
+    kcx = (x + r1)/W - .5, kcy = (y + r2)/H - .5, d = X kcx + Y kcy + dir, o = orig + 0.1 d, then d
+    normalised; the pixel is x = gid % W, y = gid / W."""
+    w, h = 40, 30
+    cam = make_camera((0.5, 0.25, 3.0), yaw=0.4, pitch=-0.15, width=w, height=h)
+    O.update_camera(cam)
+    rng = np.random.default_rng(11)
+    seeds = rng.integers(2, 2 ** 32 - 1, size=2 * w * h, dtype=np.uint64).astype(np.uint32)
+    pix = np.array([0, w - 1, w * (h - 1), w * h - 1, 17 * w + 3, 5 * w + 31], dtype=np.int32)
+    got_seeds = seeds.copy()
+    rays = O.camera_rays(cam, got_seeds, w, h, pix)
+    X, Y, D, P = (cam[n][0].astype(F32) for n in ("x", "y", "dir", "orig"))
+    for q, gid in enumerate(pix):
+        s0, s1 = int(seeds[2 * gid]), int(seeds[2 * gid + 1])
+        u1, s0, s1 = py_get_random(s0, s1)
+        u2, s0, s1 = py_get_random(s0, s1)
+        r1, r2 = F32(u1 - F32(0.5)), F32(u2 - F32(0.5))
+        x, y = F32(gid % w), F32(gid // w)
+        kcx = F32(F32(F32(x + r1) * F32(F32(1) / F32(w))) - F32(0.5))
+        kcy = F32(F32(F32(y + r2) * F32(F32(1) / F32(h))) - F32(0.5))
+        d = np.array([F32(F32(X[i] * kcx) + F32(Y[i] * kcy)) + D[i] for i in range(3)], dtype=F32)
+        o = np.array([F32(F32(0.1) * d[i]) + P[i] for i in range(3)], dtype=F32)
+        dd = F32(F32(d[0] * d[0]) + F32(d[1] * d[1])) + F32(d[2] * d[2])
+        inv = F32(F32(1) / F32(np.sqrt(dd)))
+        dn = np.array([inv * d[i] for i in range(3)], dtype=F32)
+        assert rays["o"][q].tobytes() == o.tobytes(), q
+        assert rays["d"][q].tobytes() == dn.tobytes(), q
+        assert (int(got_seeds[2 * gid]), int(got_seeds[2 * gid + 1])) == (s0, s1)
+
+
+def _emitter_scene(center, rad, e, floor=False):
+    shapes = spheres_at([center], r=rad)
+    shapes["e"] = e
+    shapes["area"] = scenes.sphere_area(shapes["rad"])
+    if floor:   # a large diffuse sphere below: radiance then varies from sample to sample
+        f = spheres_at([(0.0, -1e4 - 2.0, -10.0)], r=1e4)
+        f["c"] = (0.75, 0.75, 0.75)
+        f["refl"] = DIFF
+        f["area"] = scenes.sphere_area(f["rad"])
+        shapes = _concat(shapes, f)
+    return shapes
+
+
+def _concat(a, b):
+    from ptgpu_pc_amd.layout import concat_shapes
+    return concat_shapes([a, b])
+
+
+def test_row_flip_and_pixel_orientation():
+    """RadianceGPU writes colors[(H-1-y) W + x] and pixels[y W + x] (rendering_kernel.cl:1127,1141):
+    an emitter up and to the right of the view axis (+x, +y world; camera looking down -z) is seen at
+    pixel ids with x > W/2 and y > H/2, so it lands in the upper-right block of `pixels` (rows by y)
+    and in the first rows of `colors` (rows by H-1-y)."""
+    w = h = 32
+    shapes = _emitter_scene((2.0, 2.0, -10.0), 1.5, (4.0, 4.0, 4.0))
+    cam = make_camera((0, 0, 0), yaw=0.0, width=w, height=h)
+    O.update_camera(cam)
+    c, _, p = oracle_render(O.accel_none(), 5, shapes, 1, cam, w, h, 2, sem=0)
+    lit_c = (c.reshape(h, w, 3) != 0).any(axis=2)
+    lit_p = p.reshape(h, w) != 0xFF000000
+    assert lit_c.sum() > 20 and lit_c.sum() == lit_p.sum()
+    assert not lit_c[h // 2:, :].any() and not lit_c[:, : w // 2].any()     # colors: rows H-1-y < H/2
+    assert not lit_p[: h // 2, :].any() and not lit_p[:, : w // 2].any()     # pixels: rows y >= H/2
+    np.testing.assert_array_equal(lit_c[::-1], lit_p)
+
+
+def test_pack_known_values_and_channel_order():
+    """toInt(v) = (int)(pow(clamp(v, 0, 1), 1/2.2) * 255 + .5) (vec.h:60) and the pack
+    r | g << 8 | b << 16 | 0xff000000 (rendering_kernel.cl:1141-1145).  From inside an emissive sphere
+    every camera ray meets the surface head-on (|dp| = 1 to within an ulp), so the colour is e:
+    toInt(0.5) = 186, toInt(1) = 255, toInt(2) = 255 (clamped); a frame that sees nothing is 0 -> 0."""
+    w = h = 8
+    shapes = _emitter_scene((0.0, 0.0, 0.0), 10.0, (0.5, 1.0, 2.0))
+    cam = make_camera((0, 0, 0), yaw=0.3, width=w, height=h)
+    O.update_camera(cam)
+    c, _, p = oracle_render(O.accel_none(), 5, shapes, 1, cam, w, h, 1, sem=0)
+    np.testing.assert_allclose(c.reshape(-1, 3), np.tile([0.5, 1.0, 2.0], (w * h, 1)), rtol=1e-6)
+    assert (p == np.uint32(186 | (255 << 8) | (255 << 16) | 0xFF000000)).all()
+    cam["yaw"] = F32(math.pi)
+    away = _emitter_scene((0.0, 0.0, -50.0), 1.0, (0.5, 1.0, 2.0))
+    O.update_camera(cam)
+    c, _, p = oracle_render(O.accel_none(), 5, away, 1, cam, w, h, 1, sem=0)
+    assert (c == 0).all() and (p == np.uint32(0xFF000000)).all()
+
+
+def test_running_mean_over_samples():
+    """Sample k > 0 folds in as colors = (colors k + r) / (k + 1) (rendering_kernel.cl:1127-1135): after
+    K launches a pixel holds the arithmetic mean of its K sample radiances.  Each r_k is rendered on
+    its own (currentSample 0 assigns) from the seeds sample k starts from."""
+    w, h = 24, 16
+    shapes = _emitter_scene((2.0, 2.0, -10.0), 1.5, (4.0, 4.0, 4.0), floor=True)
+    cam = make_camera((0, 0, 0), yaw=0.0, pitch=-0.2, width=w, height=h)
+    O.update_camera(cam)
+    seeds = default_seeds(w, h)
+    samples = []
+    for _ in range(4):
+        col = np.zeros(w * h * 3, F32)
+        O.radiance_frame(col, seeds, cam, shapes, 1, O.accel_none(), w, h, 0, 1, None, 5, sem=0)
+        samples.append(col.astype(np.float64))
+    col = np.zeros(w * h * 3, F32)
+    s2 = default_seeds(w, h)
+    for k in range(4):   # four launches, currentSample 0..3
+        O.radiance_frame(col, s2, cam, shapes, 1, O.accel_none(), w, h, k, 1, None, 5, sem=0)
+        mean = np.mean(samples[:k + 1], axis=0)
+        np.testing.assert_allclose(col, mean, rtol=2e-6, atol=1e-7)
+    np.testing.assert_array_equal(s2, seeds)
+    assert np.std(np.array(samples), axis=0).max() > 0.01   # the samples differ
+
+
+def test_shadow_rays_through_bvh_equal_bruteforce():
+    """The oracle's BVH-walked IntersectP (shadow_accel) answers the reference's brute-force scan on
+    every ray, and a frame rendered with it has the same bits."""
+    shapes, lc, cam = scenes.soup(20000, seed=21, width=20, height=20)
+    s, srt, btn, btl = O.build_lbvh(shapes)
+    rays = random_rays(4000, 9, lo=0.2, hi=0.8)
+    t, ids = O.trace_closest(s, O.accel_none(), rays)
+    rng = np.random.default_rng(2)
+    maxt = np.where(ids >= 0, t * F32(rng.choice([0.5, 1.0, 1.5], size=len(t))), F32(2.0)).astype(F32)
+    h_b = O.trace_any(s, rays, maxt)
+    h_v = O.trace_any_bvh(s, O.accel_bvh(btn, btl, shadow_accel=True), rays, maxt)
+    np.testing.assert_array_equal(h_b, h_v)
+    assert 100 < h_b.sum() < len(h_b) - 100
+    O.update_camera(cam)
+    a = oracle_render(O.accel_bvh(btn, btl), 5, s, lc, cam, 20, 20, 2, sem=0)
+    b = oracle_render(O.accel_bvh(btn, btl, shadow_accel=True), 5, s, lc, cam, 20, 20, 2, sem=0)
+    for x, y in zip(a, b):
+        assert x.tobytes() == y.tobytes()
+
+
+def test_sincos_kernel_semantics_against_the_mathematics():
+    """The kernel semantics' sin/cos (Taylor series in binary64, one rounding to binary32) against
+    float64 sin/cos rounded to float32: within one binary32 ulp everywhere on the sampling angles
+    2 pi u (u in [0, 1)) and beyond, and equal to the correctly rounded value almost always."""
+    rng = np.random.default_rng(4)
+    u = rng.random(6000).astype(F32)
+    xs = np.concatenate([F32(2.0) * F32(math.pi) * u,                       # the angles the kernel draws
+                         rng.uniform(-20.0, 20.0, 3000).astype(F32),
+                         np.array([0.0, -0.0, math.pi / 4, math.pi / 2, math.pi, 1.5 * math.pi, 2 * math.pi,
+                                   1e-7, 3.0, 6.2831], dtype=F32)])
+    exact_s, exact_c = 0, 0
+    for x in xs:
+        s, c = O.sincos(x, sem=0)
+        rs, rc = np.float32(math.sin(float(x))), np.float32(math.cos(float(x)))
+        for got, want in ((s, rs), (c, rc)):
+            ulp = np.spacing(np.abs(want)) if want != 0 else np.float32(1e-45)
+            assert abs(float(got) - float(want)) <= float(ulp), (x, got, want)
+        exact_s += s == rs
+        exact_c += c == rc
+    assert exact_s >= 0.999 * len(xs) and exact_c >= 0.999 * len(xs)
+    assert O.sincos(0.0) == (0.0, 1.0)
