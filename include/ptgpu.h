@@ -261,6 +261,17 @@ typedef struct {
  * into pixels, seeds written back.  Trees are read in the reference layout. */
 int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *args);
 
+/* No reference counterpart: ptgpu_radiance_gpu for callers that cannot allocate device memory (code
+ * set up for HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * arguments, work and bits as ptgpu_radiance_gpu; the call itself allocates nothing.  The traversal
+ * layout converted from the caller's tree, its parent map, the light list and the pixel counter live
+ * in one static device array of 128 MiB per GPU (held until the process ends), enough for about
+ * 400,000 shapes in BVH mode; a frame whose tree needs more is refused with PTGPU_ERR_UNSUPPORTED
+ * before any device work.  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other contexts or other threads) race.  The wavefront engine (engine 2) is refused:
+ * it allocates its queues. */
+int ptgpu_radiance_gpu_noalloc(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *args);
+
 /* Fast path: render `spp` consecutive samples (currentSample .. currentSample+spp-1) in one
  * launch, traversing the handle's prepared device layout.  Results are identical to `spp`
  * successive ptgpu_radiance_gpu calls.  Exactly one of bvh/kd may be non-NULL, or both NULL for

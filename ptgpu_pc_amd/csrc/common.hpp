@@ -198,6 +198,18 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
                  unsigned long long *d_ray_counter);
 // light list cached in a KD handle (as the LBVH handle caches its own)
 int kdtree_lights(ptgpu_kdtree *kd, const Shape *d_shapes, int n, int light_cnt, const int **lights, int *nlights);
+// the same with caller-placed buffers (no allocation)
+int build_index_prims_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, Prim *prims);
+size_t wide_bytes(const DevAccel &acc);
+int build_wide_into(ptgpu_ctx *ctx, DevAccel *acc, TNode4 *w4);   // room for wide_bytes(*acc)
+size_t light_scan_bytes(int n);
+int build_light_list_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt, int *out, unsigned *scan,
+                          int *found);
+size_t ref_parents_bytes(int n);
+int convert_bvh_ref_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn,
+                         const BVHNodeGPU *d_btl, TNode *nodes, Prim *prims, int *parents, DevAccel *acc);
+int convert_kd_ref_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU *d_kng, int kngCnt,
+                        const int *d_kn, int knCnt, TNode *nodes, Prim *prims, DevAccel *acc);
 // reference-layout trees -> traversal layout
 int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn,
                     const BVHNodeGPU *d_btl, DevBuf &nodes, DevBuf &prims, DevAccel *acc);
@@ -216,8 +228,11 @@ struct RenderLaunch {
     unsigned int *pixels;
     unsigned long long *ray_counter;
     unsigned long long *trav_stats;
+    unsigned *work_counter = nullptr;   // device word for the persistent waves' pixel counter; NULL: the context's
 };
 int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
+// device address of this GPU's allocation-free scratch, looked up once (device current)
+int noalloc_scratch_init(int device);
 int launch_render_wavefront(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
 int wf_supported(int nlights);
 void wf_release(ptgpu_ctx *ctx);

@@ -112,6 +112,9 @@ int ptgpu_ctx_create(int device, void *stream, ptgpu_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
     if (c->cu_count <= 0) c->cu_count = 256;
+    // the allocation-free entry's scratch address on this device (checked again by that entry,
+    // which reports a failed look-up; a context that never uses it works without it)
+    if (ptgpu::noalloc_scratch_init(device) != PTGPU_OK) (void)hipGetLastError();   // keep no stale HIP error
     *out = c;
     return PTGPU_OK;
 }

@@ -17,6 +17,7 @@
 //     the next pixel from a global counter, wave-wide through one atomic (ballot + popcount).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lbvh.hpp"
@@ -1206,22 +1207,34 @@ __global__ void k_convert_kd(const Shape *shapes, const KDNodeGPU *kng, int kngC
 // ---------------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------------
-int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_prims) {
-    PT_TRY(out_prims.ensure((size_t)n * sizeof(Prim)));
-    hipLaunchKernelGGL(k_index_prims, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_shapes, n,
-                       out_prims.as<Prim>());
+int build_index_prims_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, Prim *prims) {
+    hipLaunchKernelGGL(k_index_prims, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_shapes, n, prims);
     PT_HIP(hipGetLastError());
     return PTGPU_OK;
 }
+
+int build_index_prims(ptgpu_ctx *ctx, const Shape *d_shapes, int n, DevBuf &out_prims) {
+    PT_TRY(out_prims.ensure((size_t)n * sizeof(Prim)));
+    PT_TRY(build_index_prims_into(ctx, d_shapes, n, out_prims.as<Prim>()));
+    return PTGPU_OK;
+}
+
+// one buffer: the fp32 four-wide nodes, then their binary16 copy
+size_t wide_bytes(const DevAccel &acc) { return (size_t)(acc.nnodes > 0 ? acc.nnodes : 0) * (sizeof(TNode4) + sizeof(TNode4H)); }
 
 int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4) {
     acc->nodes4 = nullptr;
     acc->nodes4h = nullptr;
     if (acc->mode == PTGPU_ACCEL_NONE || acc->root < 0 || acc->nnodes <= 0 || !acc->nodes) return PTGPU_OK;
-    // one buffer: the fp32 four-wide nodes, then their binary16 copy
+    PT_TRY(out_nodes4.ensure(wide_bytes(*acc)));
+    return build_wide_into(ctx, acc, out_nodes4.as<TNode4>());
+}
+
+int build_wide_into(ptgpu_ctx *ctx, DevAccel *acc, TNode4 *w4) {
+    acc->nodes4 = nullptr;
+    acc->nodes4h = nullptr;
+    if (acc->mode == PTGPU_ACCEL_NONE || acc->root < 0 || acc->nnodes <= 0 || !acc->nodes) return PTGPU_OK;
     const size_t n4 = (size_t)acc->nnodes;
-    PT_TRY(out_nodes4.ensure(n4 * (sizeof(TNode4) + sizeof(TNode4H))));
-    TNode4 *w4 = out_nodes4.as<TNode4>();
     const dim3 grid((unsigned)((n4 + 255) / 256));
     hipLaunchKernelGGL(k_collapse4, grid, dim3(256), 0, ctx->stream, acc->nodes, acc->nnodes, w4);
     PT_HIP(hipGetLastError());
@@ -1235,6 +1248,8 @@ int build_wide(ptgpu_ctx *ctx, DevAccel *acc, DevBuf &out_nodes4) {
     return PTGPU_OK;
 }
 
+size_t light_scan_bytes(int n) { return ((size_t)((n + kLightChunk - 1) / kLightChunk) + 4) * 4; }
+
 int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt, DevBuf &out, DevBuf &scan,
                      int *found) {
     *found = 0;
@@ -1243,17 +1258,24 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
         return PTGPU_OK;
     }
     PT_NO_CAPTURE(ctx, "building a light list");
-    const int nb = (n + kLightChunk - 1) / kLightChunk;
-    PT_TRY(scan.ensure(((size_t)nb + 4) * 4));
+    PT_TRY(scan.ensure(light_scan_bytes(n)));
     PT_TRY(out.ensure((size_t)light_cnt * 4));
-    unsigned *counts = scan.as<unsigned>();
+    return build_light_list_into(ctx, d_shapes, n, light_cnt, out.as<int>(), scan.as<unsigned>(), found);
+}
+
+int build_light_list_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt, int *out, unsigned *scan,
+                          int *found) {
+    *found = 0;
+    if (light_cnt <= 0 || n <= 0) return PTGPU_OK;
+    PT_NO_CAPTURE(ctx, "building a light list");
+    const int nb = (n + kLightChunk - 1) / kLightChunk;
+    unsigned *counts = scan;
     unsigned *total = counts + nb;
     hipLaunchKernelGGL(k_light_count, dim3(nb), dim3(256), 0, ctx->stream, d_shapes, n, counts);
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_light_scan, dim3(1), dim3(64), 0, ctx->stream, counts, nb, total);
     PT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_light_write, dim3(nb), dim3(256), 0, ctx->stream, d_shapes, n, counts, light_cnt,
-                       out.as<int>());
+    hipLaunchKernelGGL(k_light_write, dim3(nb), dim3(256), 0, ctx->stream, d_shapes, n, counts, light_cnt, out);
     PT_HIP(hipGetLastError());
     unsigned h_total = 0;
     PT_HIP(hipMemcpyAsync(&h_total, total, 4, hipMemcpyDeviceToHost, ctx->stream));
@@ -1262,17 +1284,27 @@ int build_light_list(ptgpu_ctx *ctx, const Shape *d_shapes, int n, int light_cnt
     return PTGPU_OK;
 }
 
+size_t ref_parents_bytes(int n) { return (size_t)(n > 1 ? 2 * n - 1 : 1) * 4 + 64; }
+
 int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn, const BVHNodeGPU *d_btl,
                     DevBuf &nodes, DevBuf &prims, DevAccel *acc) {
+    PT_TRY(ctx->tmp_parents.ensure(ref_parents_bytes(n)));
+    PT_TRY(nodes.ensure((size_t)(n > 1 ? n - 1 : 1) * sizeof(TNode)));
+    PT_TRY(prims.ensure((size_t)n * sizeof(Prim)));
+    return convert_bvh_ref_into(ctx, d_shapes, n, d_btn, d_btl, nodes.as<TNode>(), prims.as<Prim>(),
+                                ctx->tmp_parents.as<int>(), acc);
+}
+
+int convert_bvh_ref_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeGPU *d_btn,
+                         const BVHNodeGPU *d_btl, TNode *nodes, Prim *prims, int *parents, DevAccel *acc) {
     acc->depth = 0;
     if (n >= 2) {
         // The caller's links are checked before any kernel follows them, and the stack is sized from
         // the real height: a caller's tree (getTrees after optimize, or any other builder's) can be
         // much deeper than ceil(log2 n).
         PT_NO_CAPTURE(ctx, "measuring a caller's BVH");
-        PT_TRY(ctx->tmp_parents.ensure((size_t)(2 * n - 1) * 4 + 64));
-        int *par = ctx->tmp_parents.as<int>();
-        unsigned *err = reinterpret_cast<unsigned *>(ctx->tmp_parents.as<char>() + (size_t)(2 * n - 1) * 4);
+        int *par = parents;
+        unsigned *err = reinterpret_cast<unsigned *>(par + (2 * n - 1));
         PT_TRY(fill_u32(ctx->stream, par, 0xFFFFFFFFu, (size_t)(2 * n - 1)));
         PT_TRY(fill_u32(ctx->stream, err, 0u, 2));
         hipLaunchKernelGGL(k_ref_parents, dim3((n + 254) / 256), dim3(256), 0, ctx->stream, d_btn, n, par, err);
@@ -1296,13 +1328,11 @@ int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeG
         PT_HIP(hipStreamSynchronize(ctx->stream));
         PT_CHECK(ns == 0, PTGPU_ERR_INVALID, "btl: a leaf's nShape is out of range");
     }
-    PT_TRY(nodes.ensure((size_t)(n > 1 ? n - 1 : 1) * sizeof(TNode)));
-    PT_TRY(prims.ensure((size_t)n * sizeof(Prim)));
     hipLaunchKernelGGL(k_convert_bvh, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, d_shapes, n, d_btn, d_btl,
-                       nodes.as<TNode>(), prims.as<Prim>());
+                       nodes, prims);
     PT_HIP(hipGetLastError());
-    acc->nodes = nodes.as<TNode>();
-    acc->prims = prims.as<Prim>();
+    acc->nodes = nodes;
+    acc->prims = prims;
     acc->nprims = n;
     acc->nnodes = n - 1;
     acc->nshapes = n;
@@ -1316,18 +1346,24 @@ int convert_bvh_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const BVHNodeG
 int convert_kd_ref(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU *d_kng, int kngCnt, const int *d_kn,
                    int knCnt, DevBuf &nodes, DevBuf &prims, DevAccel *acc) {
     PT_CHECK(kngCnt >= 2, PTGPU_ERR_INVALID, "kngCnt %d < 2", kngCnt);
-    PT_NO_CAPTURE(ctx, "converting a KD heap");
     PT_TRY(nodes.ensure((size_t)kngCnt * sizeof(TNode)));
     PT_TRY(prims.ensure((size_t)(knCnt > 0 ? knCnt : 1) * sizeof(Prim)));
+    return convert_kd_ref_into(ctx, d_shapes, n, d_kng, kngCnt, d_kn, knCnt, nodes.as<TNode>(), prims.as<Prim>(), acc);
+}
+
+int convert_kd_ref_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNodeGPU *d_kng, int kngCnt,
+                        const int *d_kn, int knCnt, TNode *nodes, Prim *prims, DevAccel *acc) {
+    PT_CHECK(kngCnt >= 2, PTGPU_ERR_INVALID, "kngCnt %d < 2", kngCnt);
+    PT_NO_CAPTURE(ctx, "converting a KD heap");
     const int m = kngCnt > knCnt ? kngCnt : knCnt;
     hipLaunchKernelGGL(k_convert_kd, dim3((m + 255) / 256), dim3(256), 0, ctx->stream, d_shapes, d_kng, kngCnt, d_kn,
-                       knCnt, nodes.as<TNode>(), prims.as<Prim>());
+                       knCnt, nodes, prims);
     PT_HIP(hipGetLastError());
     KDNodeGPU root;
     PT_HIP(hipMemcpyAsync(&root, d_kng + 1, sizeof(KDNodeGPU), hipMemcpyDeviceToHost, ctx->stream));
     PT_HIP(hipStreamSynchronize(ctx->stream));
-    acc->nodes = nodes.as<TNode>();
-    acc->prims = prims.as<Prim>();
+    acc->nodes = nodes;
+    acc->prims = prims;
     acc->nprims = knCnt;
     acc->nnodes = kngCnt;
     acc->nshapes = n;
@@ -1371,8 +1407,12 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     if (wf_ok && ctx->engine == 2) return launch_render_wavefront(ctx, acc, L);
     const int st = stack_for(acc);
     PT_CHECK(st > 0, PTGPU_ERR_UNSUPPORTED, "tree depth %d exceeds the 62-level traversal stack", acc.depth);
-    PT_TRY(ctx->tmp_counter.ensure(64));
-    PT_TRY(fill_u32(ctx->stream, ctx->tmp_counter.p, 0u, 1));
+    unsigned *work_counter = L.work_counter;
+    if (!work_counter) {
+        PT_TRY(ctx->tmp_counter.ensure(64));
+        work_counter = ctx->tmp_counter.as<unsigned>();
+    }
+    PT_TRY(fill_u32(ctx->stream, work_counter, 0u, 1));
     RenderParams P;
     P.acc = acc;
     P.colors = L.colors;
@@ -1389,7 +1429,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.pixels = L.pixels;
     P.ray_counter = L.ray_counter;
     P.trav_stats = L.trav_stats;
-    P.work_counter = ctx->tmp_counter.as<unsigned>();
+    P.work_counter = work_counter;
     P.total = total;
     P.regroup = ctx->regroup;
     P.quad = ctx->quad;
@@ -1549,6 +1589,143 @@ int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) 
     L.pixels = a->pixels;
     L.ray_counter = nullptr;
     L.trav_stats = nullptr;
+    return launch_render(ctx, acc, L);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free RadianceGPU (ptgpu_radiance_gpu_noalloc)
+// ---------------------------------------------------------------------------------------------
+namespace ptgpu {
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+constexpr int kScratchDevices = 64;
+}  // namespace ptgpu
+
+// The work space of ptgpu_radiance_gpu_noalloc: the traversal layout converted from the caller's tree
+// (binary nodes, primitives, four-wide nodes), the parent map of a caller's BVH, the light list and
+// the persistent waves' pixel counter.  Each GPU holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns for that GPU.
+static __device__ __align__(256) unsigned char scratch[ptgpu::kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[ptgpu::kScratchDevices] = {};
+
+// carves 256-byte-aligned pieces out of one device range
+struct Carve {
+    char *base;
+    size_t used;
+    template <class T> T *take(size_t bytes) {
+        T *p = reinterpret_cast<T *>(base + used);
+        used += (bytes + 255) & ~size_t(255);
+        return p;
+    }
+};
+size_t up256(size_t b) { return (b + 255) & ~size_t(255); }
+}  // namespace
+
+// Device address of this GPU's scratch array, looked up once per device with that device current
+// (the context's set-up calls this; the entry point calls it again if the look-up failed then).
+int ptgpu::noalloc_scratch_init(int device) {
+    PT_CHECK(device >= 0 && device < kScratchDevices, PTGPU_ERR_UNSUPPORTED, "device %d: scratch table holds %d",
+             device, kScratchDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return PTGPU_OK;
+    void *ptr = nullptr;
+    PT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    PT_CHECK(ptr != nullptr, PTGPU_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[device] = ptr;
+    return PTGPU_OK;
+}
+
+extern "C" {
+
+int ptgpu_radiance_gpu_noalloc(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) {
+    PT_CHECK(ctx && a, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(a->colors && a->seeds && a->camera && a->shapes && a->pixels, PTGPU_ERR_INVALID, "null buffer");
+    PT_CHECK(a->shapeCnt >= 1 && a->shapeCnt < (1u << 30), PTGPU_ERR_INVALID, "shapeCnt %u out of range", a->shapeCnt);
+    PT_CHECK(a->width > 0 && a->height > 0, PTGPU_ERR_INVALID, "bad frame size %dx%d", a->width, a->height);
+    PT_CHECK((unsigned long long)a->width * (unsigned long long)a->height < (1ull << 30), PTGPU_ERR_UNSUPPORTED,
+             "frame %dx%d: 2^30 pixels or more (seed indices 2*pixel must stay within int)", a->width, a->height);
+    PT_CHECK(a->currentSample >= 0, PTGPU_ERR_INVALID, "currentSample < 0");
+    PT_CHECK(accel == PTGPU_ACCEL_NONE || accel == PTGPU_ACCEL_BVH || accel == PTGPU_ACCEL_KD, PTGPU_ERR_INVALID,
+             "unknown accel mode %d", accel);
+    PT_CHECK(ctx->engine != 2, PTGPU_ERR_UNSUPPORTED,
+             "the allocation-free entry runs the megakernel engines (the wavefront engine allocates its queues)");
+    const int n = (int)a->shapeCnt;
+    // the scratch this call needs, checked before any device work
+    size_t nodes_b = 0, prims_b = 0, wide_b = 0, par_b = 0;
+    if (accel == PTGPU_ACCEL_BVH) {
+        PT_CHECK(a->btn && a->btl, PTGPU_ERR_INVALID, "BVH mode needs btn and btl");
+        nodes_b = (size_t)(n > 1 ? n - 1 : 1) * sizeof(TNode);
+        prims_b = (size_t)n * sizeof(Prim);
+        wide_b = (size_t)(n > 1 ? n - 1 : 0) * (sizeof(TNode4) + sizeof(TNode4H));
+        par_b = ref_parents_bytes(n);
+    } else if (accel == PTGPU_ACCEL_KD) {
+        PT_CHECK(a->kng && a->kn, PTGPU_ERR_INVALID, "KD mode needs kng and kn");
+        PT_CHECK(a->kngCnt >= 2 && a->knCnt >= 0, PTGPU_ERR_INVALID, "kngCnt %d / knCnt %d", a->kngCnt, a->knCnt);
+        nodes_b = (size_t)a->kngCnt * sizeof(TNode);
+        prims_b = (size_t)(a->knCnt > 0 ? a->knCnt : 1) * sizeof(Prim);
+        wide_b = (size_t)a->kngCnt * (sizeof(TNode4) + sizeof(TNode4H));
+    } else {
+        prims_b = (size_t)n * sizeof(Prim);
+    }
+    const size_t lights_b = (size_t)(a->lightCnt > 0 ? a->lightCnt : 1) * 4;
+    const size_t need = up256(64) + up256(nodes_b) + up256(prims_b) + up256(wide_b) + up256(par_b) + up256(lights_b) +
+                        up256(light_scan_bytes(n));
+    PT_CHECK(need <= kNoAllocScratchBytes, PTGPU_ERR_UNSUPPORTED,
+             "this frame's tree needs %zu bytes of scratch; the allocation-free entry holds %zu "
+             "(ptgpu_radiance_gpu has no such limit)",
+             need, kNoAllocScratchBytes);
+    PT_HIP(hipSetDevice(ctx->device));
+    PT_TRY(noalloc_scratch_init(ctx->device));
+    Carve cv{static_cast<char *>(g_scratch_ptr[ctx->device]), 0};
+    unsigned *counter = cv.take<unsigned>(64);
+    TNode *nodes = cv.take<TNode>(nodes_b);
+    Prim *prims = cv.take<Prim>(prims_b);
+    TNode4 *wide = cv.take<TNode4>(wide_b);
+    int *parents = cv.take<int>(par_b);
+    int *lights = cv.take<int>(lights_b);
+    unsigned *scan = cv.take<unsigned>(light_scan_bytes(n));
+    DevAccel acc;
+    if (accel == PTGPU_ACCEL_BVH) {
+        PT_TRY(convert_bvh_ref_into(ctx, a->shapes, n, a->btn, a->btl, nodes, prims, parents, &acc));
+        PT_TRY(build_wide_into(ctx, &acc, wide));
+    } else if (accel == PTGPU_ACCEL_KD) {
+        PT_TRY(convert_kd_ref_into(ctx, a->shapes, n, a->kng, a->kngCnt, a->kn, a->knCnt, nodes, prims, &acc));
+        PT_TRY(build_wide_into(ctx, &acc, wide));
+    } else {
+        PT_TRY(build_index_prims_into(ctx, a->shapes, n, prims));
+        acc.nodes = nullptr;
+        acc.prims = prims;
+        acc.nprims = n;
+        acc.nnodes = 0;
+        acc.nshapes = n;
+        acc.root = ~0;
+        acc.root_cnt = n;
+        acc.depth = 0;
+        acc.mode = PTGPU_ACCEL_NONE;
+    }
+    int nl = 0;
+    PT_TRY(build_light_list_into(ctx, a->shapes, n, (int)a->lightCnt, lights, scan, &nl));
+    RenderLaunch L;
+    L.colors = a->colors;
+    L.seeds = a->seeds;
+    L.camera = a->camera;
+    L.shapes = a->shapes;
+    L.shapeCnt = a->shapeCnt;
+    L.lights = lights;
+    L.nlights = nl;
+    L.width = a->width;
+    L.height = a->height;
+    L.currentSample = a->currentSample;
+    L.spp = 1;
+    L.max_depth = ctx->max_depth;
+    L.pixels = a->pixels;
+    L.ray_counter = nullptr;
+    L.trav_stats = nullptr;
+    L.work_counter = counter;
     return launch_render(ctx, acc, L);
 }
 

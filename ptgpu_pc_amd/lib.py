@@ -117,6 +117,7 @@ _SIGS = {
     "ptgpu_kdtree_stats": [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)],
     "ptgpu_kdtree_destroy": [vp],
     "ptgpu_radiance_gpu": [vp, i32, C.POINTER(RadianceArgs)],
+    "ptgpu_radiance_gpu_noalloc": [vp, i32, C.POINTER(RadianceArgs)],
     "ptgpu_render": [vp, vp, vp, C.POINTER(RenderArgs)],
     "ptgpu_render_batch": [vp, vp, i32, vp],
     "ptgpu_server_create": [vp, C.c_size_t, i32, vp],
@@ -722,15 +723,18 @@ def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = Non
 
 
 def radiance_gpu(ctx: Context, accel: int, colors, seeds, camera, shapes, shapeCnt, lightCnt, btn=None, btl=None,
-                 kng=None, kngCnt=0, kn=None, knCnt=0, width=0, height=0, currentSample=0, pixels=None) -> None:
-    """One RadianceGPU launch with the reference argument list (device pointers or DeviceArrays)."""
+                 kng=None, kngCnt=0, kn=None, knCnt=0, width=0, height=0, currentSample=0, pixels=None,
+                 noalloc: bool = False) -> None:
+    """One RadianceGPU launch with the reference argument list (device pointers or DeviceArrays).
+    noalloc: through ptgpu_radiance_gpu_noalloc (the library's static scratch; allocates nothing)."""
     def p(x):
         if x is None:
             return None
         return x.ptr if isinstance(x, DeviceArray) else int(x)
     a = RadianceArgs(p(colors), p(seeds), p(camera), p(shapes), shapeCnt, lightCnt, p(btn), p(btl), p(kng), kngCnt,
                      p(kn), knCnt, width, height, currentSample, p(pixels))
-    _check(ctx.lib.ptgpu_radiance_gpu(ctx.handle, accel, C.byref(a)))
+    fn = ctx.lib.ptgpu_radiance_gpu_noalloc if noalloc else ctx.lib.ptgpu_radiance_gpu
+    _check(fn(ctx.handle, accel, C.byref(a)))
 
 
 def trace_closest(ctx: Context, bvh, kd, shapes: DeviceArray, shapeCnt: int, rays: DeviceArray, n: int,
