@@ -269,6 +269,12 @@ def build_sweep(sizes, reps=5):
     import ptgpu_pc_amd as pt
     rng = np.random.default_rng(11)
     ctx = pt.Context(0)
+    # one copy of known size, so that counter runs of this sweep can be calibrated
+    c_src, c_dst = ctx.alloc(CALIB_BYTES, np.uint8), ctx.alloc(CALIB_BYTES, np.uint8)
+    ctx.copy_calibration(c_dst, c_src, CALIB_BYTES)
+    ctx.synchronize()
+    c_src.free()
+    c_dst.free()
     for n in sizes:
         cen = rng.random((n, 3), dtype=np.float32)
         from ptgpu_pc_amd.layout import SHAPE_DTYPE, TRIANGLE

@@ -130,8 +130,8 @@ __global__ __launch_bounds__(1024) void k_mrefit_mid(const int4 *mnode, const in
 __global__ __launch_bounds__(1024) void k_mrefit_top(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
                                                      TNode *tnodes, float *gbi, int *gh, const int2 *xtop,
                                                      const unsigned *xtop_count, unsigned xtop_cap, unsigned *gcnt,
-                                                     unsigned *root_height) {
-    mrefit_top_body(mnode, mpar_int, btl, tnodes, gbi, gh, xtop, xtop_count, xtop_cap, gcnt, root_height);
+                                                     unsigned *root_height, unsigned *err) {
+    mrefit_top_body(mnode, mpar_int, btl, tnodes, gbi, gh, xtop, xtop_count, xtop_cap, gcnt, root_height, err);
 }
 
 }  // namespace ptgpu
@@ -292,7 +292,7 @@ static int clbvh_traversal_morton(ptgpu_clbvh *h) {
             PT_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_mrefit_top, dim3(1), dim3(1024), 0, st, h->mnode.as<int4>(), mpar_int,
                                h->btl.as<BVHNodeGPU>(), h->tnodes.as<TNode>(), gbi, gh, (const int2 *)ms.xtop,
-                               (const unsigned *)(ctr + 10), ms.xtop_cap, gcnt, ctr + 11);
+                               (const unsigned *)(ctr + 10), ms.xtop_cap, gcnt, ctr + 11, ctr + 9);
             PT_HIP(hipGetLastError());
         }
     }
@@ -343,7 +343,7 @@ int clbvh_prepare_traversal(ptgpu_clbvh *h) {
         PT_HIP(hipMemcpyAsync(w, misc_counters(h) + 9, 12, hipMemcpyDeviceToHost, h->ctx->stream));
         PT_HIP(hipStreamSynchronize(h->ctx->stream));
         PT_CHECK(w[0] == 0, PTGPU_ERR_HIP, "LBVH build failed (error word %u: 1 sort look-back spin limit, 2 "
-                 "traversal-tree list overflow)", w[0]);
+                 "traversal-tree list overflow, 4 climb beyond the height bound)", w[0]);
         h->acc.depth = (int)w[2];
         h->height_pending = false;
     }
@@ -595,7 +595,7 @@ int ptgpu_clbvh_get_trees(ptgpu_clbvh *h, BVHNodeGPU **h_btn, BVHNodeGPU **h_btl
     PT_HIP(hipMemcpyAsync(&err, misc_counters(h) + 9, 4, hipMemcpyDeviceToHost, h->ctx->stream));
     PT_HIP(hipStreamSynchronize(h->ctx->stream));
     PT_CHECK(err == 0, PTGPU_ERR_HIP, "LBVH build failed (error word %u: 1 sort look-back spin limit, 2 "
-             "traversal-tree list overflow)", err);
+             "traversal-tree list overflow, 4 climb beyond the height bound)", err);
     *h_btn = h->h_btn.data();
     *h_btl = h->h_btl.data();
     return PTGPU_OK;

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(1024) void kb_mrefit_top(const BuildSeg *g) {
     const BuildSeg &S = g[blockIdx.x];
     if (S.topo != 1 || S.n <= kMChunk) return;
     mrefit_top_body(S.mnode, S.mpar_int, S.btl, S.tnodes, S.gbi, S.gh, S.xtop, S.ctr + 10, S.xtop_cap, S.gcnt,
-                    S.ctr + 11);
+                    S.ctr + 11, S.ctr + 9);
 }
 
 // heights[s] = traversal-tree height, errs[s] = build error word

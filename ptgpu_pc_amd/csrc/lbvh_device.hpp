@@ -18,9 +18,8 @@ __device__ __forceinline__ int shape_i(const Shape *s, int word) {
 }
 
 // CLBVH::getBound CLBVH.cpp:49-75 from the Shape words (type @0, p1..p3 / rad,p @1..9).
-__device__ __forceinline__ Bound shape_bound(const Shape *s, int *type_out) {
-    const float *w = reinterpret_cast<const float *>(s);
-    int type = reinterpret_cast<const int *>(s)[0];
+__device__ __forceinline__ Bound bound_of_words(const float *w, int *type_out) {
+    int type = __float_as_int(w[0]);
     *type_out = type;
     Bound b;
     if (type == SPHERE) {
@@ -35,6 +34,9 @@ __device__ __forceinline__ Bound shape_bound(const Shape *s, int *type_out) {
         b.min_z = sel_min(sel_min(z1, z2), z3); b.max_z = sel_max(sel_max(z1, z2), z3);
     }
     return b;
+}
+__device__ __forceinline__ Bound shape_bound(const Shape *s, int *type_out) {
+    return bound_of_words(reinterpret_cast<const float *>(s), type_out);
 }
 
 // -------------------------------------------------------------------------------------------
@@ -70,6 +72,30 @@ struct alignas(16) ShapeBox {
 };
 static_assert(sizeof(ShapeBox) == 64, "ShapeBox");
 __device__ __forceinline__ const float *box_words(const ShapeBox &r) { return reinterpret_cast<const float *>(&r); }
+
+// Contiguous runs of fixed-size records moved between global memory and a wave's LDS staging area
+// with 16-byte accesses: lane l's record occupies dwords [l * WORDS, (l + 1) * WORDS) of the stage,
+// and the run of `count` (<= 64) records starts at a 16-byte aligned address.  A wave then reads or
+// writes its run in WORDS / 4 instructions instead of WORDS dword accesses per lane at a WORDS * 4-byte
+// stride (each of which touches as many cache lines as the run does).
+template <int WORDS>
+__device__ __forceinline__ void wave_store_records(const unsigned *stage, unsigned *dst, int count, int lane) {
+    const int total = count * WORDS;
+    const int full = total >> 2;
+    for (int c = lane; c < full; c += kWave)
+        *reinterpret_cast<uint4 *>(dst + 4 * c) = *reinterpret_cast<const uint4 *>(stage + 4 * c);
+    const int rem = total - 4 * full;
+    if (lane < rem) dst[4 * full + lane] = stage[4 * full + lane];
+}
+template <int WORDS>
+__device__ __forceinline__ void wave_load_records(const unsigned *src, unsigned *stage, int count, int lane) {
+    const int total = count * WORDS;
+    const int full = total >> 2;
+    for (int c = lane; c < full; c += kWave)
+        *reinterpret_cast<uint4 *>(stage + 4 * c) = *reinterpret_cast<const uint4 *>(src + 4 * c);
+    const int rem = total - 4 * full;
+    if (lane < rem) stage[4 * full + lane] = src[4 * full + lane];
+}
 
 __device__ __forceinline__ void bounds_body(const Shape *shapes, int n, float *partials, ShapeBox *boxes, int bid,
                                             int nblocks) {
@@ -132,15 +158,64 @@ __device__ __forceinline__ unsigned expand10(unsigned x) {
 }
 
 // CLBVH.cpp:125-164: quantise each centroid to 10 bits per axis (1024 wraps to 0 through the
-// low-10-bit interleave) and interleave x above y above z.  Also emits the sort key ~code (30 bits)
-// so that an ascending stable sort yields the reference's stable descending order.
-// Also stores the reference's per-shape outputs: index @68, morton_code @72, b @76 (one 32-byte run).
-__device__ __forceinline__ void morton_body(Shape *shapes, int n, const float *scene6, unsigned *keys,
-                                            const ShapeBox *boxes, int bid) {
-    int i = bid * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+// low-10-bit interleave) and interleave x above y above z.  The Morton pass also emits the sort key
+// ~code (30 bits), so that an ascending stable sort yields the reference's stable descending order,
+// and stores the reference's per-shape outputs: index @68, morton_code @72, b @76.
+__device__ __forceinline__ unsigned morton_code_of(float px, float py, float pz, const float *scene6) {
     const float mnx = scene6[0], mxx = scene6[1], mny = scene6[2], mxy = scene6[3], mnz = scene6[4],
                 mxz = scene6[5];
+    unsigned a = f2u_code(((px - mnx) / (mxx - mnx)) * 1024.f);
+    unsigned b = f2u_code(((py - mny) / (mxy - mny)) * 1024.f);
+    unsigned c = f2u_code(((pz - mnz) / (mxz - mnz)) * 1024.f);
+    return (expand10(a) << 2) | (expand10(b) << 1) | expand10(c);
+}
+
+// A wave handles 64 consecutive shapes.  When the Shape array is 16-byte aligned, the wave loads its
+// 64 whole records (26 words each) into LDS with 16-byte loads, fills in index, morton_code and b,
+// and stores the records back whole: full cache lines, where writing only each record's 32-byte tail
+// at a 104-byte stride left every line partly written.  The centroid then comes from the record's
+// own words; otherwise (a caller's unaligned array) each lane reads its box record and stores its tail.
+__device__ __forceinline__ void morton_body(Shape *shapes, int n, const float *scene6, unsigned *keys,
+                                            const ShapeBox *boxes, int bid) {
+    constexpr int kW = 26;   // words per Shape
+    __shared__ __align__(16) unsigned stage[kBoundsThreads / kWave][kWave * kW];
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    const long long first = (long long)bid * blockDim.x + (long long)wv * kWave;
+    if (first >= n) return;   // the whole wave is past the end
+    const int count = (long long)n - first < kWave ? (int)((long long)n - first) : kWave;
+    const int i = (int)first + lane;
+    if ((reinterpret_cast<size_t>(shapes) & 15) == 0) {
+        unsigned *st = stage[wv];
+        wave_load_records<kW>(reinterpret_cast<const unsigned *>(shapes + first), st, count, lane);
+        __builtin_amdgcn_wave_barrier();
+        if (lane < count) {
+            unsigned *rw = st + lane * kW;
+            float w[10];
+#pragma unroll
+            for (int f = 0; f < 10; f++) w[f] = __uint_as_float(rw[f]);
+            int type;
+            const Bound b = bound_of_words(w, &type);
+            float px, py, pz;
+            if (type == SPHERE) {
+                px = w[2]; py = w[3]; pz = w[4];
+            } else {
+                px = (b.min_x + b.max_x) / 2.0f;
+                py = (b.min_y + b.max_y) / 2.0f;
+                pz = (b.min_z + b.max_z) / 2.0f;
+            }
+            const unsigned code = morton_code_of(px, py, pz, scene6);
+            rw[17] = (unsigned)i;   // index @68
+            rw[18] = code;          // morton_code @72
+            rw[19] = __float_as_uint(b.min_x); rw[20] = __float_as_uint(b.max_x);   // b @76
+            rw[21] = __float_as_uint(b.min_y); rw[22] = __float_as_uint(b.max_y);
+            rw[23] = __float_as_uint(b.min_z); rw[24] = __float_as_uint(b.max_z);
+            keys[i] = (~code) & 0x3FFFFFFFu;
+        }
+        __builtin_amdgcn_wave_barrier();
+        wave_store_records<kW>(st, reinterpret_cast<unsigned *>(shapes + first), count, lane);
+        return;
+    }
+    if (lane >= count) return;
     const ShapeBox r = boxes[i];
     const float *w = box_words(r);   // w[10..15]: the AABB
     const int type = __float_as_int(w[0]);
@@ -152,12 +227,8 @@ __device__ __forceinline__ void morton_body(Shape *shapes, int n, const float *s
         py = (w[12] + w[13]) / 2.0f;
         pz = (w[14] + w[15]) / 2.0f;
     }
-    Shape *s = shapes + i;
-    unsigned a = f2u_code(((px - mnx) / (mxx - mnx)) * 1024.f);
-    unsigned b = f2u_code(((py - mny) / (mxy - mny)) * 1024.f);
-    unsigned c = f2u_code(((pz - mnz) / (mxz - mnz)) * 1024.f);
-    unsigned code = (expand10(a) << 2) | (expand10(b) << 1) | expand10(c);
-    unsigned *ws = reinterpret_cast<unsigned *>(s);
+    const unsigned code = morton_code_of(px, py, pz, scene6);
+    unsigned *ws = reinterpret_cast<unsigned *>(shapes + i);
     ws[17] = (unsigned)i;   // index @68
     ws[18] = code;          // morton_code @72
     for (int f = 0; f < 6; f++) ws[19 + f] = __float_as_uint(w[10 + f]);   // b @76
@@ -527,6 +598,8 @@ __device__ __forceinline__ void refit_levels_body(const ShapeBox *boxes, const u
                                                   const float *in_bounds, float *out_bounds, BVHNodeGPU *btn,
                                                   BVHNodeGPU *btl, TNode *tnodes, Prim *prims, bool tnodes_on, int bid) {
     __shared__ float sb[6][kRefitItems];
+    // per wave: 64 leaf records (15 words) or 64 Prims (12 words), staged for contiguous stores
+    __shared__ __align__(16) unsigned stage[LEAF ? 4 : 1][LEAF ? 64 * 15 : 4];
     const long long item_base = (long long)bid * kRefitItems;
     const long long m = ((long long)n + (1ll << s0) - 1) >> s0;   // items at level s0
     // leaf level: all of this thread's gathers are issued before any store (a store between them
@@ -548,15 +621,41 @@ __device__ __forceinline__ void refit_levels_body(const ShapeBox *boxes, const u
     for (int q = 0; q < kPer; q++) {
         const int j = threadIdx.x + q * 256;
         const long long it = item_base + j;
+        if (LEAF) {
+            // the wave's 64 items are consecutive leaves: their records go out as one contiguous run
+            const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+            const long long first = it - lane;
+            const int count = first >= m ? 0 : (m - first < kWave ? (int)(m - first) : kWave);
+            unsigned *st = stage[wv];
+            Bound B;
+            const float *w = box_words(recs[q]);
+            if (it < m) {
+                B.min_x = w[10]; B.max_x = w[11]; B.min_y = w[12]; B.max_y = w[13]; B.min_z = w[14]; B.max_z = w[15];
+                BVHNodeGPU rec;
+                write_leaf_record(&rec, sids[q], index_tree_parent(it, 0, n), B);
+                const unsigned *rw = reinterpret_cast<const unsigned *>(&rec);
+#pragma unroll
+                for (int f = 0; f < 15; f++) st[lane * 15 + f] = rw[f];
+                sb[0][j] = B.min_x; sb[1][j] = B.max_x; sb[2][j] = B.min_y;
+                sb[3][j] = B.max_y; sb[4][j] = B.min_z; sb[5][j] = B.max_z;
+            }
+            __builtin_amdgcn_wave_barrier();
+            wave_store_records<15>(st, reinterpret_cast<unsigned *>(btl + first), count, lane);
+            __builtin_amdgcn_wave_barrier();
+            if (it < m) {
+                const Prim p = make_prim(w, sids[q]);
+                const unsigned *pw = reinterpret_cast<const unsigned *>(&p);
+#pragma unroll
+                for (int f = 0; f < 12; f++) st[lane * 12 + f] = pw[f];
+            }
+            __builtin_amdgcn_wave_barrier();
+            wave_store_records<12>(st, reinterpret_cast<unsigned *>(prims + first), count, lane);
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
         if (it >= m) continue;
         Bound B;
-        if (LEAF) {
-            const int sid = sids[q];
-            const float *w = box_words(recs[q]);
-            B.min_x = w[10]; B.max_x = w[11]; B.min_y = w[12]; B.max_y = w[13]; B.min_z = w[14]; B.max_z = w[15];
-            write_leaf_record(btl + it, sid, index_tree_parent(it, 0, n), B);
-            prims[it] = make_prim(w, sid);
-        } else {
+        {
             const float *f = in_bounds + it * 6;
             B.min_x = f[0]; B.max_x = f[1]; B.min_y = f[2]; B.max_y = f[3]; B.min_z = f[4]; B.max_z = f[5];
         }
@@ -648,6 +747,10 @@ __device__ __forceinline__ void karras_morton_body(const unsigned *keys, int n, 
 constexpr int kMChunk = 1024;
 constexpr unsigned kMSuper = 65536;            // leaves per super-chunk (64 chunks)
 constexpr unsigned kXCap = 8192;               // arrivals per super-chunk list
+// Climb steps of one thread: a tree over 30-bit codes with ties split by a 32-bit index is at most 62
+// levels high, so a climb longer than this follows a malformed parent chain (a cycle): it stops and
+// sets bit 4 of the build's error word instead of spinning.
+constexpr int kMaxClimb = 64;
 __host__ __device__ constexpr unsigned mtop_cap(unsigned nsuper) { return 128u * (nsuper + 1u); }
 
 __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, const int *mpar_int, const int *mpar_leaf,
@@ -688,7 +791,14 @@ __device__ __forceinline__ void mrefit_chunk_body(int n, const int4 *mnode, cons
         int child = ~k, ch = 0;
         int p = slp[q];
         bool spans = false;
+        int steps = 0;
         while (p >= 0) {
+            if (++steps > kMaxClimb) {
+                atomicOr(err, 4u);
+                p = -1;
+                child = -1;
+                break;
+            }
             // a node whose leaf range lies inside the chunk has its index there too (Karras numbering:
             // the index is an end of the range), so an index outside the chunk means a spanning node
             if (p < base || p >= base + kMChunk) {
@@ -802,7 +912,11 @@ __device__ __forceinline__ void mrefit_mid_body(const int4 *mnode, const int *mp
             defer(p, E.y);
             continue;
         }
-        for (;;) {
+        for (int steps = 0;; steps++) {
+            if (steps > kMaxClimb) {
+                atomicOr(err, 4u);
+                break;
+            }
             const unsigned old = __hip_atomic_fetch_add(gcnt + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0) break;
             const int hp = mrefit_node(p, P, btl, tnodes, gbi, gh);
@@ -826,12 +940,16 @@ __device__ __forceinline__ void mrefit_mid_body(const int4 *mnode, const int *mp
 __device__ __forceinline__ void mrefit_top_body(const int4 *mnode, const int *mpar_int, const BVHNodeGPU *btl,
                                                 TNode *tnodes, float *gbi, int *gh, const int2 *xlist,
                                                 const unsigned *xcount, unsigned xcap, unsigned *gcnt,
-                                                unsigned *root_height) {
+                                                unsigned *root_height, unsigned *err) {
     const unsigned c = *xcount;
     const unsigned count = c < xcap ? c : xcap;
     for (unsigned e = threadIdx.x; e < count; e += blockDim.x) {
         int p = xlist[e].x;
-        for (;;) {
+        for (int steps = 0;; steps++) {
+            if (steps > kMaxClimb) {
+                atomicOr(err, 4u);
+                break;
+            }
             const unsigned old = __hip_atomic_fetch_add(gcnt + p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
             if (old == 0) break;
             const int hp = mrefit_node(p, mnode[p], btl, tnodes, gbi, gh);
