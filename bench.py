@@ -14,6 +14,7 @@ Prints one JSON line.
   python bench.py --serve            # only the served comparison
   python bench.py --batch            # only the batched (64-scene) comparison
   python bench.py --c3               # only the 327K-triangle icosphere, LBVH vs KD trees
+  python bench.py --c1               # only the Cornell spheres frame (no acceleration)
   python bench.py --build-sweep 16,20,23
 """
 from __future__ import annotations
@@ -42,7 +43,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
+def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0, w=None, h=None, depth=None, what=None):
     """The reference's own host path (geomfunc.h RadiancePathTracing, BVH mode, brute-force shadow
     rays), built from the reference sources into oracle/_ref/, timed on this host's cores on a
     pixel subset; its query count comes from the oracle in host-twin semantics, which follows the
@@ -50,6 +51,7 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
     from ptgpu_pc_amd.layout import default_seeds
+    W, H, DEPTH = (w or globals()["W"]), (h or globals()["H"]), (depth or globals()["DEPTH"])
     threads = min(os.cpu_count() or 1, 16)
     ref = O.ref_fn("ref_render", 1, DEPTH)
     kind = "reference" if ref is not None else "port"
@@ -78,9 +80,10 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0):
         rays += q[0] + q[1]
         total_pix += npix
         npix = min(npix * 2, 4096)
+    what = what or f"the same {W}x{H} frame"
     return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
-            "sample": f"{total_pix} pixels x 1 spp of the same 512x512 frame (BVH closest hit, brute-force "
-                      f"shadow rays over 65,538 shapes), {elapsed:.1f} s"}
+            "sample": f"{total_pix} pixels x 1 spp of {what} (BVH closest hit, brute-force shadow rays over "
+                      f"{len(shapes):,} shapes), {elapsed:.1f} s"}
 
 
 RENDER_KERNEL = "k_render_lanes"
@@ -171,9 +174,17 @@ def profile_summary(d: Path, out_md: Path) -> None:
     if ktc is not None and (d / "c3.json").exists():
         c3_summary(ktc, json.loads((d / "c3.json").read_text().strip().splitlines()[-1]), cmds,
                    ROOT / "profiles" / "c3_icosphere_1024x768x16spp.md")
+    ratios = {}
+    if "SQ_THREAD_CYCLES_VALU" in sq and "SQ_ACTIVE_INST_VALU" in sq:
+        ratios["valu_lane_util"] = sq["SQ_THREAD_CYCLES_VALU"] / (sq["SQ_ACTIVE_INST_VALU"] * 64)
+    if "SQ_ACTIVE_INST_VALU" in sq and "GRBM_GUI_ACTIVE" in sq:
+        ratios["valu_busy"] = sq["SQ_ACTIVE_INST_VALU"] * 4 / (1024 * sq["GRBM_GUI_ACTIVE"] / 8)
+    if "TCC_HIT_sum" in sq:
+        ratios["l2_hit"] = sq["TCC_HIT_sum"] / (sq["TCC_HIT_sum"] + sq["TCC_MISS_sum"])
     TRAFFIC_FILE.write_text(json.dumps({"kernel": RENDER_KERNEL, "workload": WORKLOAD, "traffic_bytes": traffic,
                                         "fetch_kib": f_k, "write_kib": w_k, "fetch_factor": f_scale,
                                         "write_factor": w_scale, "avg_ms_rocprof": render_avg,
+                                        **{k: round(v, 4) for k, v in ratios.items()},
                                         "summary": out_md.name}, indent=1) + "\n")
     print(out_md.read_text())
 
@@ -240,15 +251,23 @@ def c3_summary(db: Path, c3: dict, cmds: list, out_md: Path) -> None:
     print(out_md.read_text())
 
 
-def measured_traffic():
-    """HBM bytes per render launch from the committed rocprofv3 summary of this workload (or None)."""
+def measured_counters():
+    """Counter figures of the render kernel from the committed rocprofv3 summary of this workload (or
+    None): HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE), the profile's average duration, L2
+    hit rate, VALU busy fraction and VALU lane utilisation."""
     try:
         t = json.loads(TRAFFIC_FILE.read_text())
     except (OSError, ValueError):
         return None
     if t.get("kernel") != RENDER_KERNEL or t.get("workload") != WORKLOAD:
         return None
-    return int(t["traffic_bytes"])
+    return t
+
+
+def measured_traffic():
+    """HBM bytes per render launch from the committed rocprofv3 summary of this workload (or None)."""
+    t = measured_counters()
+    return int(t["traffic_bytes"]) if t else None
 
 
 # algorithmic bytes per primitive of each build stage (each logical load/store counted once at its
@@ -261,6 +280,118 @@ BUILD_BYTES_PER_PRIM = sum(STAGE_BYTES.values())   # 328 B per primitive with 8-
 # writes a 16-B node and 8 B of parents) + refit (reads a 24-B leaf bound, the 16-B node and a 4-B
 # parent, writes a 64-B traversal node)
 TRAV_TREE_BYTES = 4 + 16 + 8 + 24 + 16 + 4 + 64
+
+
+def render_roofline(alg_bytes, avg_ms, box_tests, prim_tests, box_bytes, measured=True, kernel=None):
+    """Roofline object of a render launch.  The path-tracing kernel is bound neither by HBM nor by VALU:
+    its time is the longest per-pixel chain of dependent node fetches (samples of a pixel run in order,
+    each query a chain of node visits), so `bound` says "latency"; `achieved` / `frac` price the
+    algorithmic bytes against the HBM peak as the contract asks, and `measured` gives what the counters
+    say the kernel actually moves (calibrated HBM traffic, L2 hit rate, VALU busy and lane use)."""
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    out = {"bound": "latency", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "kernel": kernel or f"{RENDER_KERNEL} (path-tracing megakernel, per-lane states)",
+           "limiter": "per-pixel chains of dependent node fetches (the samples of a pixel run in order)",
+           "per_launch": {"box_tests": box_tests, "prim_tests": prim_tests, "bytes_per_box_test": box_bytes,
+                          "alg_bytes": alg_bytes, "avg_ms": round(avg_ms, 4)}}
+    t = measured_counters() if measured else None
+    if t:
+        out["traffic"] = int(t["traffic_bytes"])
+        hbm = t["traffic_bytes"] / (t["avg_ms_rocprof"] * 1e-3) / 1e9
+        out["measured"] = {"hbm_gbs": round(hbm, 1), "hbm_frac": round(hbm / HBM_PEAK_GBS, 4),
+                           **{k: t[k] for k in ("l2_hit", "valu_busy", "valu_lane_util") if k in t},
+                           "source": f"profiles/{t['summary']}"}
+    return out
+
+
+def cpu_build_baseline(shapes):
+    """The reference's host-side LBVH construction as the oracle restates it, single-threaded: host
+    Morton codes (CLBVH.cpp:90-164), the literal stable descending bubble sort (CLBVH.cpp:31-47),
+    Karras topology over leaf indices (BVH.cl:141-208) and the bottom-up refit (BVH.cl:214-258)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    t0 = time.perf_counter()
+    O.build_lbvh(shapes, literal_sort=True)
+    sec = time.perf_counter() - t0
+    return {"value": round(len(shapes) / sec / 1e6, 4), "unit": "Mtri/s", "cores": 1, "kind": "port",
+            "sample": f"one whole LBVH build of the {len(shapes):,} shapes with the literal O(N^2) bubble sort, "
+                      f"{sec:.2f} s"}
+
+
+def c1_compare(steps: int = 10, warmup: int = 2, cpu: bool = True):
+    """Cornell spheres (9 shapes, no acceleration), 256x256, 4 spp, MAX_DEPTH 5: the GPU frame (one
+    fused 4-spp launch) against the reference's own host path (geomfunc.h, oracle/_ref) on the whole
+    frame on this host's cores."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import default_seeds
+    w = h = 256
+    spp, depth = 4, 5
+    shapes, lc, cam = scenes.cornell(w, h)
+    pt.update_camera(cam)
+    ctx = pt.Context(0)
+    d_sh, d_cam = ctx.to_device(shapes), ctx.to_device(cam)
+    seeds0 = ctx.to_device(default_seeds(w, h))
+    d_seeds = ctx.alloc(w * h * 8, np.uint32)
+    d_col = ctx.alloc(w * h * 12, np.float32)
+    d_pix = ctx.alloc(w * h * 4, np.uint32)
+    d_rays = ctx.alloc(16, np.uint64).zero()
+    d_stats = ctx.alloc(16, np.uint64).zero()
+
+    def frame(rays=None, stats=None):
+        d_seeds.copy_from(seeds0)
+        pt.render(ctx, None, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=len(shapes),
+                  lightCnt=lc, width=w, height=h, currentSample=0, spp=spp, max_depth=depth, pixels=d_pix,
+                  ray_counter=rays, trav_stats=stats)
+    for _ in range(warmup):
+        frame()
+    frame(stats=d_stats)
+    ctx.synchronize()
+    _, sphere_tests = (int(x) for x in d_stats.download(np.uint64)[:2])
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.record(0)
+        frame(rays=d_rays)
+        ctx.record(1)
+        ms.append(ctx.elapsed_ms(0, 1))
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    rays = int(d_rays.download(np.uint64)[:2].sum()) // steps
+    avg = float(np.mean(ms))
+    # algorithmic bytes: 16 B per sphere test (radius and centre) and the frame buffers
+    alg = 16 * sphere_tests + w * h * (24 + 16 + 4)
+    out = {"workload": "Cornell spheres (9 shapes, no acceleration), 256x256, 4 spp, MAX_DEPTH 5",
+           "mrays_s": round(rays * steps / wall / 1e6, 1), "frame_ms": round(wall * 1e3 / steps, 3),
+           "render_ms": round(avg, 4), "rays_per_frame": rays,
+           "roofline": {"bound": "latency", "achieved": round(alg / (avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                        "kernel": "k_render (lockstep megakernel; brute force over the 9 spheres)",
+                        "limiter": "per-pixel sample chains: 65,536 pixels fill a quarter of the chip's lanes",
+                        "per_launch": {"sphere_tests": sphere_tests, "alg_bytes": alg}}}
+    ctx.close()
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        ref = O.ref_fn("ref_render", 0, depth)
+        colors = np.zeros(w * h * 3, np.float32)
+        seeds = default_seeds(w, h)
+        t0 = time.perf_counter()
+        if ref is not None:
+            ref(colors.ctypes.data, seeds.ctypes.data, cam.ctypes.data, shapes.ctypes.data, len(shapes), lc, None,
+                None, 0, 0, w, h, 0, spp, None, 0, threads)
+        else:
+            O.radiance_frame(colors, seeds, cam, shapes, lc, O.accel_none(), w, h, 0, spp, None, depth, sem=1,
+                             threads=threads)
+        sec = time.perf_counter() - t0
+        q = O.radiance_frame(np.zeros(w * h * 3, np.float32), default_seeds(w, h), cam, shapes, lc, O.accel_none(),
+                             w, h, 0, spp, None, depth, sem=1, threads=threads)
+        out["cpu_baseline"] = {"value": round((q[0] + q[1]) / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "reference" if ref is not None else "port",
+                               "sample": f"the whole 256x256 x 4 spp frame, {sec:.2f} s"}
+    return out
 
 
 def build_sweep(sizes, reps=5):
@@ -674,7 +805,7 @@ def graph_compare(frames: int = 200, sizes=(128, 256), spp: int = 1):
                         "pixel readback into pinned memory, wait"}
 
 
-def c3_compare(steps: int = 3):
+def c3_compare(steps: int = 3, tune: bool = True, cpu: bool = False):
     """The 327,680-triangle displaced icosphere (seed 3) in a 12-triangle box with an emissive top,
     1024x768, 16 spp, MAX_DEPTH 6: the LBVH (plain and after treelet-7 restructuring) against the
     reference's KD tree (mean split) and the SAH KD tree (depth 24).  Every tree renders the same
@@ -694,6 +825,18 @@ def c3_compare(steps: int = 3):
     d_col = ctx.alloc(W3 * H3 * 12, np.float32)
     d_pix = ctx.alloc(W3 * H3 * 4, np.uint32)
     d_rays = ctx.alloc(16, np.uint64).zero()
+
+    d_stats = ctx.alloc(16, np.uint64)
+
+    def roofline_of(bvh, kd, ms, box_bytes):
+        """one untimed counting launch of the same frame: box and triangle tests"""
+        d_seeds.copy_from(seeds0)
+        d_stats.zero()
+        pt.render(ctx, bvh, kd, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=n, lightCnt=lc,
+                  width=W3, height=H3, currentSample=0, spp=SPP3, max_depth=D3, pixels=d_pix, trav_stats=d_stats)
+        box, prim = (int(x) for x in d_stats.download(np.uint64)[:2])
+        alg = box_bytes * box + 36 * prim + W3 * H3 * (24 + 16 + 4)
+        return render_roofline(alg, ms, box, prim, box_bytes, measured=False)
 
     def render(bvh, kd):
         best, rays = 1e9, 0
@@ -724,7 +867,7 @@ def c3_compare(steps: int = 3):
         build_ms = (time.perf_counter() - t0) * 1e3
         ms, rays, pix = render(bvh, None)
         out[name] = {"build_ms": round(build_ms, 2), "render_ms": round(ms, 2), "mrays_s": round(rays / ms / 1e3, 1),
-                     "rays": rays}
+                     "rays": rays, "roofline": roofline_of(bvh, None, ms, 16)}
         ref_pix = out.setdefault("_pix", pix)
         out[name]["pixels_identical"] = pix.tobytes() == ref_pix.tobytes()
     bvh.close()
@@ -735,21 +878,45 @@ def c3_compare(steps: int = 3):
         ms, rays, pix = render(None, kd)
         out[name] = {"build_ms_host": round(build_ms, 1), "render_ms": round(ms, 2),
                      "mrays_s": round(rays / ms / 1e3, 1), "rays": rays,
-                     "pixels_identical": pix.tobytes() == out["_pix"].tobytes(), **kd.stats()}
+                     "pixels_identical": pix.tobytes() == out["_pix"].tobytes(), **kd.stats(),
+                     "roofline": roofline_of(None, kd, ms, 32)}   # KD leaves share primitives: fp32 four-wide nodes
         kd.close()
     out.pop("_pix")
-    # the autotuner on this scene and frame (one frame per build): digit width, launch shape, treelet size
-    bvh = pt.CLBVH(ctx, d_sh, n)
-    bvh.buildRadixTree()
-    bvh.buildBVHTree()
-    d_seeds.copy_from(seeds0)
-    tuning = pt.autotune(ctx, bvh, frames_per_build=1, trials=1, colors=d_col, seeds=d_seeds, camera=d_cam,
-                         shapes=d_sh, shapeCnt=n, lightCnt=lc, width=W3, height=H3, currentSample=0, spp=SPP3,
-                         max_depth=D3, pixels=d_pix)
-    bvh.close()
+    tuning = None
+    if tune:
+        # the autotuner on this scene and frame (one frame per build): digit width, launch shape, treelet size
+        bvh = pt.CLBVH(ctx, d_sh, n)
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        d_seeds.copy_from(seeds0)
+        tuning = pt.autotune(ctx, bvh, frames_per_build=1, trials=1, colors=d_col, seeds=d_seeds, camera=d_cam,
+                             shapes=d_sh, shapeCnt=n, lightCnt=lc, width=W3, height=H3, currentSample=0, spp=SPP3,
+                             max_depth=D3, pixels=d_pix)
+        bvh.close()
     ctx.close()
-    return {"workload": "327,680-triangle displaced icosphere (seed 3) + 12-triangle box, 2 emissive, 1024x768, "
-                        "16 spp, MAX_DEPTH 6", "shapes": n, "trees": out, "tuning": tuning}
+    res = {"workload": "327,680-triangle displaced icosphere (seed 3) + 12-triangle box, 2 emissive, 1024x768, "
+                       "16 spp, MAX_DEPTH 6", "shapes": n, "trees": out, "tuning": tuning}
+    if cpu:
+        # the reference's host builds as the oracle restates them, single-threaded (the KD tree of
+        # KDTree.cpp:89-270; the LBVH with a stable sort in place of the O(N^2) bubble sort, which would
+        # take minutes at this size), and the reference's own host render path on a pixel subset
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as O
+        t0 = time.perf_counter()
+        s_h, _, btn, btl = O.build_lbvh(shapes)
+        lbvh_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.KD(shapes)
+        kd_s = time.perf_counter() - t0
+        res["cpu_build_baseline"] = {
+            "lbvh": {"value": round(n / lbvh_s / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+                     "sample": f"one LBVH build of the {n:,} shapes (std::stable_sort for the bubble sort), "
+                               f"{lbvh_s:.2f} s"},
+            "kd_mean_split": {"value": round(n / kd_s / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+                              "sample": f"one KDTree::build of the {n:,} shapes, {kd_s:.2f} s"}}
+        res["cpu_baseline"] = cpu_baseline(s_h, lc, cam, btn, btl, budget_s=6.0, w=W3, h=H3, depth=D3,
+                                           what="the same 1024x768 frame")
+    return res
 
 
 def main():
@@ -772,6 +939,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in defaults instead of autotuning")
     ap.add_argument("--graph", action="store_true", help="only the recorded-graph vs eager per-frame comparison")
     ap.add_argument("--c3", action="store_true", help="only the 327K-triangle icosphere: LBVH (+treelet 7) vs KD trees")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 327K-triangle icosphere block in the default line")
+    ap.add_argument("--c1", action="store_true", help="only the Cornell spheres (no acceleration) frame")
+    ap.add_argument("--no-c1", action="store_true", help="skip the Cornell spheres block in the default line")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph comparison in the default line")
     ap.add_argument("--frames", action="store_true",
                     help="only the C2 frame rendered 1, 2 and 4 per launch (settings of --tuning FILE, else the defaults)")
@@ -794,7 +964,10 @@ def main():
         print(json.dumps(graph_compare()), flush=True)
         return
     if args.c3:
-        print(json.dumps(c3_compare()), flush=True)
+        print(json.dumps(c3_compare(cpu=not args.no_cpu)), flush=True)
+        return
+    if args.c1:
+        print(json.dumps(c1_compare(cpu=not args.no_cpu)), flush=True)
         return
     if args.frames:
         tun = json.loads(Path(args.tuning).read_text()) if args.tuning else None
@@ -996,17 +1169,15 @@ def main():
     render_avg = float(np.mean(render_ms))
     build_avg = float(np.mean(build_ms))
 
-    # algorithmic bytes of one render launch: 24 B per child box tested, 36 B of vertices per
-    # triangle test, and the frame buffers (colors 12 B read+write, seeds 8 B read+write, pixels
-    # 4 B write per pixel)
-    alg_bytes = 24 * box_tests + 36 * prim_tests + W * H * (24 + 16 + 4)
-    achieved = alg_bytes / (render_avg * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic() if args.engine in ("auto", "lanes") and not args.treelet else None,
-                "kernel": f"{RENDER_KERNEL} (path-tracing megakernel, per-lane states)",
-                "per_launch": {"box_tests": box_tests, "prim_tests": prim_tests, "alg_bytes": alg_bytes,
-                               "avg_ms": round(render_avg, 4)}}
+    # algorithmic bytes of one render launch: the node bytes per child box tested (16 B with the
+    # default four-wide binary16 nodes, 32 B with fp32 four-wide or binary nodes), 36 B of vertices per
+    # triangle test, and the frame buffers (colors 12 B read+write, seeds 8 B read+write, pixels 4 B
+    # write per pixel)
+    wide = int(tuning["wide"]) if tuning else (args.wide if args.wide is not None else 2)
+    box_bytes = 16 if wide == 2 else 32
+    alg_bytes = box_bytes * box_tests + 36 * prim_tests + W * H * (24 + 16 + 4)
+    roofline = render_roofline(alg_bytes, render_avg, box_tests, prim_tests, box_bytes,
+                               measured=args.engine in ("auto", "lanes") and not args.treelet)
 
     served = None
     if not args.no_serve:
@@ -1025,11 +1196,20 @@ def main():
     if not args.no_batch:
         batched = batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)
 
-    cpu = None
+    c1 = None
+    if not args.no_c1:
+        c1 = c1_compare(cpu=not args.no_cpu)
+
+    c3 = None
+    if not args.no_c3:
+        c3 = c3_compare(steps=2, tune=False, cpu=not args.no_cpu)
+
+    cpu = cpu_build = None
     if not args.no_cpu:
         btn, btl = bvh.getTrees()
         s_host = d_shapes.download(dtype=shapes.dtype, shape=(n,))
         cpu = cpu_baseline(s_host, lc, cam, btn, btl)
+        cpu_build = cpu_build_baseline(shapes)
 
     out = {
         "metric": "Mrays/s + LBVH build Mtri/s; served p50/p99 frame latency vs plain GPU path",
@@ -1069,7 +1249,10 @@ def main():
         "batched": batched,
         "frames_per_launch": per_launch,
         "graph": graphed,
+        "c1": c1,
+        "c3": c3,
         "cpu_baseline": cpu,
+        "cpu_build_baseline": cpu_build,
     }
     print(json.dumps(out), flush=True)
     if pipeline:

@@ -261,6 +261,18 @@ typedef struct {
  * into pixels, seeds written back.  Trees are read in the reference layout. */
 int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *args);
 
+/* RadianceBoxGPU (rendering_kernel.cl:1152-1236, launched by ExecuteBoxKernel smallptGPU.cpp:636-650): one
+ * RadianceGPU sample for the pixels of one box, x..x+bwidth-1 by y..y+bheight-1 of the frame (pixels
+ * past the frame's edge are skipped), on whole-frame buffers.  The reference's seed row offset
+ * (y + ywithinbox - 1) is not reproduced: each pixel uses its own seeds, as in RadianceGPU. */
+int ptgpu_radiance_box_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *args, int x, int y, int bwidth,
+                           int bheight);
+/* DrawBox (smallptGPU.cpp:652-723) on the host: the same box rendered by the calling thread and up to
+ * `threads` - 1 more, on host buffers (every pointer in args a host pointer), with MAX_DEPTH max_depth.
+ * The kernels' arithmetic is used, so the box gets the bits ptgpu_radiance_box_gpu gives it. */
+int ptgpu_host_radiance_box(const ptgpu_radiance_args *args, int accel, int max_depth, int x, int y, int bwidth,
+                            int bheight, int threads);
+
 /* No reference counterpart: ptgpu_radiance_gpu for callers that cannot allocate device memory (code
  * set up for HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The same
  * arguments, work and bits as ptgpu_radiance_gpu; the call itself allocates nothing.  The traversal
@@ -294,6 +306,34 @@ typedef struct {
                                         selects a counting variant of the kernel (slower) */
 } ptgpu_render_args;
 int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *args);
+
+/* ptgpu_render over the listed tiles only: d_tile_origins holds ntiles (x, y) int pairs in device
+ * memory, each the top-left pixel of a tile_w x tile_h tile (pixels past the frame's edge are
+ * skipped).  All tiles go out in one launch; no reference counterpart (the reference launches
+ * RadianceBoxGPU once per box). */
+int ptgpu_render_tiles(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *args,
+                       const int *d_tile_origins, int ntiles, int tile_w, int tile_h);
+
+/* The reference's CPU_PARTRENDERING frame (DrawAllBoxes, smallptGPU.cpp:725-884): the frame is cut into
+ * bwidth x bheight tiles in row order; the first goes to the GPU, and each later one to the host when
+ * (GPU tiles) / (host tiles, counted from 1) has reached *rcpu, else to the GPU.  The GPU tiles go out
+ * in one launch (ptgpu_render_tiles) while `threads` host threads render the host tiles (the host
+ * side of ptgpu_host_radiance_box over the same prepared tree); the host tiles' rows are then copied
+ * into the frame buffers.  *rcpu becomes (host ms per host tile) / (GPU ms per GPU tile), the
+ * reference's update, for the next frame.  args is one ptgpu_render call's (spp samples from
+ * currentSample; no counters); the frame gets the bits a ptgpu_render of it gives. */
+typedef struct ptgpu_hybrid ptgpu_hybrid;
+typedef struct {
+    int gpu_tiles;
+    int cpu_tiles;
+    float gpu_ms;     /* the GPU tiles' launch, HIP events */
+    float cpu_ms;     /* the host tiles, wall time */
+    float total_ms;   /* the whole frame, wall time */
+} ptgpu_hybrid_stats;
+int ptgpu_hybrid_create(ptgpu_ctx *ctx, int threads, ptgpu_hybrid **out);
+int ptgpu_hybrid_frame(ptgpu_hybrid *h, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *args,
+                       int bwidth, int bheight, float *rcpu, ptgpu_hybrid_stats *stats);
+int ptgpu_hybrid_destroy(ptgpu_hybrid *h);
 
 /* Many frames in one launch (no reference counterpart: the batched form of the path).  Each job is
  * one ptgpu_render call's work -- a prepared tree (exactly one of bvh / kd, each with at least two

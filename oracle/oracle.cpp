@@ -749,41 +749,36 @@ static int or_intersect_p_bvh(const Shape *shapes, unsigned int shapeCnt, const 
 }
 
 /* sin and cos of a sampling angle.  Kernel semantics (sem 0): OpenCL leaves sin/cos to the
- * implementation within 4 ulp; the kernel semantics fix one evaluation -- x reduced modulo pi/2 with
- * pi/2 = hi + lo (hi with 33 significant bits), the Taylor series of sin to r^11 and of cos to r^12
- * on [-pi/4, pi/4], all in binary64, rounded once to binary32.  Host-twin semantics (sem 1): the C
- * library's sinf/cosf, as the reference's host build calls them (geomfunc.h). */
+ * implementation within 4 ulp; the kernel semantics fix one evaluation in binary32 -- Cody-Waite
+ * reduction x - k (P1 + P2 + P3), k = rint(x 2/pi), with pi/2 split so that k P1 and k P2 are exact,
+ * then sin r = r - r^3/3! + ... + r^9/9! and cos r = 1 - r^2/2! + ... - r^10/10! in Horner form on
+ * [-pi/4, pi/4].  Host-twin semantics (sem 1): the C library's sinf/cosf, as the reference's host
+ * build calls them (geomfunc.h). */
 void or_sincos(float xf, float *s, float *c, int sem) {
     if (sem != 0) {
         *s = sinf(xf);
         *c = cosf(xf);
         return;
     }
-    const double two_over_pi = 0.6366197723675814, pio2_hi = 1.5707963267341256, pio2_lo = 6.077100506506192e-11;
-    const double x = xf;
-    const double k = rint(x * two_over_pi);
-    const double r = (x - k * pio2_hi) - k * pio2_lo;
-    const double r2 = r * r;
-    /* sin r = r - r^3/3! + r^5/5! - ... - r^11/11!;  cos r = 1 - r^2/2! + ... + r^12/12! */
-    const double sin_r =
+    const float two_over_pi = 0.6366197466850281f;
+    const float p1 = 1.5703125f, p2 = 4.837512969970703e-4f, p3 = 7.549790126404332e-8f;
+    const float k = rintf(xf * two_over_pi);
+    const float r = ((xf - k * p1) - k * p2) - k * p3;
+    const float r2 = r * r;
+    const float sin_r =
         r + r * r2 *
-                (-0.16666666666666666 +
-                 r2 * (0.008333333333333333 +
-                       r2 * (-0.0001984126984126984 + r2 * (2.7557319223985893e-06 + r2 * -2.505210838544172e-08))));
-    const double cos_r =
-        1.0 + r2 * (-0.5 + r2 * (0.041666666666666664 +
-                                 r2 * (-0.001388888888888889 +
-                                       r2 * (2.48015873015873e-05 +
-                                             r2 * (-2.755731922398589e-07 + r2 * 2.08767569878681e-09)))));
-    double sv, cv;
+                (-0.1666666716337204f +
+                 r2 * (0.008333333767950535f + r2 * (-0.00019841270113829523f + r2 * 2.7557318844628753e-06f)));
+    const float cos_r =
+        1.0f + r2 * (-0.5f + r2 * (0.0416666679084301f +
+                                   r2 * (-0.0013888889225199819f +
+                                         r2 * (2.4801587642286904e-05f + r2 * -2.755731998149713e-07f))));
     switch ((int)k & 3) {             /* x = k pi/2 + r */
-        case 0: sv = sin_r; cv = cos_r; break;
-        case 1: sv = cos_r; cv = -sin_r; break;
-        case 2: sv = -sin_r; cv = -cos_r; break;
-        default: sv = -cos_r; cv = sin_r; break;
+        case 0: *s = sin_r; *c = cos_r; break;
+        case 1: *s = cos_r; *c = -sin_r; break;
+        case 2: *s = -sin_r; *c = -cos_r; break;
+        default: *s = -cos_r; *c = sin_r; break;
     }
-    *s = (float)sv;
-    *c = (float)cv;
 }
 
 /* UniformSampleSphere rendering_kernel.cl:127-135 */

@@ -179,6 +179,7 @@ struct ptgpu_ctx {
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
     ptgpu::DevBuf tmp_parents;           // parent map and height of a caller's BVH (ptgpu_radiance_gpu)
+    ptgpu::DevBuf tmp_tiles;             // tile origins of RadianceBoxGPU launches
     ptgpu::BatchChannel batch;            // ptgpu_render_batch launches (on the context's stream)
     ptgpu::BuildStaging build_staging;    // batched LBVH builds
     bool capturing = false;               // between ptgpu_graph_begin and ptgpu_graph_end
@@ -229,8 +230,17 @@ struct RenderLaunch {
     unsigned long long *ray_counter;
     unsigned long long *trav_stats;
     unsigned *work_counter = nullptr;   // device word for the persistent waves' pixel counter; NULL: the context's
+    const int2 *tiles = nullptr;        // device tile origins: render only these tile_w x tile_h tiles
+    int ntiles = 0, tile_w = 0, tile_h = 0;
 };
 int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
+// ptgpu_render's checks and preparation (traversal structure, light list) without the launch
+int prepare_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a, DevAccel *acc,
+                   RenderLaunch *L);
+// ptgpu_render's preparation, then a launch over the listed tiles (device int2 origins, tile_w x tile_h
+// each); acc and L receive the resolved structure and launch description (host tiles use them too)
+int render_tiles(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a, const int2 *d_tiles,
+                 int ntiles, int tile_w, int tile_h, DevAccel *acc, RenderLaunch *L);
 // device address of this GPU's allocation-free scratch, looked up once (device current)
 int noalloc_scratch_init(int device);
 int launch_render_wavefront(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);

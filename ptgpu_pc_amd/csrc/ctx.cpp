@@ -130,6 +130,7 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     ctx->tmp_scan.release();
     ctx->tmp_counter.release();
     ctx->tmp_parents.release();
+    ctx->tmp_tiles.release();
     ctx->batch.release();
     ctx->build_staging.release();
     wf_release(ctx);

@@ -58,7 +58,21 @@ struct RenderParams {
     const RenderJob *jobs;        // batch launches: job table, pixel prefix sums (njobs + 1)
     const unsigned *job_begin;
     int njobs;
+    const int2 *tiles;            // tiled launches (RadianceBoxGPU): tile origins; NULL renders the whole frame
+    int tile_w, tile_h;
 };
+
+// The frame pixel of claimed work item m: pixel m of the frame, or, in a tiled launch, pixel
+// m % (tile_w * tile_h) of tile m / (tile_w * tile_h) in row-major order within the tile; -1 when a
+// partial tile at the frame's right or bottom edge has no pixel there.
+__device__ __forceinline__ int claimed_pixel(const RenderParams &P, unsigned m) {
+    if (!P.tiles) return (int)m;
+    const unsigned per = (unsigned)(P.tile_w * P.tile_h);
+    const int2 o = P.tiles[m / per];
+    const unsigned t = m % per;
+    const int x = o.x + (int)(t % (unsigned)P.tile_w), y = o.y + (int)(t / (unsigned)P.tile_w);
+    return (x < P.width && y < P.height) ? y * P.width + x : -1;
+}
 
 // The path-tracing megakernel.  Lane state: one pixel (gid), its remaining samples, the RNG state
 // and running mean; path state: ray, radiance, throughput, specularBounce, depth.
@@ -92,9 +106,10 @@ __global__ __launch_bounds__(256) void k_render(RenderParams P) {
                 if (base + cnt >= (unsigned)P.total) drained = true;
                 if (!active) {
                     const unsigned my = base + (unsigned)__popcll(need & lt);
-                    if (my < (unsigned)P.total) {
+                    const int g = my < (unsigned)P.total ? claimed_pixel(P, my) : -1;
+                    if (g >= 0) {   // (a tile pixel past the frame's edge: the lane claims again)
                         active = true;
-                        gid = (int)my;
+                        gid = g;
                         const int x = gid % P.width, y = gid / P.width;
                         ci = (P.height - y - 1) * P.width + x;
                         s0 = P.seeds[2 * gid];
@@ -805,17 +820,21 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         j_spp = P.jobs[jb].spp;
                         j_currentSample = P.jobs[jb].currentSample;
                     } else {
-                        gid = (int)my;
+                        gid = claimed_pixel(P, my);
                     }
-                    const int W = JFR(width), H = JFR(height);
-                    const int x = gid % W, y = gid / W;
-                    s0 = JF(seeds)[2 * gid];
-                    s1 = JF(seeds)[2 * gid + 1];
-                    k = 0;
-                    if (JFR(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
-                    camera_ray(MULTI ? j_camera : &s_cam, s0, s1, W, H, x, y, ro, rd);
-                    rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
-                    st = kSeg;
+                    if (gid >= 0) {
+                        const int W = JFR(width), H = JFR(height);
+                        const int x = gid % W, y = gid / W;
+                        s0 = JF(seeds)[2 * gid];
+                        s1 = JF(seeds)[2 * gid + 1];
+                        k = 0;
+                        if (JFR(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
+                        camera_ray(MULTI ? j_camera : &s_cam, s0, s1, W, H, x, y, ro, rd);
+                        rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
+                        st = kSeg;
+                    } else {
+                        st = drained ? kIdle : kNeed;   // a tile pixel past the frame's edge: claim again
+                    }
                 } else {
                     st = kIdle;
                 }
@@ -961,23 +980,6 @@ __global__ __launch_bounds__(256) void k_trace(DevAccel A, const Ray *rays, cons
 }
 
 // ---- scene preparation kernels ----
-__device__ __forceinline__ Prim prim_of(const Shape *s, int id, int flags) {
-    const float *w = reinterpret_cast<const float *>(s);
-    const int type = reinterpret_cast<const int *>(s)[0] | flags;
-    Prim p;
-    if ((type & 0xff) == SPHERE) {
-        p.p0 = make_float4(w[2], w[3], w[4], __int_as_float(id));
-        p.p1 = make_float4(w[1], 0.f, 0.f, __int_as_float(type));
-        p.p2 = make_float4(0.f, 0.f, 0.f, __int_as_float(type));
-    } else {
-        const float x1 = w[1], y1 = w[2], z1 = w[3];
-        p.p0 = make_float4(x1, y1, z1, __int_as_float(id));
-        p.p1 = make_float4(w[4] - x1, w[5] - y1, w[6] - z1, __int_as_float(type));
-        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, __int_as_float(type));
-    }
-    return p;
-}
-
 __global__ void k_index_prims(const Shape *shapes, int n, Prim *prims) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) prims[i] = prim_of(shapes + i, i, kLastInLeaf);
@@ -1398,10 +1400,12 @@ static int stack_for(const DevAccel &acc) {
 }
 
 int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
-    const int total = L.width * L.height;
+    // work items: every pixel, or every pixel slot of the listed tiles (slots past the frame's edge are skipped)
+    const int total = L.tiles ? L.ntiles * L.tile_w * L.tile_h : L.width * L.height;
     if (total <= 0) return PTGPU_OK;
-    const bool wf_ok = acc.mode != PTGPU_ACCEL_NONE && wf_supported(L.nlights) && !L.trav_stats;
-    if (ctx->engine == 2) PT_CHECK(wf_ok, PTGPU_ERR_UNSUPPORTED, "wavefront engine: needs a tree, <= 8 lights, no stats");
+    const bool wf_ok = acc.mode != PTGPU_ACCEL_NONE && wf_supported(L.nlights) && !L.trav_stats && !L.tiles;
+    if (ctx->engine == 2)
+        PT_CHECK(wf_ok, PTGPU_ERR_UNSUPPORTED, "wavefront engine: needs a tree, <= 8 lights, no stats, no tiles");
     // the wavefront schedule only on request: one frame keeps at most W*H paths in flight, too few
     // to amortise its per-bounce barriers (measured 2x slower than the megakernels at 512x512)
     if (wf_ok && ctx->engine == 2) return launch_render_wavefront(ctx, acc, L);
@@ -1436,6 +1440,9 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.jobs = nullptr;
     P.job_begin = nullptr;
     P.njobs = 0;
+    P.tiles = L.tiles;
+    P.tile_w = L.tile_w;
+    P.tile_h = L.tile_h;
     // persistent waves: enough workgroups to fill every CU several times, never more than needed
     int blocks = ctx->cu_count * ctx->blocks_per_cu;
     const int need = (total + 255) / 256;
@@ -1544,9 +1551,8 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
     return PTGPU_OK;
 }
 
-extern "C" {
-
-int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) {
+// RadianceGPU (box == NULL) and RadianceBoxGPU (box = {x, y, bwidth, bheight}) over reference-layout trees.
+static int radiance_gpu_impl(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a, const int *box) {
     PT_CHECK(ctx && a, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(a->colors && a->seeds && a->camera && a->shapes && a->pixels, PTGPU_ERR_INVALID, "null buffer");
     PT_CHECK(a->shapeCnt >= 1, PTGPU_ERR_INVALID, "shapeCnt is 0");
@@ -1589,7 +1595,31 @@ int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) 
     L.pixels = a->pixels;
     L.ray_counter = nullptr;
     L.trav_stats = nullptr;
+    if (box) {   // the one box, as a one-entry tile list
+        PT_TRY(ctx->tmp_tiles.ensure(64));
+        PT_HIP(hipMemcpyAsync(ctx->tmp_tiles.p, box, 8, hipMemcpyHostToDevice, ctx->stream));
+        L.tiles = ctx->tmp_tiles.as<int2>();
+        L.ntiles = 1;
+        L.tile_w = box[2];
+        L.tile_h = box[3];
+    }
     return launch_render(ctx, acc, L);
+}
+
+extern "C" {
+
+int ptgpu_radiance_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a) {
+    return radiance_gpu_impl(ctx, accel, a, nullptr);
+}
+
+int ptgpu_radiance_box_gpu(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_args *a, int x, int y, int bwidth,
+                           int bheight) {
+    PT_CHECK(a, PTGPU_ERR_INVALID, "null argument");
+    PT_CHECK(x >= 0 && y >= 0 && bwidth >= 1 && bheight >= 1 && x < a->width && y < a->height &&
+                 (long long)bwidth * bheight < (1ll << 30),
+             PTGPU_ERR_INVALID, "bad box (%d,%d) %dx%d in a %dx%d frame", x, y, bwidth, bheight, a->width, a->height);
+    const int box[4] = {x, y, bwidth, bheight};
+    return radiance_gpu_impl(ctx, accel, a, box);
 }
 
 }  // extern "C"
@@ -1729,7 +1759,12 @@ int ptgpu_radiance_gpu_noalloc(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_a
     return launch_render(ctx, acc, L);
 }
 
-int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a) {
+}  // extern "C"
+
+// Checks a ptgpu_render argument set, resolves its traversal structure and light list (the handle's
+// prepared layout, or index-order primitives without a tree) and fills the launch description.
+int ptgpu::prepare_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a,
+                          DevAccel *acc, RenderLaunch *L) {
     PT_CHECK(ctx && a, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(a->colors && a->seeds && a->camera && a->shapes, PTGPU_ERR_INVALID, "null buffer");
     PT_CHECK(a->shapeCnt >= 1, PTGPU_ERR_INVALID, "shapeCnt is 0");
@@ -1740,8 +1775,7 @@ int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu
     PT_CHECK(a->spp >= 1 && a->currentSample >= 0, PTGPU_ERR_INVALID, "bad spp/currentSample");
     PT_CHECK(a->max_depth >= 0 && a->max_depth < 1024, PTGPU_ERR_INVALID, "bad max_depth");
     PT_HIP(hipSetDevice(ctx->device));
-    DevAccel acc;
-    PT_TRY(resolve_accel(ctx, bvh, kd, a->shapes, a->shapeCnt, &acc));
+    PT_TRY(resolve_accel(ctx, bvh, kd, a->shapes, a->shapeCnt, acc));
     const int *lights = nullptr;
     int nl = 0;
     if (bvh) {
@@ -1759,23 +1793,53 @@ int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu
                                 &nl));
         lights = ctx->tmp_lights.as<int>();
     }
+    L->colors = a->colors;
+    L->seeds = a->seeds;
+    L->camera = a->camera;
+    L->shapes = a->shapes;
+    L->shapeCnt = a->shapeCnt;
+    L->lights = lights;
+    L->nlights = nl;
+    L->width = a->width;
+    L->height = a->height;
+    L->currentSample = a->currentSample;
+    L->spp = a->spp;
+    L->max_depth = a->max_depth;
+    L->pixels = a->pixels;
+    L->ray_counter = a->ray_counter;
+    L->trav_stats = a->trav_stats;
+    return PTGPU_OK;
+}
+
+int ptgpu::render_tiles(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a,
+                        const int2 *d_tiles, int ntiles, int tile_w, int tile_h, DevAccel *acc, RenderLaunch *L) {
+    PT_CHECK(tile_w >= 1 && tile_h >= 1 && ntiles >= 0 && (long long)ntiles * tile_w * tile_h < (1ll << 30),
+             PTGPU_ERR_INVALID, "bad tiles (%d of %dx%d)", ntiles, tile_w, tile_h);
+    PT_TRY(prepare_render(ctx, bvh, kd, a, acc, L));
+    if (ntiles == 0) return PTGPU_OK;
+    PT_CHECK(d_tiles, PTGPU_ERR_INVALID, "null tile list");
+    L->tiles = d_tiles;
+    L->ntiles = ntiles;
+    L->tile_w = tile_w;
+    L->tile_h = tile_h;
+    return launch_render(ctx, *acc, *L);
+}
+
+extern "C" {
+
+int ptgpu_render(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a) {
+    DevAccel acc;
     RenderLaunch L;
-    L.colors = a->colors;
-    L.seeds = a->seeds;
-    L.camera = a->camera;
-    L.shapes = a->shapes;
-    L.shapeCnt = a->shapeCnt;
-    L.lights = lights;
-    L.nlights = nl;
-    L.width = a->width;
-    L.height = a->height;
-    L.currentSample = a->currentSample;
-    L.spp = a->spp;
-    L.max_depth = a->max_depth;
-    L.pixels = a->pixels;
-    L.ray_counter = a->ray_counter;
-    L.trav_stats = a->trav_stats;
+    PT_TRY(prepare_render(ctx, bvh, kd, a, &acc, &L));
     return launch_render(ctx, acc, L);
+}
+
+int ptgpu_render_tiles(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const ptgpu_render_args *a,
+                       const int *d_tile_origins, int ntiles, int tile_w, int tile_h) {
+    DevAccel acc;
+    RenderLaunch L;
+    return render_tiles(ctx, bvh, kd, a, reinterpret_cast<const int2 *>(d_tile_origins), ntiles, tile_w, tile_h,
+                        &acc, &L);
 }
 
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter) {

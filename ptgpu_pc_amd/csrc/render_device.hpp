@@ -7,6 +7,10 @@
 
 #include "device_util.hpp"
 
+// The per-path math below is also compiled for the host: the hybrid CPU/GPU renderer runs it on host
+// cores for its CPU tiles, so those tiles get the kernel's bits.
+#define PT_HD __host__ __device__ __forceinline__
+
 namespace ptgpu {
 
 constexpr float kSlack = 1.0f + 1.0e-5f;   // relative slack on the slab test's exit distance
@@ -14,62 +18,80 @@ constexpr float kInf = 1e20f;              // the reference's initial t (renderi
 constexpr unsigned kMaxVisits = 1u << 24;  // node visits per query; only a cyclic tree reaches it
 
 struct f3 { float x, y, z; };
-__device__ __forceinline__ f3 mk(float x, float y, float z) { f3 v; v.x = x; v.y = y; v.z = z; return v; }
-__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ f3 mul3(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ f3 smul(float k, f3 b) { return mk(k * b.x, k * b.y, k * b.z); }
-__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+PT_HD f3 mk(float x, float y, float z) { f3 v; v.x = x; v.y = y; v.z = z; return v; }
+PT_HD f3 sub3(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+PT_HD f3 add3(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+PT_HD f3 mul3(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+PT_HD f3 smul(float k, f3 b) { return mk(k * b.x, k * b.y, k * b.z); }
+PT_HD float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+PT_HD f3 cross3(f3 a, f3 b) {
     return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ f3 norm3(f3 v) { float l = 1.f / sqrtf(dot3(v, v)); return smul(l, v); }
-__device__ __forceinline__ bool iszero3(f3 v) { return v.x == 0.f && v.y == 0.f && v.z == 0.f; }
-__device__ __forceinline__ f3 ldvec(const Vec *p) {
+PT_HD f3 norm3(f3 v) { float l = 1.f / sqrtf(dot3(v, v)); return smul(l, v); }
+PT_HD bool iszero3(f3 v) { return v.x == 0.f && v.y == 0.f && v.z == 0.f; }
+PT_HD f3 ldvec(const Vec *p) {
     const float *f = reinterpret_cast<const float *>(p);
     return mk(f[0], f[1], f[2]);
 }
 
 // ---- sin and cos of the sampling angles (rendering_kernel.cl:131-132, 652-654) ----
 // The reference calls OpenCL's sin/cos, whose results the OpenCL specification bounds (4 ulp) but
-// does not fix.  This library evaluates them one documented way: reduction of x modulo pi/2 with
-// pi/2 split in two parts (the high part has 33 significant bits, so k * hi is exact), then the
-// Taylor series of sin (to r^11) and cos (to r^12) on [-pi/4, pi/4], all in binary64, rounded once
-// to binary32.  The truncation error is below 1e-11 there, so each result is within a hair of
-// correct rounding, and the same IEEE binary64 operations give the same bits in any conforming
-// implementation of them (the CPU restatement uses this evaluation too).
-__device__ __forceinline__ void sincos_f32(float xf, float *s, float *c) {
-    const double x = xf;
-    const double k = rint(x * 0.6366197723675814);                 // 2/pi
-    const double r = (x - k * 1.5707963267341256) - k * 6.077100506506192e-11;
-    const double r2 = r * r;
-    const double sp =
+// does not fix.  This library evaluates them one documented way, in binary32: Cody-Waite reduction
+// of x by k = rint(x * 2/pi) multiples of pi/2 split in three parts (P1 and P2 with few enough
+// significant bits that k * P1 and k * P2 are exact), then the Taylor series of sin (to r^9) and cos
+// (to r^10) on [-pi/4, pi/4] in Horner form; the truncation error there is below 0.05 ulp and the
+// whole evaluation stays within about one ulp.  The same IEEE binary32 operations in the same order
+// give the same bits on any conforming implementation (the CPU restatement evaluates them so too).
+PT_HD void sincos_f32(float x, float *s, float *c) {
+    const float k = rintf(x * 0.6366197466850281f);                        // 2/pi
+    const float r = ((x - k * 1.5703125f) - k * 4.837512969970703e-4f) - k * 7.549790126404332e-8f;
+    const float r2 = r * r;
+    const float sp =
         r + r * r2 *
-                (-0.16666666666666666 +
-                 r2 * (0.008333333333333333 +
-                       r2 * (-0.0001984126984126984 + r2 * (2.7557319223985893e-06 + r2 * -2.505210838544172e-08))));
-    const double cp =
-        1.0 + r2 * (-0.5 + r2 * (0.041666666666666664 +
-                                 r2 * (-0.001388888888888889 +
-                                       r2 * (2.48015873015873e-05 +
-                                             r2 * (-2.755731922398589e-07 + r2 * 2.08767569878681e-09)))));
-    const int q = (int)k & 3;                                    // quadrant
-    const double sv = (q & 1) ? cp : sp, cv = (q & 1) ? sp : cp;
-    *s = (float)((q & 2) ? -sv : sv);
-    *c = (float)(((q + 1) & 2) ? -cv : cv);
+                (-0.1666666716337204f +
+                 r2 * (0.008333333767950535f + r2 * (-0.00019841270113829523f + r2 * 2.7557318844628753e-06f)));
+    const float cp =
+        1.0f + r2 * (-0.5f + r2 * (0.0416666679084301f +
+                                   r2 * (-0.0013888889225199819f +
+                                         r2 * (2.4801587642286904e-05f + r2 * -2.755731998149713e-07f))));
+    const int q = (int)k & 3;                                              // quadrant
+    const float sv = (q & 1) ? cp : sp, cv = (q & 1) ? sp : cp;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
+}
+
+// The traversal record of shape `s` (index id): v0 and the FP32 edges e1 = v1 - v0, e2 = v2 - v0 the
+// reference forms in its triangle test, or the sphere's centre and radius; `flags` (kLastInLeaf)
+// joins the type word.
+PT_HD Prim prim_of(const Shape *s, int id, int flags) {
+    const float *w = reinterpret_cast<const float *>(s);
+    const int type = reinterpret_cast<const int *>(s)[0] | flags;
+    const float fid = __builtin_bit_cast(float, id), ftype = __builtin_bit_cast(float, type);
+    Prim p;
+    if ((type & 0xff) == SPHERE) {
+        p.p0 = make_float4(w[2], w[3], w[4], fid);
+        p.p1 = make_float4(w[1], 0.f, 0.f, ftype);
+        p.p2 = make_float4(0.f, 0.f, 0.f, ftype);
+    } else {
+        const float x1 = w[1], y1 = w[2], z1 = w[3];
+        p.p0 = make_float4(x1, y1, z1, fid);
+        p.p1 = make_float4(w[4] - x1, w[5] - y1, w[6] - z1, ftype);
+        p.p2 = make_float4(w[7] - x1, w[8] - y1, w[9] - z1, ftype);
+    }
+    return p;
 }
 
 // ---- GetRandom, rendering_kernel.cl:4-17 ----
-__device__ __forceinline__ float get_random(unsigned &s0, unsigned &s1) {
+PT_HD float get_random(unsigned &s0, unsigned &s1) {
     s0 = 36969u * (s0 & 65535u) + (s0 >> 16);
     s1 = 18000u * (s1 & 65535u) + (s1 >> 16);
     unsigned ires = (s0 << 16) + s1;
-    float f = __uint_as_float((ires & 0x007fffffu) | 0x40000000u);
+    float f = __builtin_bit_cast(float, (ires & 0x007fffffu) | 0x40000000u);
     return (f - 2.f) / 2.f;
 }
 
 // ---- primitive tests (rendering_kernel.cl:26-125) on the Prim layout ----
-__device__ __forceinline__ float tri_isect(const float4 &p0, const float4 &p1, const float4 &p2, f3 o, f3 d) {
+PT_HD float tri_isect(const float4 &p0, const float4 &p1, const float4 &p2, f3 o, f3 d) {
     const f3 v0 = mk(p0.x, p0.y, p0.z), e1 = mk(p1.x, p1.y, p1.z), e2 = mk(p2.x, p2.y, p2.z);
     const f3 pvec = cross3(d, e2);
     const float det = dot3(e1, pvec);
@@ -84,7 +106,7 @@ __device__ __forceinline__ float tri_isect(const float4 &p0, const float4 &p1, c
     const float t = dot3(e2, qvec) * inv;
     return t > kEpsilon ? t : 0.f;
 }
-__device__ __forceinline__ float sph_isect(const float4 &p0, float rad, f3 o, f3 d) {
+PT_HD float sph_isect(const float4 &p0, float rad, f3 o, f3 d) {
     const f3 op = sub3(mk(p0.x, p0.y, p0.z), o);
     const float b = dot3(op, d);
     float det = b * b - dot3(op, op) + rad * rad;
@@ -95,10 +117,10 @@ __device__ __forceinline__ float sph_isect(const float4 &p0, float rad, f3 o, f3
     t = b + det;
     return t > kEpsilon ? t : 0.f;
 }
-__device__ __forceinline__ float prim_isect(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id) {
+PT_HD float prim_isect(const Prim *__restrict__ prims, int k, f3 o, f3 d, int *id) {
     const float4 p0 = prims[k].p0, p1 = prims[k].p1, p2 = prims[k].p2;
-    *id = __float_as_int(p0.w);
-    if ((__float_as_int(p2.w) & 0xff) == SPHERE) return sph_isect(p0, p1.x, o, d);
+    *id = __builtin_bit_cast(int, p0.w);
+    if ((__builtin_bit_cast(int, p2.w) & 0xff) == SPHERE) return sph_isect(p0, p1.x, o, d);
     return tri_isect(p0, p1, p2, o, d);
 }
 
@@ -142,7 +164,7 @@ template <int K>
 __device__ __forceinline__ float qxorf(float v) { return __int_as_float(qxor<K>(__float_as_int(v))); }
 
 // slab test of one box; returns entry distance (>= 0) or +inf on a miss
-__device__ __forceinline__ float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
+PT_HD float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv,
                                       float tlimit) {
     const float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
     const float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
@@ -281,7 +303,7 @@ __device__ __forceinline__ bool anyhit(const DevAccel &A, f3 o, f3 d, float maxt
 }
 
 // ---- camera ray, rendering_kernel.cl:1042-1070 ----
-__device__ __forceinline__ void camera_ray(const Camera *cam, unsigned &s0, unsigned &s1, int width, int height,
+PT_HD void camera_ray(const Camera *cam, unsigned &s0, unsigned &s1, int width, int height,
                                            int x, int y, f3 &ro, f3 &rd) {
     const float invW = 1.f / (float)width, invH = 1.f / (float)height;
     const float r1 = get_random(s0, s1) - .5f;
@@ -294,7 +316,7 @@ __device__ __forceinline__ void camera_ray(const Camera *cam, unsigned &s0, unsi
     rd = norm3(rdir);
 }
 
-__device__ __forceinline__ int to_int(float x) {   // vec.h:60 with the OpenCL clamp builtin
+PT_HD int to_int(float x) {   // vec.h:60 with the OpenCL clamp builtin
     return (int)(powf(fminf(fmaxf(x, 0.f), 1.f), 1.f / 2.2f) * 255.f + .5f);
 }
 
@@ -305,7 +327,7 @@ struct Surface {
     float dp;
     int refl;
 };
-__device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
+PT_HD Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
     // words 0-16 loaded together before the type is looked at: one memory round trip, where loads
     // placed in the type's branches would wait for the type word first
     float w[17];
@@ -313,7 +335,7 @@ __device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
 #pragma unroll
     for (int i = 0; i < 17; i++) w[i] = src[i];
     asm volatile("" ::: "memory");   // compiler-only: keeps the loads here, ahead of the branches
-    const int type = __float_as_int(w[0]);
+    const int type = __builtin_bit_cast(int, w[0]);
     Surface S;
     f3 nrm;
     if (type == SPHERE) {
@@ -329,16 +351,16 @@ __device__ __forceinline__ Surface surface_at(const Shape *sh, f3 hp, f3 rd) {
     S.nl = smul(-1.f * sgn, nrm);
     S.e = mk(w[10], w[11], w[12]);
     S.c = mk(w[13], w[14], w[15]);
-    S.refl = __float_as_int(w[16]);
+    S.refl = __builtin_bit_cast(int, w[16]);
     return S;
 }
 
 // ---- one light of SampleLights (rendering_kernel.cl:421-490): draws two randoms (u1 first),
 // returns true when a shadow ray must decide the contribution s * e; sd/len describe that ray.
 // lw: words 0-12 of the light's Shape (type, geometry, emission); area: its word 25
-__device__ __forceinline__ bool sample_light_words(const float *lw, float area, unsigned &s0, unsigned &s1, f3 hp,
+PT_HD bool sample_light_words(const float *lw, float area, unsigned &s0, unsigned &s1, f3 hp,
                                                    f3 nl, f3 *sd_out, float *len_out, f3 *contrib) {
-    const int ltype = __float_as_int(lw[0]);
+    const int ltype = __builtin_bit_cast(int, lw[0]);
     f3 lp, ln;
     if (ltype == TRIANGLE) {
         const float u1 = get_random(s0, s1);
@@ -379,7 +401,7 @@ __device__ __forceinline__ bool sample_light_words(const float *lw, float area, 
     *contrib = smul(s, mk(lw[10], lw[11], lw[12]));
     return true;
 }
-__device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsigned &s1, f3 hp, f3 nl, f3 *sd_out,
+PT_HD bool sample_light(const Shape *L, unsigned &s0, unsigned &s1, f3 hp, f3 nl, f3 *sd_out,
                                              float *len_out, f3 *contrib) {
     // the words the sample reads, loaded together ahead of the branch on the light's type (one
     // memory round trip instead of the type word first, then the geometry)
@@ -393,7 +415,7 @@ __device__ __forceinline__ bool sample_light(const Shape *L, unsigned &s0, unsig
 }
 
 // ---- cosine-weighted diffuse bounce (rendering_kernel.cl:629-661) ----
-__device__ __forceinline__ f3 diffuse_dir(f3 nl, unsigned &s0, unsigned &s1) {
+PT_HD f3 diffuse_dir(f3 nl, unsigned &s0, unsigned &s1) {
     const float r1 = 2.f * kPi * get_random(s0, s1);
     const float r2 = get_random(s0, s1);
     const float r2s = sqrtf(r2);
@@ -411,7 +433,7 @@ __device__ __forceinline__ f3 diffuse_dir(f3 nl, unsigned &s0, unsigned &s1) {
 }
 
 // ---- specular and refractive continuation (rendering_kernel.cl:666-752); updates thr/rd ----
-__device__ __forceinline__ void specular_or_refract(const Surface &S, f3 &rd, f3 &thr, unsigned &s0, unsigned &s1) {
+PT_HD void specular_or_refract(const Surface &S, f3 &rd, f3 &thr, unsigned &s0, unsigned &s1) {
     f3 nd = smul(2.f * dot3(S.nrm, rd), S.nrm);
     nd = sub3(rd, nd);
     if (S.refl == SPEC) {
@@ -454,14 +476,14 @@ __device__ __forceinline__ void specular_or_refract(const Surface &S, f3 &rd, f3
 }
 
 // ---- running mean of RadianceGPU (rendering_kernel.cl:1127-1136) ----
-__device__ __forceinline__ f3 running_mean(f3 col, f3 rad, int cs) {
+PT_HD f3 running_mean(f3 col, f3 rad, int cs) {
     if (cs == 0) return rad;
     const float k1 = (float)cs;
     const float k2 = 1.f / ((float)cs + 1.f);
     return mk((col.x * k1 + rad.x) * k2, (col.y * k1 + rad.y) * k2, (col.z * k1 + rad.z) * k2);
 }
 
-__device__ __forceinline__ unsigned pack_rgba(f3 col) {
+PT_HD unsigned pack_rgba(f3 col) {
     return (unsigned)to_int(col.x) | ((unsigned)to_int(col.y) << 8) | ((unsigned)to_int(col.z) << 16) | 0xff000000u;
 }
 

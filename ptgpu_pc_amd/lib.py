@@ -118,6 +118,12 @@ _SIGS = {
     "ptgpu_kdtree_destroy": [vp],
     "ptgpu_radiance_gpu": [vp, i32, C.POINTER(RadianceArgs)],
     "ptgpu_radiance_gpu_noalloc": [vp, i32, C.POINTER(RadianceArgs)],
+    "ptgpu_radiance_box_gpu": [vp, i32, C.POINTER(RadianceArgs), i32, i32, i32, i32],
+    "ptgpu_host_radiance_box": [C.POINTER(RadianceArgs), i32, i32, i32, i32, i32, i32, i32],
+    "ptgpu_render_tiles": [vp, vp, vp, C.POINTER(RenderArgs), vp, i32, i32, i32],
+    "ptgpu_hybrid_create": [vp, i32, C.POINTER(vp)],
+    "ptgpu_hybrid_frame": [vp, vp, vp, C.POINTER(RenderArgs), i32, i32, C.POINTER(C.c_float), vp],
+    "ptgpu_hybrid_destroy": [vp],
     "ptgpu_render": [vp, vp, vp, C.POINTER(RenderArgs)],
     "ptgpu_render_batch": [vp, vp, i32, vp],
     "ptgpu_server_create": [vp, C.c_size_t, i32, vp],
@@ -608,6 +614,64 @@ def render(ctx: Context, bvh: CLBVH | None, kd: KDTree | None, *, colors: Device
     _check(ctx.lib.ptgpu_render(ctx.handle, _h(bvh), _h(kd), C.byref(a)))
 
 
+def render_tiles(ctx: Context, bvh: CLBVH | None, kd: KDTree | None, tiles: np.ndarray, tile_w: int, tile_h: int,
+                 **render_kwargs) -> None:
+    """`render` over the listed tiles only: tiles (n, 2) int32 top-left pixels of tile_w x tile_h tiles."""
+    t = np.ascontiguousarray(tiles, dtype=np.int32).reshape(-1, 2)
+    d_t = ctx.to_device(t) if len(t) else None
+    a = _render_args(**render_kwargs)
+    _check(ctx.lib.ptgpu_render_tiles(ctx.handle, _h(bvh), _h(kd), C.byref(a), d_t.ptr if d_t else None, len(t),
+                                      int(tile_w), int(tile_h)))
+    if d_t:
+        ctx.synchronize()
+        d_t.free()
+
+
+class HybridStats(C.Structure):
+    _fields_ = [("gpu_tiles", i32), ("cpu_tiles", i32), ("gpu_ms", C.c_float), ("cpu_ms", C.c_float),
+                ("total_ms", C.c_float)]
+
+
+class Hybrid:
+    """The reference's CPU_PARTRENDERING frame (DrawAllBoxes): tiles split between the GPU and `threads`
+    host threads by the ratio rcpu, which each frame updates from the measured time per tile."""
+
+    def __init__(self, ctx: Context, threads: int):
+        self.ctx = ctx
+        h = vp()
+        _check(ctx.lib.ptgpu_hybrid_create(ctx.handle, int(threads), C.byref(h)))
+        self.handle = h.value
+        ctx._children.add(self)
+
+    def frame(self, bvh: CLBVH | None, kd: KDTree | None, bwidth: int, bheight: int, rcpu: float,
+              **render_kwargs) -> tuple:
+        """One frame; returns (the updated rcpu, stats dict)."""
+        a = _render_args(**render_kwargs)
+        r = C.c_float(rcpu)
+        st = HybridStats()
+        _check(self.ctx.lib.ptgpu_hybrid_frame(self.handle, _h(bvh), _h(kd), C.byref(a), int(bwidth), int(bheight),
+                                               C.byref(r), C.byref(st)))
+        return r.value, {k: getattr(st, k) for k, _ in HybridStats._fields_}
+
+    def close(self) -> None:
+        if self.handle and self.ctx.handle:
+            self.ctx.lib.ptgpu_hybrid_destroy(self.handle)
+        self.handle = None
+
+
+def host_radiance_box(accel: int, colors: np.ndarray, seeds: np.ndarray, camera: np.ndarray, shapes: np.ndarray,
+                      lightCnt: int, width: int, height: int, currentSample: int, pixels: np.ndarray, x: int, y: int,
+                      bwidth: int, bheight: int, max_depth: int = 6, btn=None, btl=None, kng=None, kn=None,
+                      threads: int = 1) -> None:
+    """DrawBox on the host (ptgpu_host_radiance_box): one sample of one box, in place on host arrays."""
+    lib = load_library()
+    p = lambda x: _ptr(x) if x is not None else None  # noqa: E731
+    a = RadianceArgs(p(colors), p(seeds), p(camera), p(shapes), len(shapes), lightCnt, p(btn), p(btl), p(kng),
+                     len(kng) if kng is not None else 0, p(kn), len(kn) if kn is not None else 0, width, height,
+                     currentSample, p(pixels))
+    _check(lib.ptgpu_host_radiance_box(C.byref(a), accel, max_depth, x, y, bwidth, bheight, threads))
+
+
 def autotune(ctx: Context, bvh: CLBVH, *, frames_per_build: int = 1, trials: int = 2, **render_kwargs) -> dict:
     """ptgpu_autotune: digit width, launch shape and treelet size for this scene and frame (the keyword
     arguments of `render`); applied to ctx and bvh, which is left holding a tree built with them."""
@@ -724,15 +788,19 @@ def render_batch(ctx: Context, jobs: list, ray_counter: DeviceArray | None = Non
 
 def radiance_gpu(ctx: Context, accel: int, colors, seeds, camera, shapes, shapeCnt, lightCnt, btn=None, btl=None,
                  kng=None, kngCnt=0, kn=None, knCnt=0, width=0, height=0, currentSample=0, pixels=None,
-                 noalloc: bool = False) -> None:
+                 noalloc: bool = False, box=None) -> None:
     """One RadianceGPU launch with the reference argument list (device pointers or DeviceArrays).
-    noalloc: through ptgpu_radiance_gpu_noalloc (the library's static scratch; allocates nothing)."""
+    noalloc: through ptgpu_radiance_gpu_noalloc (the library's static scratch; allocates nothing).
+    box = (x, y, bwidth, bheight): RadianceBoxGPU over that box only."""
     def p(x):
         if x is None:
             return None
         return x.ptr if isinstance(x, DeviceArray) else int(x)
     a = RadianceArgs(p(colors), p(seeds), p(camera), p(shapes), shapeCnt, lightCnt, p(btn), p(btl), p(kng), kngCnt,
                      p(kn), knCnt, width, height, currentSample, p(pixels))
+    if box is not None:
+        _check(ctx.lib.ptgpu_radiance_box_gpu(ctx.handle, accel, C.byref(a), *[int(v) for v in box]))
+        return
     fn = ctx.lib.ptgpu_radiance_gpu_noalloc if noalloc else ctx.lib.ptgpu_radiance_gpu
     _check(fn(ctx.handle, accel, C.byref(a)))
 

@@ -665,9 +665,10 @@ def test_shadow_rays_through_bvh_equal_bruteforce():
 
 
 def test_sincos_kernel_semantics_against_the_mathematics():
-    """The kernel semantics' sin/cos (Taylor series in binary64, one rounding to binary32) against
+    """The kernel semantics' sin/cos (Cody-Waite reduction and Taylor series, all in binary32) against
     float64 sin/cos rounded to float32: within one binary32 ulp everywhere on the sampling angles
-    2 pi u (u in [0, 1)) and beyond, and equal to the correctly rounded value almost always."""
+    2 pi u (u in [0, 1)) and beyond, and equal to the correctly rounded value for most arguments.
+    A dropped series term or a wrong reduction constant costs far more than one ulp."""
     rng = np.random.default_rng(4)
     u = rng.random(6000).astype(F32)
     xs = np.concatenate([F32(2.0) * F32(math.pi) * u,                       # the angles the kernel draws
@@ -680,8 +681,10 @@ def test_sincos_kernel_semantics_against_the_mathematics():
         rs, rc = np.float32(math.sin(float(x))), np.float32(math.cos(float(x)))
         for got, want in ((s, rs), (c, rc)):
             ulp = np.spacing(np.abs(want)) if want != 0 else np.float32(1e-45)
-            assert abs(float(got) - float(want)) <= float(ulp), (x, got, want)
+            # pi/2 split in three binary32 parts leaves a residue of 1.7e-15 per multiple removed (plus
+            # the rounding of k P3): an absolute error below 3e-15 |x|, seen only near zeros of sin and cos
+            assert abs(float(got) - float(want)) <= float(ulp) + 3e-15 * abs(float(x)), (x, got, want)
         exact_s += s == rs
         exact_c += c == rc
-    assert exact_s >= 0.999 * len(xs) and exact_c >= 0.999 * len(xs)
+    assert exact_s >= 0.75 * len(xs) and exact_c >= 0.75 * len(xs)
     assert O.sincos(0.0) == (0.0, 1.0)
