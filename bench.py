@@ -800,9 +800,74 @@ def graph_compare(frames: int = 200, sizes=(128, 256), spp: int = 1):
                                "pixels_identical": last["eager"].tobytes() == last["graph"].tobytes()}
     bvh.close()
     ctx.close()
-    return {"frames": frames, "spp": spp, "sizes": out,
+    return {"frames": frames, "spp": spp, "sizes": out, "c2_step": graph_c2_step(),
             "workload": "C2 scene, orbiting camera, per frame: camera upload, seed reset, colour clear, render, "
                         "pixel readback into pinned memory, wait"}
+
+
+def graph_c2_step(steps: int = 20, treelet: int = 5):
+    """The whole C2 step -- LBVH rebuild from the device-resident shapes, treelet pass, 8-spp 512x512
+    render, pixel readback, wait -- issued call by call against replayed as one recorded graph.  Issued
+    eagerly, the first traversal after each rebuild reads the tree height back (a wait in mid-step);
+    recorded, the render sizes its stack from the last height read plus a margin and checks the
+    built height on the device."""
+    import ptgpu_pc_amd as pt
+    from ptgpu_pc_amd import scenes
+    from ptgpu_pc_amd.layout import default_seeds
+    shapes, lc, cam = scenes.soup(N_TRI, seed=1)
+    pt.update_camera(cam)
+    n = len(shapes)
+    ctx = pt.Context(0)
+    d_sh = ctx.to_device(shapes)
+    d_cam = ctx.to_device(cam)
+    d_seeds0 = ctx.to_device(default_seeds(W, H))
+    d_seeds = ctx.alloc(W * H * 8, np.uint32)
+    d_col = ctx.alloc(W * H * 12, np.float32)
+    d_pix = ctx.alloc(W * H * 4, np.uint32)
+    h_pix = ctx.pinned(W * H * 4, np.uint32)
+    bvh = pt.CLBVH(ctx, d_sh, n)
+
+    def step():
+        bvh.reset()
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        if treelet:
+            bvh.optimize(treelet)
+        ctx.memcpy_async(d_seeds, d_seeds0, W * H * 8)
+        d_col.zero()
+        pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=n, lightCnt=lc,
+                  width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=d_pix)
+        ctx.memcpy_async(h_pix, d_pix, W * H * 4)
+
+    step()                           # heights read back, buffers sized, light list built
+    ctx.synchronize()
+    ctx.graph_begin()
+    try:
+        step()
+    finally:
+        g = ctx.graph_end()
+    times, last = {}, {}
+    for mode in ("eager", "graph", "eager", "graph"):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            if mode == "graph":
+                g.launch()
+            else:
+                step()
+            ctx.synchronize()
+        times[mode] = min(times.get(mode, 1e9), time.perf_counter() - t0)
+        last[mode] = h_pix.array.copy()
+    g.close()
+    h_pix.close()
+    bvh.close()
+    ctx.close()
+    return {"steps": steps, "treelet": treelet,
+            "eager_ms_per_step": round(1e3 * times["eager"] / steps, 3),
+            "graph_ms_per_step": round(1e3 * times["graph"] / steps, 3),
+            "speedup": round(times["eager"] / times["graph"], 3),
+            "pixels_identical": last["eager"].tobytes() == last["graph"].tobytes(),
+            "workload": f"C2 step: LBVH rebuild + treelet {treelet} + {SPP}-spp {W}x{H} render + pixel readback, "
+                        "one stream, one step in flight"}
 
 
 def c3_compare(steps: int = 3, tune: bool = True, cpu: bool = False):

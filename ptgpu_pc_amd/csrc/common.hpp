@@ -93,6 +93,10 @@ struct DevAccel {
     int depth = 0;         // tree height: deepest leaf in edges from the root (bounds the stack)
     int mode = PTGPU_ACCEL_NONE;
     int leaf_runs = 0;     // 1: every leaf range is disjoint and its last Prim carries kLastInLeaf
+    // A traversal recorded into a graph right after a build, before its height could be read back:
+    // `depth` is then a bound, and the build's status words (error word, -, height) are checked
+    // against it on the device before any traversal.  NULL when `depth` is the tree's height.
+    const unsigned *height_check = nullptr;
 };
 
 // ---- errors ----
@@ -180,6 +184,7 @@ struct ptgpu_ctx {
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
     ptgpu::DevBuf tmp_parents;           // parent map and height of a caller's BVH (ptgpu_radiance_gpu)
     ptgpu::DevBuf tmp_tiles;             // tile origins of RadianceBoxGPU launches
+    ptgpu::DevBuf rec_status;            // recorded renders' device-side checks: flags, height seen
     ptgpu::BatchChannel batch;            // ptgpu_render_batch launches (on the context's stream)
     ptgpu::BuildStaging build_staging;    // batched LBVH builds
     bool capturing = false;               // between ptgpu_graph_begin and ptgpu_graph_end

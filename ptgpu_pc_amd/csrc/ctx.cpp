@@ -115,6 +115,13 @@ int ptgpu_ctx_create(int device, void *stream, ptgpu_ctx **out) {
     // the allocation-free entry's scratch address on this device (checked again by that entry,
     // which reports a failed look-up; a context that never uses it works without it)
     if (ptgpu::noalloc_scratch_init(device) != PTGPU_OK) (void)hipGetLastError();   // keep no stale HIP error
+    // status words of recorded renders (allocated here: a recording must not be the first to allocate)
+    int s = c->rec_status.ensure(16);
+    if (s == PTGPU_OK) s = ptgpu::fill_u32(c->stream, c->rec_status.p, 0u, 4);
+    if (s != PTGPU_OK) {
+        ptgpu_ctx_destroy(c);
+        return s;
+    }
     *out = c;
     return PTGPU_OK;
 }
@@ -131,6 +138,7 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     ctx->tmp_counter.release();
     ctx->tmp_parents.release();
     ctx->tmp_tiles.release();
+    ctx->rec_status.release();
     ctx->batch.release();
     ctx->build_staging.release();
     wf_release(ctx);
@@ -145,6 +153,19 @@ int ptgpu_synchronize(ptgpu_ctx *ctx) {
     PT_CHECK(ctx, PTGPU_ERR_INVALID, "ctx is NULL");
     PT_NO_CAPTURE(ctx, "ptgpu_synchronize");
     PT_HIP(hipStreamSynchronize(ctx->stream));
+    // what the device-side checks of recorded renders found since the last synchronisation
+    unsigned w[2] = {0, 0};
+    PT_HIP(hipMemcpyAsync(w, ctx->rec_status.p, 8, hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    if (w[0]) {
+        PT_TRY(ptgpu::fill_u32(ctx->stream, ctx->rec_status.p, 0u, 2));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        PT_CHECK(!(w[0] & 1u), PTGPU_ERR_HIP, "a recorded render found a failed LBVH build and rendered nothing");
+        PT_CHECK(false, PTGPU_ERR_UNSUPPORTED,
+                 "a recorded render found a tree of height %u, deeper than the traversal stack it was recorded "
+                 "with, and rendered nothing (render the scene once outside the recording, then record again)",
+                 w[1]);
+    }
     return PTGPU_OK;
 }
 

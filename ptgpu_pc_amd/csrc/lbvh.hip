@@ -336,15 +336,28 @@ static int clbvh_sort(ptgpu_clbvh *h) {
 namespace ptgpu {
 // The Morton-keyed tree's height is known on the device after the refit; the traversal stack is
 // sized from it, so the first use after a build reads it back (one 4-byte copy).
-int clbvh_prepare_traversal(ptgpu_clbvh *h) {
-    if (h->height_pending) {
-        PT_NO_CAPTURE(h->ctx, "the first traversal after a build (it reads the tree height back)");
+int clbvh_prepare_traversal(ptgpu_clbvh *h, bool recording) {
+    if (h->height_pending && (recording || h->ctx->capturing)) {
+        // Recorded into a graph: the height cannot be read back.  Size the stack for a bound and
+        // let the traversal check the built tree against it on the device (error word, height).
+        // The read-back stays pending for the next eager use.
+        const int known = h->known_height[h->optimized ? 1 : 0];
+        int bound = known > 0 ? known + kRecordedHeightMargin : kRecordedHeightDefault;
+        // a known height the four-wide stack holds keeps the bound within it (the margin is a guess;
+        // the four-wide walk is the fast one)
+        if (known > 0 && known <= kRecordedHeightDefault && bound > kRecordedHeightDefault)
+            bound = kRecordedHeightDefault;
+        h->acc.depth = bound;
+        h->acc.height_check = misc_counters(h) + 9;
+    } else if (h->height_pending) {
         unsigned w[3] = {0, 0, 0};   // error word, pass-3 arrivals, height
         PT_HIP(hipMemcpyAsync(w, misc_counters(h) + 9, 12, hipMemcpyDeviceToHost, h->ctx->stream));
         PT_HIP(hipStreamSynchronize(h->ctx->stream));
         PT_CHECK(w[0] == 0, PTGPU_ERR_HIP, "LBVH build failed (error word %u: 1 sort look-back spin limit, 2 "
                  "traversal-tree list overflow, 4 climb beyond the height bound)", w[0]);
         h->acc.depth = (int)w[2];
+        h->acc.height_check = nullptr;
+        h->known_height[h->optimized ? 1 : 0] = (int)w[2];
         h->height_pending = false;
     }
     if (h->wide_pending) {
@@ -394,6 +407,7 @@ void ptgpu::clbvh_mark_built(ptgpu_clbvh *h) {
     h->acc.nodes4 = nullptr;
     h->acc.nodes4h = nullptr;
     h->wide_pending = n >= 2;   // four-wide nodes are derived at the first traversal after a build
+    h->acc.height_check = nullptr;
     if (n >= 2) {
         h->acc.root = 0;
         h->acc.root_cnt = 0;

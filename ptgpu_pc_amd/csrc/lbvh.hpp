@@ -6,7 +6,13 @@
 #include "common.hpp"
 
 namespace ptgpu {
-int clbvh_prepare_traversal(ptgpu_clbvh *h);
+// Makes the handle's traversal layout current (tree height, four-wide nodes).  `recording`: the
+// caller's stream is being recorded into a graph, so nothing may be read back.
+int clbvh_prepare_traversal(ptgpu_clbvh *h, bool recording = false);
+// Height bound of a traversal recorded before the height could be read back: the height last read
+// for a build of the same kind (plain or optimised) plus this margin, or the default without one.
+constexpr int kRecordedHeightMargin = 4;
+constexpr int kRecordedHeightDefault = 41;   // the deepest tree the 64-entry four-wide stack walks
 int clbvh_export_optimized(ptgpu_clbvh *h);
 
 // Scratch of the Morton-keyed traversal tree: topology, parents, and the refit's crossing queue.
@@ -51,6 +57,7 @@ struct ptgpu_clbvh {
     bool export_pending = false;             // btn/btl still describe the tree before optimize
     int trav_topology = 1;    // traversal tree: 0 = the exported index-keyed tree, 1 = Morton-keyed LBVH
     bool height_pending = false;
+    int known_height[2] = {0, 0};   // last heights read back: [0] plain build, [1] after optimize
     unsigned *sorted = nullptr;
     int lights_for = -1, nlights = 0;
     ptgpu::DevAccel acc;

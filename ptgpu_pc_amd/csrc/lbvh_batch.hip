@@ -350,6 +350,7 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         clbvh_mark_built(h);
         if (h->height_pending) {
             h->acc.depth = (int)h_out[s];
+            h->known_height[0] = (int)h_out[s];
             h->height_pending = false;
         }
         sort_err |= h_out[ns + s] != 0;

@@ -60,7 +60,24 @@ struct RenderParams {
     int njobs;
     const int2 *tiles;            // tiled launches (RadianceBoxGPU): tile origins; NULL renders the whole frame
     int tile_w, tile_h;
+    int depth_cap;                // deepest tree the launch's stack holds (checked when acc.height_check is set)
+    unsigned *status;             // the context's check words: [0] flags, [1] deepest height seen
 };
+
+// A launch recorded right after a build sized its stack for a height bound (acc.height_check set).
+// The built tree is checked before any work: a build error (flag 1) or a tree deeper than the stack
+// holds (flag 2) is recorded in the context's status words and the launch renders nothing.  Every
+// thread reads the same words, so whole workgroups return together, before any barrier.
+__device__ __forceinline__ bool recorded_tree_unfit(const RenderParams &P) {
+    if (!P.acc.height_check) return false;
+    const unsigned err = P.acc.height_check[0], h = P.acc.height_check[2];
+    if (err == 0u && h <= (unsigned)P.depth_cap) return false;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicOr(P.status, err ? 1u : 2u);
+        atomicMax(P.status + 1, h);
+    }
+    return true;
+}
 
 // The frame pixel of claimed work item m: pixel m of the frame, or, in a tiled launch, pixel
 // m % (tile_w * tile_h) of tile m / (tile_w * tile_h) in row-major order within the tile; -1 when a
@@ -78,6 +95,7 @@ __device__ __forceinline__ int claimed_pixel(const RenderParams &P, unsigned m) 
 // and running mean; path state: ray, radiance, throughput, specularBounce, depth.
 template <int STACK, bool STATS>
 __global__ __launch_bounds__(256) void k_render(RenderParams P) {
+    if (recorded_tree_unfit(P)) return;
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
     const DevAccel &A = P.acc;
@@ -231,6 +249,7 @@ enum : int {
 
 template <int STACK, bool STATS, bool WIDE, bool MULTI, bool HALF>
 __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
+    if (!MULTI && recorded_tree_unfit(P)) return;
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
     const unsigned long long lt = lanemask_lt();
@@ -1392,6 +1411,10 @@ static int stack_for_wide(const DevAccel &acc) {
     return -1;
 }
 
+// the deepest tree each stack holds (inverse of the two functions above and below)
+static int wide_depth_cap(int stack) { return 2 * ((stack - 1) / 3) - 1; }
+static int binary_depth_cap(int stack) { return stack - 2; }
+
 static int stack_for(const DevAccel &acc) {
     if (acc.mode == PTGPU_ACCEL_NONE) return 24;
     if (acc.depth + 2 <= 24) return 24;
@@ -1408,6 +1431,8 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
         PT_CHECK(wf_ok, PTGPU_ERR_UNSUPPORTED, "wavefront engine: needs a tree, <= 8 lights, no stats, no tiles");
     // the wavefront schedule only on request: one frame keeps at most W*H paths in flight, too few
     // to amortise its per-bounce barriers (measured 2x slower than the megakernels at 512x512)
+    PT_CHECK(!(acc.height_check && ctx->engine == 2), PTGPU_ERR_UNSUPPORTED,
+             "wavefront engine: a traversal recorded before the tree height was read back");
     if (wf_ok && ctx->engine == 2) return launch_render_wavefront(ctx, acc, L);
     const int st = stack_for(acc);
     PT_CHECK(st > 0, PTGPU_ERR_UNSUPPORTED, "tree depth %d exceeds the 62-level traversal stack", acc.depth);
@@ -1443,6 +1468,12 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.tiles = L.tiles;
     P.tile_w = L.tile_w;
     P.tile_h = L.tile_h;
+    P.depth_cap = binary_depth_cap(st);
+    P.status = nullptr;
+    if (acc.height_check) {
+        PT_TRY(ctx->rec_status.ensure(16));   // allocated at context creation; kept here for safety
+        P.status = ctx->rec_status.as<unsigned>();
+    }
     // persistent waves: enough workgroups to fill every CU several times, never more than needed
     int blocks = ctx->cu_count * ctx->blocks_per_cu;
     const int need = (total + 255) / 256;
@@ -1453,6 +1484,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     if ((ctx->engine == 0 || ctx->engine == 3) && acc.mode != PTGPU_ACCEL_NONE && acc.root >= 0) {
         const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc) : -1;
         const bool half = ctx->wide == 2 && acc.nodes4h;
+        if (sw > 0) P.depth_cap = wide_depth_cap(sw);
 #define PT_LANES(ST, STATS_, WIDE_, HALF_)                                                                      \
     hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_, false, HALF_>), dim3(blocks), dim3(256), 0, ctx->stream, P)
 #define PT_LANES_W(ST, STATS_) \
@@ -1491,6 +1523,8 @@ int launch_trace(ptgpu_ctx *ctx, const DevAccel &acc, unsigned int shapeCnt, con
                  int n, float *d_t, int *d_id, int *d_hit, bool any) {
     (void)shapeCnt;
     if (n <= 0) return PTGPU_OK;
+    PT_CHECK(!acc.height_check, PTGPU_ERR_UNSUPPORTED,
+             "batch queries cannot be recorded before the tree height was read back");
     const int st = stack_for(acc);
     PT_CHECK(st > 0, PTGPU_ERR_UNSUPPORTED, "tree depth %d exceeds the 62-level traversal stack", acc.depth);
     if (st == 24)
@@ -1512,7 +1546,6 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
                          DevAccel *acc);
 
 extern "C" int ptgpu_kdtree_device_accel(ptgpu_kdtree *kd, ptgpu::DevAccel *acc);
-namespace ptgpu { int clbvh_prepare_traversal(ptgpu_clbvh *h); }
 
 static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes, unsigned n,
                          DevAccel *acc) {
@@ -1520,7 +1553,7 @@ static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, con
     if (bvh) {
         PT_CHECK(bvh->state >= 3, PTGPU_ERR_INVALID, "BVH not built (call build_bvh_tree)");
         PT_CHECK((unsigned)bvh->n == n, PTGPU_ERR_INVALID, "shapeCnt %u differs from the BVH's %d", n, bvh->n);
-        PT_TRY(clbvh_prepare_traversal(bvh));
+        PT_TRY(clbvh_prepare_traversal(bvh, ctx->capturing));
         PT_CHECK(bvh->ctx->device == ctx->device, PTGPU_ERR_INVALID, "BVH built on device %d, render on device %d",
                  bvh->ctx->device, ctx->device);
         if (bvh->ctx != ctx && bvh->ctx->stream != ctx->stream) {

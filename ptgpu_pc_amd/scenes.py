@@ -113,6 +113,29 @@ def soup(n: int = 65536, seed: int = 1, width: int = 512, height: int = 512):
     return shapes, 2, cam
 
 
+
+def morton_chain(n: int = 6000, seed: int = 5, width: int = 80, height: int = 56):
+    """n small triangles on 31 points whose quantised centroids are 0 and each power of two on one axis
+    (10 per axis), plus the soup's light quad: Morton codes 0 and 2^0 .. 2^29, so the Morton-keyed LBVH
+    is a chain about 30 levels deep with small balanced subtrees at its links (height 39 at n = 6000,
+    seed 5, against 17 for a uniform soup of the same size)."""
+    rng = np.random.default_rng(seed)
+    pts = [(0.0, 0.0, 0.0)]
+    for i in range(10):
+        v = (2.0 ** i + 0.5) / 1024.0
+        pts += [(v, 0.0, 0.0), (0.0, v, 0.0), (0.0, 0.0, v)]
+    pts = np.array(pts) * np.array([1.0, 1.25, 1.0])   # the light's centroids set the y extent to 1.25
+    cen = pts[rng.integers(0, len(pts), size=(n,))].astype(F32)
+    cen[:8] = (1.0, 1.25, 1.0)                          # and these the x and z extents to 1
+    off = rng.uniform(-1e-5, 1e-5, size=(n, 3, 3)).astype(F32)
+    body = triangles_to_shapes((cen[:, None, :] + off).astype(F32))
+    quad = np.array([[(0.25, 1.25, 0.25), (0.75, 1.25, 0.25), (0.75, 1.25, 0.75)],
+                     [(0.25, 1.25, 0.25), (0.75, 1.25, 0.75), (0.25, 1.25, 0.75)]], dtype=F32)
+    light = triangles_to_shapes(quad, e=(12.0, 12.0, 12.0), c=(0.0, 0.0, 0.0))
+    cam = make_camera((0.5, 0.5, 2.5), (0.5, 0.5, 0.0), 0.0, 0.0, width, height)
+    return concat_shapes([body, light]), 2, cam
+
+
 # ---------------------------------------------------------------------------------------------
 @lru_cache(maxsize=4)
 def _icosphere(level: int):

@@ -4,15 +4,19 @@ A recorded frame (camera upload from pinned memory, seed reset, colour clear, on
 pixel readback into pinned memory) is replayed several times with different cameras written between
 replays, with eager work and synchronisations in between; every replay must equal the eager render
 of the same frame bit for bit.  Calls that wait for the device must refuse to be recorded.
+
+An animated scene's frame -- new shapes, LBVH rebuild, treelet pass, render -- records too: the render
+sizes its stack from the height the handle last read back and checks the built tree's height on the
+device, so a replay on a deeper tree renders nothing and reports the error at the next synchronisation.
 """
 import numpy as np
 import pytest
 
 import ptgpu_pc_amd as pt
 from ptgpu_pc_amd import scenes
-from ptgpu_pc_amd.layout import CAMERA_DTYPE, default_seeds
+from ptgpu_pc_amd.layout import CAMERA_DTYPE, SHAPE_DTYPE, default_seeds
 
-from test_gpu_parity import gpu_lbvh, render_gpu, say
+from test_gpu_parity import gpu_lbvh, render_gpu, same, say
 
 pytestmark = pytest.mark.gpu
 F32 = np.float32
@@ -71,23 +75,141 @@ def test_recorded_frame_replays_equal_eager(ctx):
     bvh.close()
 
 
-def test_recording_refuses_unprepared_traversal(ctx):
-    shapes, lc, cam = scenes.soup(500, seed=4, width=16, height=16)
-    pt.update_camera(cam)
-    s, bvh = gpu_lbvh(ctx, shapes)      # built, never traversed: the height read-back is still pending
-    d_sh, d_cam = ctx.to_device(s), ctx.to_device(cam)
-    d_col, d_seeds = ctx.alloc(16 * 16 * 12, F32), ctx.to_device(default_seeds(16, 16))
+def _frame_buffers(ctx, w, h):
+    return dict(d_cam=ctx.alloc(CAMERA_DTYPE.itemsize, CAMERA_DTYPE), d_seeds0=ctx.to_device(default_seeds(w, h)),
+                d_seeds=ctx.alloc(w * h * 8, np.uint32), d_col=ctx.alloc(w * h * 12, F32),
+                d_pix=ctx.alloc(w * h * 4, np.uint32))
+
+
+def _record_rebuild_frame(ctx, bvh, d_sh, h_sh, n, lc, w, h, spp, treelet, B, h_cam, h_pix):
+    """Records one animated-scene frame: new shapes from pinned memory, LBVH rebuild (+ treelet pass),
+    camera upload, seed reset, colour clear, render, pixel readback."""
     ctx.graph_begin()
     try:
-        with pytest.raises(pt.PtgpuError):
-            pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=len(s),
-                      lightCnt=lc, width=16, height=16, spp=1, max_depth=5)
-        d_col.zero()                    # something recordable, so the graph is not empty
+        ctx.memcpy_async(d_sh, h_sh, n * SHAPE_DTYPE.itemsize)
+        bvh.reset()
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        if treelet:
+            bvh.optimize(treelet)
+        ctx.memcpy_async(B["d_cam"], h_cam, CAMERA_DTYPE.itemsize)
+        ctx.memcpy_async(B["d_seeds"], B["d_seeds0"], w * h * 8)
+        B["d_col"].zero()
+        pt.render(ctx, bvh, None, colors=B["d_col"], seeds=B["d_seeds"], camera=B["d_cam"], shapes=d_sh, shapeCnt=n,
+                  lightCnt=lc, width=w, height=h, currentSample=0, spp=spp, max_depth=5, pixels=B["d_pix"])
+        ctx.memcpy_async(h_pix, B["d_pix"], w * h * 4)
     finally:
         g = ctx.graph_end()
-    g.close()
-    # eager use afterwards is unaffected
-    pt.render(ctx, bvh, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=d_sh, shapeCnt=len(s), lightCnt=lc,
-              width=16, height=16, spp=1, max_depth=5)
-    ctx.synchronize()
+    return g
+
+
+def _eager(ctx, shapes, lc, cam, w, h, spp):
+    s, bvh = gpu_lbvh(ctx, shapes)
+    out = render_gpu(ctx, s, lc, cam, w, h, spp, 5, None, bvh=bvh)[:3]
     bvh.close()
+    return out
+
+
+@pytest.mark.parametrize("treelet", [0, 5])
+def test_recorded_rebuild_frames_replay_equal_eager(ctx, treelet):
+    """The per-frame sequence of an animated scene -- new shapes, LBVH rebuild, optional treelet pass,
+    render -- recorded once, replayed with other shapes and cameras written between replays: every
+    replay equals an eager render of its shapes on a freshly built tree, bit for bit."""
+    w, h, spp, n = 80, 56, 2, 6002
+    say(f"case graph rebuild replay treelet {treelet}")
+    frames = []
+    for seed, yaw in ((23, 0.3), (31, 1.1), (37, 2.0), (23, 0.3)):
+        sh, lc, _ = scenes.soup(n - 2, seed=seed, width=w, height=h)
+        cam = scenes.orbit_camera(yaw, w, h)
+        pt.update_camera(cam)
+        frames.append((sh, cam))
+    d_sh = ctx.to_device(frames[0][0])
+    bvh = pt.CLBVH(ctx, d_sh, n)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    if treelet:
+        bvh.optimize(treelet)
+    render_gpu(ctx, frames[0][0], lc, frames[0][1], w, h, 1, 5, None, bvh=bvh, d_shapes=d_sh)   # heights known
+    B = _frame_buffers(ctx, w, h)
+    h_sh = ctx.pinned(n * SHAPE_DTYPE.itemsize, SHAPE_DTYPE)
+    h_cam = ctx.pinned(CAMERA_DTYPE.itemsize, CAMERA_DTYPE)
+    h_pix = ctx.pinned(w * h * 4, np.uint32)
+    g = _record_rebuild_frame(ctx, bvh, d_sh, h_sh, n, lc, w, h, spp, treelet, B, h_cam, h_pix)
+    outs = []
+    for k, (sh, cam) in enumerate(frames):
+        h_sh.array[:] = sh
+        h_cam.array[:] = cam
+        g.launch()
+        ctx.synchronize()
+        got = h_pix.array.copy(), B["d_col"].download(F32), B["d_seeds"].download(np.uint32)
+        c_e, s_e, p_e = _eager(ctx, sh, lc, cam, w, h, spp)
+        say(f"replay {k}: pixels equal {same(got[0], p_e)}")
+        assert same(got[0], p_e) and same(got[1], c_e) and same(got[2], s_e), k
+        outs.append(got[0])
+    assert same(outs[0], outs[3]) and not same(outs[0], outs[1])
+    # the handle stays usable eagerly after the replays (its height is read back then)
+    c_e, _, p_e = _eager(ctx, frames[3][0], lc, frames[3][1], w, h, spp)
+    c_h, _, p_h = render_gpu(ctx, frames[3][0], lc, frames[3][1], w, h, spp, 5, None, bvh=bvh, d_shapes=d_sh)[:3]
+    assert same(p_h, p_e) and same(c_h, c_e)
+    for x in (g, h_sh, h_cam, h_pix, bvh):
+        x.close()
+
+
+def test_recorded_render_checks_the_built_height(ctx):
+    """A frame recorded after a rebuild sizes its traversal stack from the height the handle last read
+    back (plus a margin).  Replayed on shapes whose tree is deeper than that stack holds, the render
+    checks the built height on the device, renders nothing, and the next ptgpu_synchronize reports it.
+    Recorded again once the deep tree's height is known, the frame replays correctly."""
+    w, h, spp, n = 80, 56, 2, 6002
+    soup, lc, cam = scenes.soup(n - 2, seed=23, width=w, height=h)
+    chain, lc2, _ = scenes.morton_chain(n - 2, seed=5, width=w, height=h)
+    assert lc2 == lc
+    pt.update_camera(cam)
+    say("case graph height check")
+    d_sh = ctx.to_device(soup)
+    bvh = pt.CLBVH(ctx, d_sh, n)
+    bvh.buildRadixTree()
+    bvh.buildBVHTree()
+    render_gpu(ctx, soup, lc, cam, w, h, 1, 5, None, bvh=bvh, d_shapes=d_sh)
+    h_soup = bvh.export_traversal()[2]
+    probe = pt.CLBVH(ctx, ctx.to_device(chain), n)
+    probe.buildRadixTree()
+    probe.buildBVHTree()
+    h_chain = probe.export_traversal()[2]
+    probe.close()
+    say(f"heights: soup {h_soup}, chain {h_chain}")
+    # the stack recorded for the soup: four-wide, sized for soup height + 4
+    need = 3 * ((h_soup + 4 + 2) // 2) + 1
+    stack = 32 if need <= 32 else 48 if need <= 48 else 64
+    assert h_chain > 2 * ((stack - 1) // 3) - 1, (h_soup, h_chain, stack)
+
+    B = _frame_buffers(ctx, w, h)
+    h_sh = ctx.pinned(n * SHAPE_DTYPE.itemsize, SHAPE_DTYPE)
+    h_cam = ctx.pinned(CAMERA_DTYPE.itemsize, CAMERA_DTYPE)
+    h_pix = ctx.pinned(w * h * 4, np.uint32)
+    h_cam.array[:] = cam
+    g = _record_rebuild_frame(ctx, bvh, d_sh, h_sh, n, lc, w, h, spp, 0, B, h_cam, h_pix)
+    h_sh.array[:] = soup
+    g.launch()
+    ctx.synchronize()
+    assert same(h_pix.array, _eager(ctx, soup, lc, cam, w, h, spp)[2])
+    # the deep tree: nothing rendered, the error reported once
+    B["d_pix"].upload(np.full(w * h, 0xDEADBEEF, np.uint32))
+    h_sh.array[:] = chain
+    g.launch()
+    with pytest.raises(pt.PtgpuError, match="deeper than the traversal stack"):
+        ctx.synchronize()
+    ctx.synchronize()
+    assert (B["d_pix"].download(np.uint32) == 0xDEADBEEF).all()
+    g.close()
+    # eager use reads the deep tree's height back and renders it
+    want = _eager(ctx, chain, lc, cam, w, h, spp)
+    got = render_gpu(ctx, chain, lc, cam, w, h, spp, 5, None, bvh=bvh, d_shapes=d_sh)[:3]
+    assert all(same(a, b) for a, b in zip(got, want))
+    # recorded again, the frame holds the deep tree
+    g = _record_rebuild_frame(ctx, bvh, d_sh, h_sh, n, lc, w, h, spp, 0, B, h_cam, h_pix)
+    g.launch()
+    ctx.synchronize()
+    assert same(h_pix.array, want[2]) and same(B["d_col"].download(F32), want[0])
+    for x in (g, h_sh, h_cam, h_pix, bvh):
+        x.close()
