@@ -630,6 +630,26 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: boo
 
     t_plain, rays_plain, _, _ = timed(plain_step)
     t_batch, rays_batch, b_ms, r_ms = timed(batch_step, events=True)
+    # the batched step recorded once (batched rebuild + batched render, nothing read back) and replayed
+    ctx.graph_begin()
+    try:
+        pt.CLBVH.build_batch(ctx, batch_h)
+        pt.render_batch(ctx, [job(k, h) for k, h in enumerate(batch_h)])
+    finally:
+        g = ctx.graph_end()
+    reseed()
+    g.launch()
+    ctx.synchronize()
+    same_rec = all(a.tobytes() == f["pixels"].download(np.uint32).tobytes() for a, f in zip(pix_plain, frames))
+    for _ in range(warmup):
+        g.launch()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.launch()
+    ctx.synchronize()
+    t_rec = (time.perf_counter() - t0) / steps
+    g.close()
     # the plain path's builds alone
     ctx.synchronize()
     t0 = time.perf_counter()
@@ -652,6 +672,8 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: boo
                         "mrays_s": round(rays_batch / t_batch / 1e6, 1), "build_ms": round(b_ms, 2),
                         "render_ms": round(r_ms, 2), "build_mtri_s": round(total_n / b_ms / 1e3, 1),
                         "build_alg_gbs": round(alg / (b_ms * 1e-3) / 1e9, 1)},
+            "batched_recorded": {"frames_s": round(nfr / t_rec, 2), "ms_per_step": round(t_rec * 1e3, 2),
+                                 "pixels_identical": same_rec},
             "frames_speedup": round(t_plain / t_batch, 2), "build_speedup": round(plain_build_ms / b_ms, 2),
             "pixels_identical": same, "tuning": tuning,
             "workload": f"{nfr} scenes, triangles log-uniform [1e3, 1e6] (soup or displaced icosphere by seed), "

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/ptgpu.h"
 
@@ -155,6 +156,17 @@ struct BatchChannel {
 }  // namespace ptgpu
 
 namespace ptgpu {
+// A table uploaded by a call that is being recorded into a graph: a pinned host copy and a device
+// buffer owned by the recording (then by its graph), so that every replay copies the table as it was
+// recorded whatever eager calls do with the reusable staging buffers in between.
+struct RecordedStage {
+    void *host = nullptr;
+    void *dev = nullptr;
+};
+void release_stages(std::vector<RecordedStage> &v);
+}  // namespace ptgpu
+
+namespace ptgpu {
 // Tables of a batched LBVH build (per-scene buffers, block offsets, clears) and their pinned
 // upload copy; reused by the next batch once the previous one has been waited for.
 struct BuildStaging {
@@ -188,7 +200,18 @@ struct ptgpu_ctx {
     ptgpu::BatchChannel batch;            // ptgpu_render_batch launches (on the context's stream)
     ptgpu::BuildStaging build_staging;    // batched LBVH builds
     bool capturing = false;               // between ptgpu_graph_begin and ptgpu_graph_end
+    std::vector<ptgpu::RecordedStage> rec_stages;   // tables staged by the calls being recorded
 };
+
+namespace ptgpu {
+// Staging for a call being recorded on `stream`, owned by the recording (then by its graph):
+// record_buffer allocates device memory; record_copy keeps a pinned copy of `src` and records its
+// upload into d_dst, so every replay copies what was recorded; record_stage does both.
+int record_buffer(ptgpu_ctx *ctx, size_t bytes, void **d_out);
+int record_copy(ptgpu_ctx *ctx, hipStream_t stream, void *d_dst, const void *src, size_t bytes);
+int record_stage(ptgpu_ctx *ctx, hipStream_t stream, const void *src, size_t copy_bytes, size_t alloc_bytes,
+                 void **d_out);
+}  // namespace ptgpu
 
 namespace ptgpu {
 // light list: indices of the first `light_cnt` shapes with non-zero emission, in index order

@@ -139,6 +139,7 @@ int ptgpu_ctx_destroy(ptgpu_ctx *ctx) {
     ctx->tmp_parents.release();
     ctx->tmp_tiles.release();
     ctx->rec_status.release();
+    ptgpu::release_stages(ctx->rec_stages);
     ctx->batch.release();
     ctx->build_staging.release();
     wf_release(ctx);

@@ -278,27 +278,44 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
     const size_t o_out = o_ctr + 64;
     const size_t dev_bytes = o_out + (size_t)2 * ns * sizeof(unsigned);
     BuildStaging &B = ctx->build_staging;
-    PT_TRY(B.table.ensure(dev_bytes));
-    if (B.pinned_bytes < dev_bytes) {
-        if (B.pinned) PT_HIP(hipHostFree(B.pinned));
-        B.pinned = nullptr;
-        B.pinned_bytes = 0;
-        PT_HIP(hipHostMalloc(&B.pinned, dev_bytes * 2, hipHostMallocDefault));
-        B.pinned_bytes = dev_bytes * 2;
+    const bool recording = ctx->capturing;
+    std::vector<char> img;   // recorded: the tables' image, staged by record_stage
+    char *dev = nullptr, *hp = nullptr;
+    if (recording) {
+        img.assign(upload, 0);
+        hp = img.data();
+    } else {
+        PT_TRY(B.table.ensure(dev_bytes));
+        if (B.pinned_bytes < dev_bytes) {
+            if (B.pinned) PT_HIP(hipHostFree(B.pinned));
+            B.pinned = nullptr;
+            B.pinned_bytes = 0;
+            PT_HIP(hipHostMalloc(&B.pinned, dev_bytes * 2, hipHostMallocDefault));
+            B.pinned_bytes = dev_bytes * 2;
+        }
+        dev = B.table.as<char>();
+        hp = static_cast<char *>(B.pinned);
     }
-    char *dev = B.table.as<char>();
+    if (recording) {   // a device buffer of the recording's own; its image is uploaded below
+        void *d = nullptr;
+        PT_TRY(record_buffer(ctx, dev_bytes, &d));
+        dev = static_cast<char *>(d);
+    }
     unsigned *d_ctr = reinterpret_cast<unsigned *>(dev + o_ctr);
     unsigned *d_out = reinterpret_cast<unsigned *>(dev + o_out);
     fills.push_back({d_ctr, (unsigned long long)C::kPasses, 0u, 0u});   // the batch's tile counters
     std::vector<int> fstarts((size_t)nf + 1, 0);
     for (int e = 0; e < nf; e++)
         fstarts[e + 1] = fstarts[e] + (int)((fills[e].words + kFillWords - 1) / kFillWords);
-    char *hp = static_cast<char *>(B.pinned);
     std::memcpy(hp, segs.data(), (size_t)ns * sizeof(BuildSeg));
     std::memcpy(hp + o_starts, starts.data(), starts.size() * sizeof(int));
     std::memcpy(hp + o_fills, fills.data(), (size_t)nf * sizeof(FillSeg));
     std::memcpy(hp + o_fstarts, fstarts.data(), fstarts.size() * sizeof(int));
-    PT_HIP(hipMemcpyAsync(dev, hp, upload, hipMemcpyHostToDevice, st));
+    if (recording) {
+        PT_TRY(record_copy(ctx, st, dev, hp, upload));
+    } else {
+        PT_HIP(hipMemcpyAsync(dev, hp, upload, hipMemcpyHostToDevice, st));
+    }
     const BuildSeg *d_segs = reinterpret_cast<const BuildSeg *>(dev);
     const int *d_starts = reinterpret_cast<const int *>(dev + o_starts);
 
@@ -335,6 +352,17 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
     if (total(kStMChunk) > 0) {
         hipLaunchKernelGGL(kb_mrefit_top, dim3(ns), dim3(1024), 0, st, d_segs);
         PT_HIP(hipGetLastError());
+    }
+    if (recording) {
+        // nothing is read back: each handle's height stays pending, and a recorded traversal checks
+        // it (with the build's error word, both in the handle's counters) on the device
+        for (int s = 0; s < ns; s++) {
+            ptgpu_clbvh *h = hs[s];
+            h->sorted = h->valsA.as<unsigned>();
+            for (bool &t : h->timed) t = false;
+            clbvh_mark_built(h);
+        }
+        return PTGPU_OK;
     }
     hipLaunchKernelGGL(kb_gather, dim3(1), dim3(256), 0, st, d_segs, ns, d_out);
     PT_HIP(hipGetLastError());
@@ -379,7 +407,6 @@ int ptgpu_clbvh_build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *handles, int cou
     PT_CHECK(ctx && (handles || count == 0), PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(count >= 0, PTGPU_ERR_INVALID, "negative scene count %d", count);
     if (count == 0) return PTGPU_OK;
-    PT_NO_CAPTURE(ctx, "ptgpu_clbvh_build_batch");
     std::unordered_set<const ptgpu_clbvh *> seen;
     const int bits = handles[0] ? handles[0]->digit_bits : 8;
     for (int s = 0; s < count; s++) {

@@ -79,6 +79,28 @@ __device__ __forceinline__ bool recorded_tree_unfit(const RenderParams &P) {
     return true;
 }
 
+// The same check over a batch's jobs (P.status is set when some job's tree was recorded right after
+// its build): the launch renders nothing if any job's tree does not fit.  Block-uniform: every block
+// reads the same words and decides after one barrier.
+__device__ bool recorded_jobs_unfit(const RenderParams &P) {
+    __shared__ int s_unfit;
+    if (threadIdx.x == 0) s_unfit = 0;
+    __syncthreads();
+    for (int j = threadIdx.x; j < P.njobs; j += blockDim.x) {
+        const unsigned *hc = P.jobs[j].acc.height_check;
+        if (!hc) continue;
+        const unsigned err = hc[0], h = hc[2];
+        if (err == 0u && h <= (unsigned)P.depth_cap) continue;
+        s_unfit = 1;
+        if (blockIdx.x == 0) {
+            atomicOr(P.status, err ? 1u : 2u);
+            atomicMax(P.status + 1, h);
+        }
+    }
+    __syncthreads();
+    return s_unfit != 0;
+}
+
 // The frame pixel of claimed work item m: pixel m of the frame, or, in a tiled launch, pixel
 // m % (tile_w * tile_h) of tile m / (tile_w * tile_h) in row-major order within the tile; -1 when a
 // partial tile at the frame's right or bottom edge has no pixel there.
@@ -250,6 +272,7 @@ enum : int {
 template <int STACK, bool STATS, bool WIDE, bool MULTI, bool HALF>
 __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     if (!MULTI && recorded_tree_unfit(P)) return;
+    if (MULTI && P.status && recorded_jobs_unfit(P)) return;
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
     const unsigned long long lt = lanemask_lt();
@@ -1878,7 +1901,6 @@ int ptgpu_render_tiles(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const
 int ptgpu_render_batch(ptgpu_ctx *ctx, const ptgpu_render_job *jobs, int njobs, unsigned long long *d_ray_counter) {
     PT_CHECK(ctx && (jobs || njobs == 0) && njobs >= 0, PTGPU_ERR_INVALID, "null argument");
     if (njobs == 0) return PTGPU_OK;
-    PT_NO_CAPTURE(ctx, "ptgpu_render_batch (its job table is staged through reused pinned memory)");
     PT_HIP(hipSetDevice(ctx->device));
     ctx->batch.stream = ctx->stream;
     return launch_batch(ctx, ctx->batch, jobs, njobs, d_ray_counter);
@@ -1893,6 +1915,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
     std::vector<unsigned> begin((size_t)njobs + 1, 0u);
     int st = 0;
     bool half = ctx->wide == 2;   // binary16 nodes only when every job's tree has them
+    bool checks = false;          // some job's tree height is checked on the device (recorded after a build)
     unsigned long long total = 0;
     for (int j = 0; j < njobs; j++) {
         const ptgpu_render_job &J = jobs[j];
@@ -1917,6 +1940,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         PT_CHECK(sw > 0, PTGPU_ERR_UNSUPPORTED, "job %d: tree height %d exceeds the traversal stack", j, R.acc.depth);
         st = sw > st ? sw : st;
         half = half && R.acc.nodes4h;
+        checks = checks || R.acc.height_check;
         if (J.bvh) {
             if (J.bvh->lights_for != (int)a->lightCnt) {
                 PT_TRY(build_light_list(ctx, a->shapes, (int)a->shapeCnt, (int)a->lightCnt, J.bvh->lights,
@@ -1950,24 +1974,34 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         PT_HIP(hipEventRecord(ch.ready, ctx->stream));
         PT_HIP(hipStreamWaitEvent(stream, ch.ready, 0));
     }
-    // stage the table through pinned memory; the channel's previous copy must be done before reuse
     const size_t tb = table.size() * sizeof(RenderJob), bb = begin.size() * sizeof(unsigned);
-    const size_t bytes = tb + bb + 64;
-    PT_HIP(hipEventSynchronize(ch.copied));
-    if (ch.pinned_bytes < bytes) {
-        if (ch.pinned) PT_HIP(hipHostFree(ch.pinned));
-        ch.pinned = nullptr;
-        ch.pinned_bytes = 0;
-        PT_HIP(hipHostMalloc(&ch.pinned, bytes * 2, hipHostMallocDefault));
-        ch.pinned_bytes = bytes * 2;
-    }
-    char *hp = static_cast<char *>(ch.pinned);
-    std::memcpy(hp, table.data(), tb);
     const size_t boff = (tb + 63) & ~size_t(63);
-    std::memcpy(hp + boff, begin.data(), bb);
-    PT_TRY(ch.table.ensure(boff + bb));
-    PT_HIP(hipMemcpyAsync(ch.table.p, hp, boff + bb, hipMemcpyHostToDevice, stream));
-    PT_HIP(hipEventRecord(ch.copied, stream));
+    void *d_table = nullptr;
+    if (ctx->capturing) {
+        // recorded: the table gets a pinned copy and a device buffer of its own, kept with the graph
+        std::vector<char> img(boff + bb, 0);
+        std::memcpy(img.data(), table.data(), tb);
+        std::memcpy(img.data() + boff, begin.data(), bb);
+        PT_TRY(record_stage(ctx, stream, img.data(), boff + bb, boff + bb, &d_table));
+    } else {
+        // stage the table through pinned memory; the channel's previous copy must be done before reuse
+        const size_t bytes = tb + bb + 64;
+        PT_HIP(hipEventSynchronize(ch.copied));
+        if (ch.pinned_bytes < bytes) {
+            if (ch.pinned) PT_HIP(hipHostFree(ch.pinned));
+            ch.pinned = nullptr;
+            ch.pinned_bytes = 0;
+            PT_HIP(hipHostMalloc(&ch.pinned, bytes * 2, hipHostMallocDefault));
+            ch.pinned_bytes = bytes * 2;
+        }
+        char *hp = static_cast<char *>(ch.pinned);
+        std::memcpy(hp, table.data(), tb);
+        std::memcpy(hp + boff, begin.data(), bb);
+        PT_TRY(ch.table.ensure(boff + bb));
+        PT_HIP(hipMemcpyAsync(ch.table.p, hp, boff + bb, hipMemcpyHostToDevice, stream));
+        PT_HIP(hipEventRecord(ch.copied, stream));
+        d_table = ch.table.p;
+    }
     PT_TRY(ch.counter.ensure(64));
     PT_TRY(fill_u32(stream, ch.counter.p, 0u, 1));
     RenderParams P;
@@ -1979,9 +2013,11 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
     P.total = (int)total;
     P.regroup = ctx->batch_regroup;
     P.quad = ctx->quad;
-    P.jobs = ch.table.as<RenderJob>();
-    P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(ch.table.p) + boff);
+    P.jobs = static_cast<const RenderJob *>(d_table);
+    P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(d_table) + boff);
     P.njobs = njobs;
+    P.depth_cap = wide_depth_cap(st);
+    P.status = checks ? ctx->rec_status.as<unsigned>() : nullptr;
     long long blocks = (long long)ctx->cu_count * ctx->batch_blocks_per_cu;
     const long long need = ((long long)total + 255) / 256;
     if (blocks > need) blocks = need;

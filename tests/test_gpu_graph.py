@@ -213,3 +213,75 @@ def test_recorded_render_checks_the_built_height(ctx):
     assert same(h_pix.array, want[2]) and same(B["d_col"].download(F32), want[0])
     for x in (g, h_sh, h_cam, h_pix, bvh):
         x.close()
+
+
+def test_recorded_batch_rebuild_frames_replay_equal_eager(ctx):
+    """A C4-style step -- several scenes' new shapes, one batched LBVH build, one batched render of
+    all their frames -- recorded once and replayed with other shapes written between replays: every
+    job equals an eager render of its shapes on a freshly built tree.  A replay in which one scene's
+    tree is deeper than the recorded stack renders no job and reports the error."""
+    spec = [(3000, 48, 32, 2), (6000, 64, 40, 1), (9000, 40, 40, 3)]   # triangles, width, height, spp
+    say("case graph batch rebuild replay")
+    scenes_k = [[scenes.soup(n, seed=seed + 10 * k, width=w, height=h)[0] for k in range(3)]
+                for seed, (n, w, h, _) in zip((3, 5, 7), spec)]
+    lc = 2
+    cams = []
+    for i, (_, w, h, _) in enumerate(spec):
+        c = scenes.orbit_camera(0.4 + i, w, h)
+        pt.update_camera(c)
+        cams.append(c)
+    d_sh = [ctx.to_device(v[0]) for v in scenes_k]
+    hs = pt.CLBVH.create_batch(ctx, d_sh)
+    h_sh = [ctx.pinned(len(v[0]) * SHAPE_DTYPE.itemsize, SHAPE_DTYPE) for v in scenes_k]
+    jobs, outs = [], []
+    for i, (n, w, h, spp) in enumerate(spec):
+        B = _frame_buffers(ctx, w, h)
+        B["d_cam"].upload(cams[i])
+        jobs.append(dict(bvh=hs[i], shapes=d_sh[i], shapeCnt=n + 2, lightCnt=lc, width=w, height=h, spp=spp,
+                         currentSample=0, max_depth=5, camera=B["d_cam"], colors=B["d_col"], seeds=B["d_seeds"],
+                         pixels=B["d_pix"], B=B))
+        outs.append(ctx.pinned(w * h * 4, np.uint32))
+
+    def step():
+        for i, j in enumerate(jobs):
+            ctx.memcpy_async(d_sh[i], h_sh[i], (spec[i][0] + 2) * SHAPE_DTYPE.itemsize)
+        pt.CLBVH.build_batch(ctx, hs)
+        for j in jobs:
+            ctx.memcpy_async(j["seeds"], j["B"]["d_seeds0"], j["width"] * j["height"] * 8)
+            j["colors"].zero()
+        pt.render_batch(ctx, [{k: v for k, v in j.items() if k != "B"} for j in jobs])
+        for j, o in zip(jobs, outs):
+            ctx.memcpy_async(o, j["pixels"], j["width"] * j["height"] * 4)
+
+    for i in range(3):
+        h_sh[i].array[:] = scenes_k[i][0]
+    step()                                          # eager: heights read back, light lists built
+    ctx.synchronize()
+    ctx.graph_begin()
+    try:
+        step()
+    finally:
+        g = ctx.graph_end()
+    for k in (1, 2, 0):
+        for i in range(3):
+            h_sh[i].array[:] = scenes_k[i][k]
+        g.launch()
+        ctx.synchronize()
+        for i, (n, w, h, spp) in enumerate(spec):
+            want = _eager(ctx, scenes_k[i][k], lc, cams[i], w, h, spp)
+            assert same(outs[i].array, want[2]), (k, i)
+            assert same(jobs[i]["colors"].download(F32), want[0]), (k, i)
+            assert same(jobs[i]["seeds"].download(np.uint32), want[1]), (k, i)
+    # one scene replaced by a deep Morton chain of the same size: no job is rendered
+    chain, _, _ = scenes.morton_chain(spec[1][0], seed=5, width=spec[1][1], height=spec[1][2])
+    h_sh[1].array[:] = chain
+    for j in jobs:
+        j["pixels"].upload(np.full(j["width"] * j["height"], 0xDEADBEEF, np.uint32))
+    g.launch()
+    with pytest.raises(pt.PtgpuError, match="deeper than the traversal stack"):
+        ctx.synchronize()
+    for j in jobs:
+        assert (j["pixels"].download(np.uint32) == 0xDEADBEEF).all()
+    g.close()
+    for x in (*h_sh, *outs, *hs):
+        x.close()
