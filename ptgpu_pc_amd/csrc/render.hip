@@ -64,6 +64,22 @@ struct RenderParams {
     unsigned *status;             // the context's check words: [0] flags, [1] deepest height seen
 };
 
+// A pointer read from device memory (a batch's job table) or moved between lanes is a generic
+// pointer to the compiler, and loads through it become flat loads, which count against both memory
+// wait counters and make every wait cover all outstanding LDS and scalar accesses too.  Every buffer
+// the render kernels read or write lives in device (global) memory, so the per-lane pointers of
+// the render loop are held as global-address-space pointers (gmem): loads and stores through them
+// are global ones, also after a generic-pointer helper is inlined on from_gmem's result.  to_gmem
+// goes through the address bits: a cast would fold with from_gmem's back into the generic pointer.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+template <class T>
+__device__ __forceinline__ gmem<T> *to_gmem(T *p) {
+    return reinterpret_cast<gmem<T> *>(reinterpret_cast<unsigned long long>(p));
+}
+template <class T>
+__device__ __forceinline__ T *from_gmem(gmem<T> *p) { return (T *)p; }
+
 // A launch recorded right after a build sized its stack for a height bound (acc.height_check set).
 // The built tree is checked before any work: a build error (flag 1) or a tree deeper than the stack
 // holds (flag 2) is recorded in the context's status words and the launch renders nothing.  Every
@@ -87,7 +103,7 @@ __device__ bool recorded_jobs_unfit(const RenderParams &P) {
     if (threadIdx.x == 0) s_unfit = 0;
     __syncthreads();
     for (int j = threadIdx.x; j < P.njobs; j += blockDim.x) {
-        const unsigned *hc = P.jobs[j].acc.height_check;
+        const gmem<const unsigned> *hc = to_gmem(P.jobs[j].acc.height_check);
         if (!hc) continue;
         const unsigned err = hc[0], h = hc[2];
         if (err == 0u && h <= (unsigned)P.depth_cap) continue;
@@ -307,23 +323,23 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         __syncthreads();
     }
     // frame fields: the launch's own, or (batch) the lane's job's
-#define JF(field) (MULTI ? P.jobs[jb].field : P.field)
+#define JF(field) to_gmem(MULTI ? P.jobs[jb].field : P.field)
     int jb = 0;
     // batch: the lane's job's fields that every shading pass reads, kept in registers from the pixel
     // claim on (a field read through the job table is one more dependent load)
-    const Shape *j_shapes = P.shapes;
-    const int *j_lights = P.lights;
+    const gmem<const Shape> *j_shapes = to_gmem(P.shapes);
+    const gmem<const int> *j_lights = to_gmem(P.lights);
     int j_nlights = P.nlights, j_max_depth = P.max_depth;
     // and those every new sample reads
-    const Camera *j_camera = P.camera;
+    const gmem<const Camera> *j_camera = to_gmem(P.camera);
     int j_width = P.width, j_height = P.height, j_spp = P.spp, j_currentSample = P.currentSample;
-#define JSHAPES (MULTI ? j_shapes : P.shapes)
+#define JSHAPES (MULTI ? from_gmem(j_shapes) : P.shapes)
 #define JFR(field) (MULTI ? j_##field : P.field)
     // the tree of the lane's current query (uniform for a single frame)
-    const TNode *t_nodes = P.acc.nodes;
-    const TNode4 *t_nodes4 = P.acc.nodes4;
-    const TNode4H *t_nodes4h = P.acc.nodes4h;
-    const Prim *t_prims = P.acc.prims;
+    const gmem<const TNode> *t_nodes = to_gmem(P.acc.nodes);
+    const gmem<const TNode4> *t_nodes4 = to_gmem(P.acc.nodes4);
+    const gmem<const TNode4H> *t_nodes4h = to_gmem(P.acc.nodes4h);
+    const gmem<const Prim> *t_prims = to_gmem(P.acc.prims);
     int t_nnodes = P.acc.nnodes, t_nprims = P.acc.nprims, t_nshapes = P.acc.nshapes, t_root = P.acc.root;
 
     int st = kNeed;
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         if (first < 0 || cnt > t_nprims - first) return false;   // malformed caller tree
         for (int q = 0; q < cnt; q++) {
             int id;
-            const float t = prim_isect(t_prims, first + q, ro, rd, &id);
+            const float t = prim_isect(from_gmem(t_prims), first + q, ro, rd, &id);
             if (STATS) st_prims++;
             if (qany) {
                 if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; return true; }
@@ -362,7 +378,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         for (int q = first; q < t_nprims; q++) {
             int id;
             bool last;
-            const float t = prim_isect_run(t_prims, q, ro, rd, &id, &last);
+            const float t = prim_isect_run(from_gmem(t_prims), q, ro, rd, &id, &last);
             if (STATS) st_prims++;
             if (qany) {
                 if (t != 0.f && t < tb && id != t_nshapes - 1) { bid = 1; return true; }
@@ -427,8 +443,8 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 int q_pl = __shfl(pl, owner);
                 unsigned q_nv = (unsigned)__shfl((int)nv, owner);
                 const bool q_any = __shfl((int)qany, owner) != 0;
-                const TNode4H *q_nodes = t_nodes4h;
-                const Prim *q_prims = t_prims;
+                const gmem<const TNode4H> *q_nodes = t_nodes4h;
+                const gmem<const Prim> *q_prims = t_prims;
                 int q_nnodes = t_nnodes, q_nprims = t_nprims, q_nshapes = t_nshapes;
                 if (MULTI) {
                     auto shp = [&](const void *ptr) -> const void * {
@@ -437,8 +453,8 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         const unsigned hi = (unsigned)__shfl((int)(unsigned)(v >> 32), owner);
                         return reinterpret_cast<const void *>(((unsigned long long)hi << 32) | lo);
                     };
-                    q_nodes = static_cast<const TNode4H *>(shp(t_nodes4h));
-                    q_prims = static_cast<const Prim *>(shp(t_prims));
+                    q_nodes = to_gmem(static_cast<const TNode4H *>(shp(from_gmem(t_nodes4h))));
+                    q_prims = to_gmem(static_cast<const Prim *>(shp(from_gmem(t_prims))));
                     q_nnodes = __shfl(t_nnodes, owner);
                     q_nprims = __shfl(t_nprims, owner);
                     q_nshapes = __shfl(t_nshapes, owner);
@@ -461,10 +477,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                             // lane issues both loads; the others read primitive 0 and ignore it), so its
                             // test overlaps the child's latency.
                             const bool qhl = qc == 0 && q_pl >= 0 && q_pl < q_nprims;
-                            const Prim *qpp = q_prims + (qhl ? q_pl : 0);
+                            const Prim *qpp = from_gmem(q_prims) + (qhl ? q_pl : 0);
                             const float4 m0 = qpp->p0, m1 = qpp->p1, m2 = qpp->p2;
                             asm volatile("" ::: "memory");   // compiler-only: keeps the order above
-                            const uint4 w = q_nodes[qhn ? q_node : 0].c[qc];   // ignored with no node
+                            const uint4 w = from_gmem(q_nodes)[qhn ? q_node : 0].c[qc];   // ignored with no node
                             float bt = q_tb;
                             int bb = q_bid;
                             bool occl = false;
@@ -484,7 +500,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                 const float t = prim_isect_vals(m0, m1, m2, q_o, q_d, &id, &last);
                                 if (!take(t, id) && !last) {
                                     for (int q = q_pl + 1; q < q_nprims; q++) {
-                                        const float t2 = prim_isect_run(q_prims, q, q_o, q_d, &id, &last);
+                                        const float t2 = prim_isect_run(from_gmem(q_prims), q, q_o, q_d, &id, &last);
                                         if (take(t2, id) || last) break;
                                     }
                                 }
@@ -498,7 +514,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                 for (int q = ~ref; q < q_nprims; q++) {
                                     int id;
                                     bool last;
-                                    const float t = prim_isect_run(q_prims, q, q_o, q_d, &id, &last);
+                                    const float t = prim_isect_run(from_gmem(q_prims), q, q_o, q_d, &id, &last);
                                     if (take(t, id) || last) break;
                                 }
                             }
@@ -582,7 +598,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 if ((unsigned)node >= (unsigned)t_nnodes || ++nv > kMaxVisits) {
                     done = true;   // malformed caller tree
                 } else {
-                    const TNode *np = t_nodes + node;
+                    const TNode *np = from_gmem(t_nodes) + node;
                     const float4 a = np->a, b = np->b, c = np->c;
                     const int4 r = np->r;
                     if (STATS) st_nodes += 2;
@@ -631,12 +647,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     // primitive leaves the node's four loads in flight, so the triangle test overlaps
                     // the node's latency.
                     const bool hl = pl >= 0 && pl < t_nprims;
-                    const Prim *pp = t_prims + (hl ? pl : 0);
+                    const Prim *pp = from_gmem(t_prims) + (hl ? pl : 0);
                     float4 a0 = pp->p0, a1 = pp->p1, a2 = pp->p2;
                     // compiler-only barrier (no operands, no instruction): the primitive's loads stay
                     // here, ahead of the node's, instead of sinking into the branch that tests them
                     asm volatile("" ::: "memory");
-                    const TNode4H *np = t_nodes4h + (hn ? node : 0);
+                    const TNode4H *np = from_gmem(t_nodes4h) + (hn ? node : 0);
                     const uint4 q0 = np->c[0], q1 = np->c[1], q2 = np->c[2], q3 = np->c[3];
                     // the pending run (one primitive per LBVH leaf), before the boxes so tb is tight
                     if (hl) {
@@ -658,7 +674,8 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         // order, and a shared loop head would make this test wait for the node too
                         if (!test_prim(a0, a1, a2, pl)) {
                             for (int q = pl + 1;; q++) {
-                                const float4 b0 = t_prims[q].p0, b1 = t_prims[q].p1, b2 = t_prims[q].p2;
+                                const Prim *tq = from_gmem(t_prims) + q;
+                                const float4 b0 = tq->p0, b1 = tq->p1, b2 = tq->p2;
                                 if (test_prim(b0, b1, b2, q)) break;
                             }
                         }
@@ -739,7 +756,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     float4 lx, ly, lz, hx, hy, hz;
                     int4 rf, cn;
                     if (HALF) {
-                        const TNode4H *np = t_nodes4h + node;
+                        const TNode4H *np = from_gmem(t_nodes4h) + node;
                         const uint4 q0 = np->c[0], q1 = np->c[1], q2 = np->c[2], q3 = np->c[3];
                         rf = make_int4((int)q0.w, (int)q1.w, (int)q2.w, (int)q3.w);
                         cn = make_int4(0, 0, 0, 0);
@@ -750,7 +767,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         lz = make_float4(half_lo(q0.z), half_lo(q1.z), half_lo(q2.z), half_lo(q3.z));
                         hz = make_float4(half_hi(q0.z), half_hi(q1.z), half_hi(q2.z), half_hi(q3.z));
                     } else {
-                        const TNode4 *np = t_nodes4 + node;
+                        const TNode4 *np = from_gmem(t_nodes4) + node;
                         lx = np->lox; ly = np->loy; lz = np->loz;
                         hx = np->hix; hy = np->hiy; hz = np->hiz;
                         rf = np->ref;
@@ -834,29 +851,30 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                             }
                             first = s_job_begin[lo];
                         } else {
+                            const gmem<const unsigned> *jbg = to_gmem(P.job_begin);
                             while (lo < hi) {
                                 const int mid = (lo + hi + 1) >> 1;
-                                if (P.job_begin[mid] <= my) lo = mid; else hi = mid - 1;
+                                if (jbg[mid] <= my) lo = mid; else hi = mid - 1;
                             }
-                            first = P.job_begin[lo];
+                            first = jbg[lo];
                         }
                         jb = lo;
                         gid = (int)(my - first);
                         // the job's tree, for every query of this pixel
                         const DevAccel &A = P.jobs[jb].acc;
-                        t_nodes = A.nodes;
-                        t_nodes4 = A.nodes4;
-                        t_nodes4h = A.nodes4h;
-                        t_prims = A.prims;
+                        t_nodes = to_gmem(A.nodes);
+                        t_nodes4 = to_gmem(A.nodes4);
+                        t_nodes4h = to_gmem(A.nodes4h);
+                        t_prims = to_gmem(A.prims);
                         t_nnodes = A.nnodes;
                         t_nprims = A.nprims;
                         t_nshapes = A.nshapes;
                         t_root = A.root;
-                        j_shapes = P.jobs[jb].shapes;
-                        j_lights = P.jobs[jb].lights;
+                        j_shapes = to_gmem(P.jobs[jb].shapes);
+                        j_lights = to_gmem(P.jobs[jb].lights);
                         j_nlights = P.jobs[jb].nlights;
                         j_max_depth = P.jobs[jb].max_depth;
-                        j_camera = P.jobs[jb].camera;
+                        j_camera = to_gmem(P.jobs[jb].camera);
                         j_width = P.jobs[jb].width;
                         j_height = P.jobs[jb].height;
                         j_spp = P.jobs[jb].spp;
@@ -870,8 +888,12 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         s0 = JF(seeds)[2 * gid];
                         s1 = JF(seeds)[2 * gid + 1];
                         k = 0;
-                        if (JFR(currentSample) > 0) col = ldvec(JF(colors) + (H - y - 1) * W + x);
-                        camera_ray(MULTI ? j_camera : &s_cam, s0, s1, W, H, x, y, ro, rd);
+                        if (JFR(currentSample) > 0) {
+                            const gmem<const float> *cv =
+                                reinterpret_cast<const gmem<const float> *>(JF(colors) + (H - y - 1) * W + x);
+                            col = mk(cv[0], cv[1], cv[2]);
+                        }
+                        camera_ray(MULTI ? from_gmem(j_camera) : &s_cam, s0, s1, W, H, x, y, ro, rd);
                         rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                         st = kSeg;
                     } else {
@@ -924,7 +946,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     const bool sh = !MULTI && li < n_lds_lights
                                         ? sample_light_words(s_lights + 16 * li, s_lights[16 * li + 13], s0, s1, ro,
                                                              nl, &rd, &len, &contrib)
-                                        : sample_light(JSHAPES + (MULTI ? j_lights : P.lights)[li], s0, s1, ro, nl, &rd, &len,
+                                        : sample_light(JSHAPES + (MULTI ? from_gmem(j_lights) : P.lights)[li], s0, s1, ro, nl, &rd, &len,
                                                        &contrib);
                     if (sh) {
                         shoot = true;
@@ -949,15 +971,15 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 const int W = JFR(width), H = JFR(height);
                 const int x = gid % W, y = gid / W;
                 if (k < JFR(spp)) {
-                    camera_ray(MULTI ? j_camera : &s_cam, s0, s1, W, H, x, y, ro, rd);
+                    camera_ray(MULTI ? from_gmem(j_camera) : &s_cam, s0, s1, W, H, x, y, ro, rd);
                     rad = mk(0.f, 0.f, 0.f); thr = mk(1.f, 1.f, 1.f); spec = true; depth = 0;
                     st = kSeg;
                 } else {
-                    float *cp = reinterpret_cast<float *>(JF(colors) + (H - y - 1) * W + x);
+                    gmem<float> *cp = reinterpret_cast<gmem<float> *>(JF(colors) + (H - y - 1) * W + x);
                     cp[0] = col.x; cp[1] = col.y; cp[2] = col.z;
-                    unsigned *px = JF(pixels);
+                    gmem<unsigned> *px = JF(pixels);
                     if (px) px[y * W + x] = pack_rgba(col);
-                    unsigned *sd = JF(seeds);
+                    gmem<unsigned> *sd = JF(seeds);
                     sd[2 * gid] = s0;
                     sd[2 * gid + 1] = s1;
                     st = drained ? kIdle : kNeed;
