@@ -1034,8 +1034,12 @@ def main():
                     help="only the C2 frame rendered 1, 2 and 4 per launch (settings of --tuning FILE, else the defaults)")
     ap.add_argument("--no-frames", action="store_true", help="skip the frames-per-launch comparison in the default line")
     ap.add_argument("--no-pipeline", action="store_true",
-                    help="build and render each frame in turn on one stream (default: the next frame's tree is "
-                         "built on a second stream while the current frame renders)")
+                    help="build and render each frame in turn on one stream (the same as --frames-in-flight 1)")
+    ap.add_argument("--frames-in-flight", type=int, default=3, choices=(1, 2, 3),
+                    help="1: build and render each frame in turn on one stream; 2: the next frame's tree is built "
+                         "on a second stream while the current frame renders; 3 (default): as 2, and frames "
+                         "alternate between two render streams with their own frame buffers, so a frame's render "
+                         "starts while the previous frame's render drains")
     ap.add_argument("--tuning", default=None, help="JSON file with the tuning object of an earlier run (no autotune)")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=500.0)
@@ -1118,7 +1122,7 @@ def main():
         bvh.buildRadixTree()
         bvh.buildBVHTree()
         d_seeds.copy_from(seeds0)
-        tuning = pt.autotune(ctx, bvh, frames_per_build=1 if args.no_pipeline else 0, trials=2, colors=d_col, seeds=d_seeds, camera=d_cam,
+        tuning = pt.autotune(ctx, bvh, frames_per_build=1 if (args.no_pipeline or args.frames_in_flight == 1) else 0, trials=2, colors=d_col, seeds=d_seeds, camera=d_cam,
                              shapes=d_shapes, shapeCnt=n, lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP,
                              max_depth=DEPTH, pixels=d_pix)
         if not args.treelet and tuning["treelet"]:
@@ -1143,50 +1147,96 @@ def main():
     # that last used its handle; the render of frame k waits for its build (the library inserts
     # that wait when the handle's context differs from the render's).  Event slots: build stream
     # 2i/2i+1 = build begin/end of parity i; render stream 6+2i/7+2i = render begin/end.
-    pipeline = not args.no_pipeline
+    # Frames in flight 2: frame k's tree is built on a second context's stream while frame k-1 renders
+    # (two handles, each over its own copy of the shapes, used in turn).  A build waits for the
+    # render that last used its handle; the render of frame k waits for its build (the library
+    # inserts that wait when the handle's context differs from the render's).
+    # Frames in flight 3: three handles, and each step enqueues frame k's render (its tree was built
+    # during the previous step) and then frame k+1's build; frames alternate between two render
+    # contexts (streams) with their own frame buffers.  A render then starts while the previous
+    # frame's render drains (its longest pixel chains finish on a few waves while the rest of the
+    # device would idle), and a build runs in whatever a draining render leaves free.  Every frame
+    # still rebuilds its tree and renders all its samples.
+    # Event slots: build context 2s/2s+1 = build begin/end of handle s; render contexts 6+2s/7+2s =
+    # render begin/end of the frame on handle s.
+    fif = 1 if args.no_pipeline else args.frames_in_flight
+    pipeline = fif >= 2
     if pipeline:
         cb = pt.Context(0)
         bits = int(tuning["digit_bits"]) if tuning else 8
-        d_shapes_b = ctx.to_device(shapes)
-        slots = [(pt.CLBVH(cb, d_shapes, n, digit_bits=bits), d_shapes),
-                 (pt.CLBVH(cb, d_shapes_b, n, digit_bits=bits), d_shapes_b)]
-        ctx.record(7)
-        ctx.record(9)
+        nslot = 2 if fif == 2 else 3
+        shape_bufs = [d_shapes] + [ctx.to_device(shapes) for _ in range(nslot - 1)]
+        slots = [(pt.CLBVH(cb, sb, n, digit_bits=bits), sb) for sb in shape_bufs]
+        rctx = [ctx, ctx]
+        fbuf = [(d_col, d_seeds, d_pix), (d_col, d_seeds, d_pix)]
+        if fif == 3:
+            r1 = pt.Context(0)
+            r1.set_engine(args.engine)
+            for k in ("blocks_per_cu", "regroup", "wide"):
+                v = tuning[k] if tuning else {"blocks_per_cu": args.blocks_per_cu, "regroup": args.regroup,
+                                              "wide": args.wide}[k]
+                if v:
+                    r1.set_tuning(k, int(v))
+            rctx[1] = r1
+            fbuf[1] = (r1.alloc(W * H * 12, np.float32).zero(), r1.alloc(W * H * 8, np.uint32),
+                       r1.alloc(W * H * 4, np.uint32))
+        for sl in range(nslot):
+            for c in rctx:
+                c.record(7 + 2 * sl)
         frame = [0]
-        last = [None]
+        last = [None]          # (slot, render context) of the last frame enqueued
+        built = [False]        # fif 3: the next frame's tree has been enqueued
 
-        def pstep(ray_counter=None, trav_stats=None):
-            """Enqueue frame k; returns (build_ms, render_ms) of frame k-1 (None for the first), read
-            after frame k's render is queued so the host's wait never idles the device."""
-            i = frame[0] & 1
-            frame[0] += 1
-            b, s_i = slots[i]
-            cb.wait_event(ctx, 7 + 2 * i)
-            cb.record(2 * i)
+        def build(sl, wait_ctx):
+            b = slots[sl][0]
+            cb.wait_event(wait_ctx, 7 + 2 * sl)    # the render that last used this handle
+            cb.record(2 * sl)
             b.reset()
             b.buildRadixTree()
             b.buildBVHTree()
             for _ in range(passes if treelet else 0):
                 b.optimize(treelet)
-            cb.record(2 * i + 1)
-            d_seeds.copy_from(seeds0)
-            ctx.wait_event(cb, 2 * i + 1)
-            ctx.record(6 + 2 * i)
-            pt.render(ctx, b, None, colors=d_col, seeds=d_seeds, camera=d_cam, shapes=s_i, shapeCnt=n,
-                      lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=d_pix,
-                      ray_counter=ray_counter, trav_stats=trav_stats)
-            ctx.record(7 + 2 * i)
-            prev, last[0] = last[0], i
+            cb.record(2 * sl + 1)
+
+        def render(k, sl, ray_counter, trav_stats):
+            rc = rctx[k & 1]
+            col_i, seeds_i, pix_i = fbuf[k & 1]
+            seeds_i.copy_from(seeds0)
+            rc.wait_event(cb, 2 * sl + 1)
+            rc.record(6 + 2 * sl)
+            pt.render(rc, slots[sl][0], None, colors=col_i, seeds=seeds_i, camera=d_cam, shapes=slots[sl][1],
+                      shapeCnt=n, lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH,
+                      pixels=pix_i, ray_counter=ray_counter, trav_stats=trav_stats)
+            rc.record(7 + 2 * sl)
+
+        def pstep(ray_counter=None, trav_stats=None):
+            """Enqueue frame k; returns (build_ms, render_ms) of frame k-1 (None for the first), read
+            after frame k's render is queued so the host's wait never idles the device."""
+            k = frame[0]
+            frame[0] += 1
+            sl = k % nslot
+            if fif == 2:
+                build(sl, rctx[k & 1])
+                render(k, sl, ray_counter, trav_stats)
+            else:
+                if not built[0]:
+                    build(sl, rctx[k & 1])
+                render(k, sl, ray_counter, trav_stats)
+                # the next frame's tree; its handle was last rendered by frame k-2, on this frame's context
+                build((k + 1) % nslot, rctx[k & 1])
+                built[0] = True
+            prev, last[0] = last[0], (sl, rctx[k & 1])
             if prev is None:
                 return None
-            return cb.elapsed_ms(2 * prev, 2 * prev + 1), ctx.elapsed_ms(6 + 2 * prev, 7 + 2 * prev)
+            return cb.elapsed_ms(2 * prev[0], 2 * prev[0] + 1), prev[1].elapsed_ms(6 + 2 * prev[0], 7 + 2 * prev[0])
 
         def pdrain():
-            """Timings of the last frame enqueued; leaves both streams idle."""
+            """Timings of the last frame enqueued; leaves every stream idle (a build enqueued for a
+            next frame is finished and used by the next step)."""
             prev, last[0] = last[0], None
-            ctx.synchronize()
-            cb.synchronize()
-            return cb.elapsed_ms(2 * prev, 2 * prev + 1), ctx.elapsed_ms(6 + 2 * prev, 7 + 2 * prev)
+            for c in (*rctx, cb):
+                c.synchronize()
+            return cb.elapsed_ms(2 * prev[0], 2 * prev[0] + 1), prev[1].elapsed_ms(6 + 2 * prev[0], 7 + 2 * prev[0])
 
     # counter calibration: one plain copy of a known 256 MiB (beyond the 256 MB MALL)
     cal_src = ctx.alloc(256 << 20, np.uint8)
@@ -1313,7 +1363,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": WORKLOAD,
                    "shapes": n, "width": W, "height": H, "spp": SPP, "max_depth": DEPTH,
-                   "frames_in_flight": 2 if pipeline else 1},
+                   "frames_in_flight": fif},
         "rays_per_step": rays // args.steps,
         "closest_per_step": closest // args.steps,
         "anyhit_per_step": anyhit // args.steps,
@@ -1346,6 +1396,8 @@ def main():
         for b, _ in slots:
             b.close()
         cb.close()
+        if rctx[1] is not ctx:
+            rctx[1].close()
     bvh.close()
     ctx.close()
 

@@ -734,18 +734,20 @@ __device__ __forceinline__ void karras_morton_body(const unsigned *keys, int n, 
     if (i == 0) mpar_int[0] = -1;
 }
 
-// Pass 1: one workgroup per 1024 consecutive leaves.  Every node whose leaf range lies inside the
+// Pass 1: one workgroup per 512 consecutive leaves.  Every node whose leaf range lies inside the
 // chunk has its Karras index inside the chunk too, so visit counters and bounds live in LDS and the
 // bottom-up climb synchronises through workgroup-scope LDS atomics only.  A climb that reaches a
 // node spanning chunks publishes its child's bound and height and queues the (node, child) arrival
-// in the list of the 64-chunk super-chunk that holds the node's first leaf.
+// in the list of the 64-chunk super-chunk that holds the node's first leaf.  512-leaf chunks keep
+// the pass at 40 KB of LDS per workgroup, so it runs beside a render's persistent workgroups (two
+// per CU hold about 100 KB of the 160 KB) instead of waiting for them to retire.
 //
 // List capacities follow from the tree height h <= 62 (30-bit codes, ties split by a 32-bit index):
 // at most h nodes cross any boundary and each node receives two arrivals, so a super-chunk's list
 // holds at most 2 * 62 * 64 entries and the list of nodes spanning super-chunks 2 * 62 per boundary.
 // An overflow (a malformed tree) sets the build's error word instead of writing out of bounds.
-constexpr int kMChunk = 1024;
-constexpr unsigned kMSuper = 65536;            // leaves per super-chunk (64 chunks)
+constexpr int kMChunk = 512;
+constexpr unsigned kMSuper = 64u * kMChunk;    // leaves per super-chunk (64 chunks)
 constexpr unsigned kXCap = 8192;               // arrivals per super-chunk list
 // Climb steps of one thread: a tree over 30-bit codes with ties split by a 32-bit index is at most 62
 // levels high, so a climb longer than this follows a malformed parent chain (a cycle): it stops and

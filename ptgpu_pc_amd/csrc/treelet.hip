@@ -388,14 +388,15 @@ int clbvh_optimize(ptgpu_clbvh *h, int tsize) {
                        L.cursor, L.order);
     PT_HIP(hipGetLastError());
     // deepest first, one launch per depth over that depth's list; depths 0..7 (at most 2^d nodes
-    // each) in one workgroup launch with a barrier between depths
+    // each) in one workgroup launch with a barrier between depths.  That workgroup is 4 waves (32
+    // treelets at a time, 44 KB of LDS): small enough to run beside a render's resident workgroups.
     const int dtop = levels < 8 ? levels : 8;
     int grid = h->ctx->cu_count * 16;
     if (grid > (m + kGroups - 1) / kGroups) grid = (m + kGroups - 1) / kGroups;
     for (int d = levels - 1; d >= dtop; d--)
         hipLaunchKernelGGL(k_treelet<1>, dim3(grid), dim3(64), 0, st, (const int *)L.order,
                            (const unsigned *)L.dstart, d, d, tsize, L.nodes);
-    hipLaunchKernelGGL(k_treelet<8>, dim3(1), dim3(512), 0, st, (const int *)L.order, (const unsigned *)L.dstart,
+    hipLaunchKernelGGL(k_treelet<4>, dim3(1), dim3(256), 0, st, (const int *)L.order, (const unsigned *)L.dstart,
                        dtop - 1, 0, tsize, L.nodes);
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_opt_emit, g, b, 0, st, L.nodes, m, h->tnodes.as<TNode>());
