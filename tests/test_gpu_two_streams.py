@@ -4,9 +4,10 @@ A render whose CLBVH handle belongs to another context waits for that context's 
 (the build and the traversal layout derived at the first render after it), so a caller can build
 frame k+1 on one stream while frame k renders on another.  Checked bit for bit against the same
 frames built and rendered on one stream: a handle rendered on a second context straight after its
-build, and six frames in flight over two handles whose shape arrays are refilled between uses
-(three scenes in turn, so a render that read the other slot's tree or a refill that overtook the
-render still using it changes the image), with and without treelet restructuring.
+build, six frames in flight over two handles whose shape arrays are refilled between uses (three
+scenes in turn, so a render that read the other slot's tree or a refill that overtook the render
+still using it changes the image), and seven frames three in flight (three handles, renders
+alternating between two contexts), with and without treelet restructuring.
 """
 import numpy as np
 import pytest
@@ -108,4 +109,63 @@ def test_two_frames_in_flight(ctx, treelet):
         for h in hs:
             h.close()
     finally:
+        cb.close()
+
+
+@pytest.mark.parametrize("treelet", [0, 5])
+def test_three_frames_in_flight(ctx, treelet):
+    """The bench's three-in-flight schedule: three handles over their own shape arrays, frame k+1's tree
+    built on a third context right after frame k's render is queued, renders alternating between two
+    contexts with their own frame buffers (so frame k+1 renders while frame k drains).  Shape arrays
+    are refilled with three scenes in turn; every frame equals its one-stream render bit for bit."""
+    say("case three frames in flight, treelet", treelet)
+    sc = _scenes()
+    n, lc = len(sc[0][0]), sc[0][1]
+    cam = sc[0][2]
+    pt.update_camera(cam)
+    refs = [_serial(ctx, s, lc, cam, treelet) for s, _, _ in sc]
+    cb, r1 = pt.Context(0), pt.Context(0)
+    try:
+        rc = [ctx, r1]
+        src = [ctx.to_device(s) for s, _, _ in sc]
+        arr = [cb.alloc(src[0].nbytes) for _ in range(3)]
+        hs = [pt.CLBVH(cb, arr[i], n) for i in range(3)]
+        d_cam = ctx.to_device(cam)
+        seeds0 = ctx.to_device(default_seeds(W, H))
+
+        def build(sl, j, after):
+            # the render that last read slot sl ran on context `after`; refill, then rebuild
+            cb.wait_event(after, 10 + sl)
+            arr[sl].copy_from(src[j])
+            hs[sl].reset()
+            hs[sl].buildRadixTree()
+            hs[sl].buildBVHTree()
+            if treelet:
+                hs[sl].optimize(treelet)
+
+        for c in rc:
+            for sl in range(3):
+                c.record(10 + sl)
+        outs = []
+        build(0, 0, ctx)
+        for k in range(7):
+            sl, r = k % 3, rc[k & 1]
+            col = r.alloc(W * H * 12, F32).zero()
+            seeds = r.alloc(W * H * 8, np.uint32).copy_from(seeds0)
+            pix = r.alloc(W * H * 4, np.uint32)
+            pt.render(r, hs[sl], None, colors=col, seeds=seeds, camera=d_cam, shapes=arr[sl], shapeCnt=n,
+                      lightCnt=lc, width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=pix)
+            r.record(10 + sl)
+            outs.append((col, seeds, pix))
+            build((k + 1) % 3, (k + 1) % 3, rc[(k + 1 - 3) & 1] if k >= 2 else ctx)
+        for c in (ctx, r1, cb):
+            c.synchronize()
+        for k, (col, seeds, pix) in enumerate(outs):
+            got = (col.download(F32), seeds.download(np.uint32), pix.download(np.uint32))
+            for x, y in zip(got, refs[k % 3]):
+                assert x.tobytes() == y.tobytes(), k
+        for h in hs:
+            h.close()
+    finally:
+        r1.close()
         cb.close()
