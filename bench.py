@@ -1058,11 +1058,11 @@ def main():
     ap.add_argument("--no-frames", action="store_true", help="skip the frames-per-launch comparison in the default line")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="build and render each frame in turn on one stream (the same as --frames-in-flight 1)")
-    ap.add_argument("--frames-in-flight", type=int, default=3, choices=(1, 2, 3),
+    ap.add_argument("--frames-in-flight", type=int, default=3, choices=(1, 2, 3, 4, 5),
                     help="1: build and render each frame in turn on one stream; 2: the next frame's tree is built "
                          "on a second stream while the current frame renders; 3 (default): as 2, and frames "
                          "alternate between two render streams with their own frame buffers, so a frame's render "
-                         "starts while the previous frame's render drains")
+                         "starts while the previous frame's render drains; 4, 5: as 3 with 3 or 4 render streams")
     ap.add_argument("--tuning", default=None, help="JSON file with the tuning object of an earlier run (no autotune)")
     ap.add_argument("--requests", type=int, default=256)
     ap.add_argument("--rate", type=float, default=500.0)
@@ -1181,6 +1181,8 @@ def main():
     # frame's render drains (its longest pixel chains finish on a few waves while the rest of the
     # device would idle), and a build runs in whatever a draining render leaves free.  Every frame
     # still rebuilds its tree and renders all its samples.
+    # Frames in flight 4 and 5: the same with 3 or 4 render contexts and as many handles as frames in
+    # flight (measured no faster than 3 at C2: two renders already fill what a draining one leaves).
     # Event slots: build context 2s/2s+1 = build begin/end of handle s; render contexts 6+2s/7+2s =
     # render begin/end of the frame on handle s.
     fif = 1 if args.no_pipeline else args.frames_in_flight
@@ -1188,12 +1190,13 @@ def main():
     if pipeline:
         cb = pt.Context(0)
         bits = int(tuning["digit_bits"]) if tuning else 8
-        nslot = 2 if fif == 2 else 3
+        nslot = fif                                  # handles (each over its own copy of the shapes)
+        nr = 1 if fif == 2 else fif - 1              # render contexts, each with its own frame buffers
         shape_bufs = [d_shapes] + [ctx.to_device(shapes) for _ in range(nslot - 1)]
         slots = [(pt.CLBVH(cb, sb, n, digit_bits=bits), sb) for sb in shape_bufs]
-        rctx = [ctx, ctx]
-        fbuf = [(d_col, d_seeds, d_pix), (d_col, d_seeds, d_pix)]
-        if fif == 3:
+        rctx = [ctx]
+        fbuf = [(d_col, d_seeds, d_pix)]
+        for _ in range(nr - 1):
             r1 = pt.Context(0)
             r1.set_engine(args.engine)
             for k in ("blocks_per_cu", "regroup", "wide"):
@@ -1201,9 +1204,9 @@ def main():
                                               "wide": args.wide}[k]
                 if v:
                     r1.set_tuning(k, int(v))
-            rctx[1] = r1
-            fbuf[1] = (r1.alloc(W * H * 12, np.float32).zero(), r1.alloc(W * H * 8, np.uint32),
-                       r1.alloc(W * H * 4, np.uint32))
+            rctx.append(r1)
+            fbuf.append((r1.alloc(W * H * 12, np.float32).zero(), r1.alloc(W * H * 8, np.uint32),
+                         r1.alloc(W * H * 4, np.uint32)))
         for sl in range(nslot):
             for c in rctx:
                 c.record(7 + 2 * sl)
@@ -1223,8 +1226,8 @@ def main():
             cb.record(2 * sl + 1)
 
         def render(k, sl, ray_counter, trav_stats):
-            rc = rctx[k & 1]
-            col_i, seeds_i, pix_i = fbuf[k & 1]
+            rc = rctx[k % nr]
+            col_i, seeds_i, pix_i = fbuf[k % nr]
             seeds_i.copy_from(seeds0)
             rc.wait_event(cb, 2 * sl + 1)
             rc.record(6 + 2 * sl)
@@ -1240,16 +1243,16 @@ def main():
             frame[0] += 1
             sl = k % nslot
             if fif == 2:
-                build(sl, rctx[k & 1])
+                build(sl, ctx)
                 render(k, sl, ray_counter, trav_stats)
             else:
                 if not built[0]:
-                    build(sl, rctx[k & 1])
+                    build(sl, rctx[(k - nslot) % nr])
                 render(k, sl, ray_counter, trav_stats)
-                # the next frame's tree; its handle was last rendered by frame k-2, on this frame's context
-                build((k + 1) % nslot, rctx[k & 1])
+                # the next frame's tree; its handle was last rendered by frame k+1-nslot
+                build((k + 1) % nslot, rctx[(k + 1 - nslot) % nr])
                 built[0] = True
-            prev, last[0] = last[0], (sl, rctx[k & 1])
+            prev, last[0] = last[0], (sl, rctx[k % nr])
             if prev is None:
                 return None
             return cb.elapsed_ms(2 * prev[0], 2 * prev[0] + 1), prev[1].elapsed_ms(6 + 2 * prev[0], 7 + 2 * prev[0])
@@ -1339,7 +1342,7 @@ def main():
     alg_bytes = box_bytes * box_tests + 36 * prim_tests + W * H * (24 + 16 + 4)
     roofline = render_roofline(alg_bytes, render_avg, box_tests, prim_tests, box_bytes,
                                measured=args.engine in ("auto", "lanes") and not args.treelet,
-                               step_ms=wall * 1e3 / args.steps if fif == 3 else None)
+                               step_ms=wall * 1e3 / args.steps if fif >= 3 else None)
 
     served = None
     if not args.no_serve:
@@ -1421,8 +1424,8 @@ def main():
         for b, _ in slots:
             b.close()
         cb.close()
-        if rctx[1] is not ctx:
-            rctx[1].close()
+        for c in rctx[1:]:
+            c.close()
     bvh.close()
     ctx.close()
 
