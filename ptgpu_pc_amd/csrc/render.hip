@@ -60,6 +60,7 @@ struct RenderParams {
     int njobs;
     const int2 *tiles;            // tiled launches (RadianceBoxGPU): tile origins; NULL renders the whole frame
     int tile_w, tile_h;
+    float one;                    // 1.0f, opaque to the compiler (slab_h's mixed-precision differences)
     int depth_cap;                // deepest tree the launch's stack holds (checked when acc.height_check is set)
     unsigned *status;             // the context's check words: [0] flags, [1] deepest height seen
 };
@@ -506,8 +507,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                 }
                             }
                             const int ref = qhn ? (int)w.w : kEmptyRef;
-                            float tc = slab(half_lo(w.x), half_lo(w.y), half_lo(w.z), half_hi(w.x), half_hi(w.y),
-                                            half_hi(w.z), q_o, q_inv, q_tb);
+                            float tc = slab_h(w.x, w.y, w.z, q_o, q_inv, q_tb, P.one);
                             if (!qhn) tc = inf;
                             if (STATS) st_nodes += ref != kEmptyRef;
                             if (!occl && ref < 0 && tc != inf && ~ref < q_nprims) {
@@ -684,14 +684,10 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     int nn = -1;
                     if (!done && hn) {
                         int r0 = (int)q0.w, r1 = (int)q1.w, r2 = (int)q2.w, r3 = (int)q3.w;
-                        float t0 = slab(half_lo(q0.x), half_lo(q0.y), half_lo(q0.z), half_hi(q0.x), half_hi(q0.y),
-                                        half_hi(q0.z), ro, inv, tb);
-                        float t1 = slab(half_lo(q1.x), half_lo(q1.y), half_lo(q1.z), half_hi(q1.x), half_hi(q1.y),
-                                        half_hi(q1.z), ro, inv, tb);
-                        float t2 = slab(half_lo(q2.x), half_lo(q2.y), half_lo(q2.z), half_hi(q2.x), half_hi(q2.y),
-                                        half_hi(q2.z), ro, inv, tb);
-                        float t3 = slab(half_lo(q3.x), half_lo(q3.y), half_lo(q3.z), half_hi(q3.x), half_hi(q3.y),
-                                        half_hi(q3.z), ro, inv, tb);
+                        float t0 = slab_h(q0.x, q0.y, q0.z, ro, inv, tb, P.one);
+                        float t1 = slab_h(q1.x, q1.y, q1.z, ro, inv, tb, P.one);
+                        float t2 = slab_h(q2.x, q2.y, q2.z, ro, inv, tb, P.one);
+                        float t3 = slab_h(q3.x, q3.y, q3.z, ro, inv, tb, P.one);
                         if (STATS) st_nodes += (r0 != kEmptyRef) + (r1 != kEmptyRef) + (r2 != kEmptyRef) + (r3 != kEmptyRef);
                         auto cswap = [](float &ta, int &ra, float &tb_, int &rb) {
                             const bool sw = tb_ < ta;
@@ -1513,6 +1509,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     P.tiles = L.tiles;
     P.tile_w = L.tile_w;
     P.tile_h = L.tile_h;
+    P.one = 1.0f;
     P.depth_cap = binary_depth_cap(st);
     P.status = nullptr;
     if (acc.height_check) {
@@ -2038,6 +2035,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
     P.jobs = static_cast<const RenderJob *>(d_table);
     P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(d_table) + boff);
     P.njobs = njobs;
+    P.one = 1.0f;
     P.depth_cap = wide_depth_cap(st);
     P.status = checks ? ctx->rec_status.as<unsigned>() : nullptr;
     long long blocks = (long long)ctx->cu_count * ctx->batch_blocks_per_cu;

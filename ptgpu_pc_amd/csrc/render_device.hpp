@@ -174,6 +174,25 @@ PT_HD float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 
     return tn <= tf * kSlack ? tn : __builtin_inff();
 }
 
+// The same test on a binary16 box: each word holds a bound pair, lo in its low half and hi in its
+// high half.  Each difference `bound - o` is one mixed-precision fma, fma(bound, one, -o) with the
+// binary16 bound read from its half of the word: the bound converts to binary32 exactly and the
+// product by one is exact, so the result is the rounded difference, bit for bit what a conversion
+// and a subtraction give.  `one` is 1.0f held where the compiler cannot see it (a literal would be
+// folded into that conversion and subtraction, two instructions instead of one).
+__device__ __forceinline__ float slab_h(unsigned wx, unsigned wy, unsigned wz, f3 o, f3 inv, float tlimit,
+                                        float one) {
+    const float tx0 = __builtin_fmaf(half_lo(wx), one, -o.x) * inv.x;
+    const float tx1 = __builtin_fmaf(half_hi(wx), one, -o.x) * inv.x;
+    const float ty0 = __builtin_fmaf(half_lo(wy), one, -o.y) * inv.y;
+    const float ty1 = __builtin_fmaf(half_hi(wy), one, -o.y) * inv.y;
+    const float tz0 = __builtin_fmaf(half_lo(wz), one, -o.z) * inv.z;
+    const float tz1 = __builtin_fmaf(half_hi(wz), one, -o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlimit));
+    return tn <= tf * kSlack ? tn : __builtin_inff();
+}
+
 // Closest hit over a prepared structure.  Returns the hit shape index or -1; t in *tout.
 template <int STACK>
 __device__ int trace_closest(const DevAccel &A, f3 o, f3 d, float *tout, int *stk, unsigned *nodes_visited,
