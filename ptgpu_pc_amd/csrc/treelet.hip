@@ -387,10 +387,11 @@ int clbvh_optimize(ptgpu_clbvh *h, int tsize) {
     hipLaunchKernelGGL(k_opt_scatter, g, b, 0, st, (const OptNode *)L.nodes, m, levels, (const unsigned *)L.dstart,
                        L.cursor, L.order);
     PT_HIP(hipGetLastError());
-    // deepest first, one launch per depth over that depth's list; depths 0..7 (at most 2^d nodes
-    // each) in one workgroup launch with a barrier between depths.  That workgroup is 4 waves (32
-    // treelets at a time, 44 KB of LDS): small enough to run beside a render's resident workgroups.
-    const int dtop = levels < 8 ? levels : 8;
+    // deepest first, one launch per depth over that depth's list; depths 0..4 (at most 2^d <= 16
+    // nodes each) in one workgroup launch with a barrier between depths, one round per depth.  That
+    // workgroup is 4 waves (32 treelets at a time, 44 KB of LDS): small enough to run beside a
+    // render's resident workgroups.
+    const int dtop = levels < 5 ? levels : 5;
     int grid = h->ctx->cu_count * 16;
     if (grid > (m + kGroups - 1) / kGroups) grid = (m + kGroups - 1) / kGroups;
     for (int d = levels - 1; d >= dtop; d--)
