@@ -151,6 +151,10 @@ struct BatchChannel {
     size_t pinned_bytes = 0;
     hipEvent_t copied = nullptr;   // the last table upload from `pinned` has finished
     hipEvent_t ready = nullptr;    // scene preparation on the context's stream, waited on by `stream`
+    // jobs are rendered in one launch per traversal-stack class; the classes after the first run on
+    // these streams, forked from and joined back into `stream` (default priority, created on first use)
+    hipStream_t sub[2] = {nullptr, nullptr};
+    hipEvent_t fork = nullptr, join[2] = {nullptr, nullptr};
     void release();
 };
 }  // namespace ptgpu

@@ -51,6 +51,14 @@ void BatchChannel::release() {
     if (copied) (void)hipEventDestroy(copied);
     if (ready) (void)hipEventDestroy(ready);
     copied = ready = nullptr;
+    for (int i = 0; i < 2; i++) {
+        if (sub[i]) (void)hipStreamDestroy(sub[i]);
+        if (join[i]) (void)hipEventDestroy(join[i]);
+        sub[i] = nullptr;
+        join[i] = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    fork = nullptr;
 }
 
 }  // namespace ptgpu

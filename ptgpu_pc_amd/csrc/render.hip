@@ -1932,6 +1932,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
                  unsigned long long *d_ray_counter) {
     std::vector<RenderJob> table((size_t)njobs);
     std::vector<unsigned> begin((size_t)njobs + 1, 0u);
+    std::vector<int> jst((size_t)njobs, 0);   // each job's traversal-stack class (32, 48 or 64 entries)
     int st = 0;
     bool half = ctx->wide == 2;   // binary16 nodes only when every job's tree has them
     bool checks = false;          // some job's tree height is checked on the device (recorded after a build)
@@ -1958,6 +1959,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         const int sw = stack_for_wide(R.acc);
         PT_CHECK(sw > 0, PTGPU_ERR_UNSUPPORTED, "job %d: tree height %d exceeds the traversal stack", j, R.acc.depth);
         st = sw > st ? sw : st;
+        jst[(size_t)j] = sw;
         half = half && R.acc.nodes4h;
         checks = checks || R.acc.height_check;
         if (J.bvh) {
@@ -1986,6 +1988,48 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         PT_CHECK(total < (1ull << 31), PTGPU_ERR_UNSUPPORTED, "batch of more than 2^31 pixels");
     }
     begin[(size_t)njobs] = (unsigned)total;
+    (void)st;
+    (void)checks;
+    // One launch per stack class, deepest first, in job order within a class.  The LDS stack of every
+    // workgroup is sized for its launch's class, so a few deep trees (64 entries, 64 KB: two
+    // workgroups per CU) do not hold the whole batch at that occupancy; the classes run at once, on
+    // the channel's stream and two more forked from it.
+    struct Cls {
+        int st;
+        std::vector<int> jobs;
+        unsigned long long total;
+        bool checks;
+        size_t jo, bo;   // offsets of its jobs and its prefix sums in the table image
+    };
+    std::vector<Cls> cls;
+    for (int c : {64, 48, 32}) {
+        Cls K{c, {}, 0, false, 0, 0};
+        for (int j = 0; j < njobs; j++)
+            if (jst[(size_t)j] == c) {
+                K.jobs.push_back(j);
+                K.total += (unsigned long long)table[(size_t)j].width * (unsigned long long)table[(size_t)j].height;
+                K.checks = K.checks || table[(size_t)j].acc.height_check;
+            }
+        if (!K.jobs.empty()) cls.push_back(std::move(K));
+    }
+    size_t img_bytes = 0;
+    for (Cls &K : cls) {
+        K.jo = img_bytes;
+        img_bytes = (K.jo + K.jobs.size() * sizeof(RenderJob) + 63) & ~size_t(63);
+        K.bo = img_bytes;
+        img_bytes = (K.bo + (K.jobs.size() + 1) * sizeof(unsigned) + 63) & ~size_t(63);
+    }
+    std::vector<char> img(img_bytes, 0);
+    for (Cls &K : cls) {
+        unsigned run = 0;
+        for (size_t q = 0; q < K.jobs.size(); q++) {
+            const RenderJob &R = table[(size_t)K.jobs[q]];
+            std::memcpy(img.data() + K.jo + q * sizeof(RenderJob), &R, sizeof(RenderJob));
+            std::memcpy(img.data() + K.bo + q * sizeof(unsigned), &run, sizeof(unsigned));
+            run += (unsigned)(R.width * R.height);
+        }
+        std::memcpy(img.data() + K.bo + K.jobs.size() * sizeof(unsigned), &run, sizeof(unsigned));
+    }
     hipStream_t stream = ch.stream;
     if (!ch.copied) PT_HIP(hipEventCreateWithFlags(&ch.copied, hipEventDisableTiming));
     if (!ch.ready) PT_HIP(hipEventCreateWithFlags(&ch.ready, hipEventDisableTiming));
@@ -1993,68 +2037,79 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         PT_HIP(hipEventRecord(ch.ready, ctx->stream));
         PT_HIP(hipStreamWaitEvent(stream, ch.ready, 0));
     }
-    const size_t tb = table.size() * sizeof(RenderJob), bb = begin.size() * sizeof(unsigned);
-    const size_t boff = (tb + 63) & ~size_t(63);
     void *d_table = nullptr;
     if (ctx->capturing) {
         // recorded: the table gets a pinned copy and a device buffer of its own, kept with the graph
-        std::vector<char> img(boff + bb, 0);
-        std::memcpy(img.data(), table.data(), tb);
-        std::memcpy(img.data() + boff, begin.data(), bb);
-        PT_TRY(record_stage(ctx, stream, img.data(), boff + bb, boff + bb, &d_table));
+        PT_TRY(record_stage(ctx, stream, img.data(), img_bytes, img_bytes, &d_table));
     } else {
         // stage the table through pinned memory; the channel's previous copy must be done before reuse
-        const size_t bytes = tb + bb + 64;
         PT_HIP(hipEventSynchronize(ch.copied));
-        if (ch.pinned_bytes < bytes) {
+        if (ch.pinned_bytes < img_bytes) {
             if (ch.pinned) PT_HIP(hipHostFree(ch.pinned));
             ch.pinned = nullptr;
             ch.pinned_bytes = 0;
-            PT_HIP(hipHostMalloc(&ch.pinned, bytes * 2, hipHostMallocDefault));
-            ch.pinned_bytes = bytes * 2;
+            PT_HIP(hipHostMalloc(&ch.pinned, img_bytes * 2, hipHostMallocDefault));
+            ch.pinned_bytes = img_bytes * 2;
         }
-        char *hp = static_cast<char *>(ch.pinned);
-        std::memcpy(hp, table.data(), tb);
-        std::memcpy(hp + boff, begin.data(), bb);
-        PT_TRY(ch.table.ensure(boff + bb));
-        PT_HIP(hipMemcpyAsync(ch.table.p, hp, boff + bb, hipMemcpyHostToDevice, stream));
+        std::memcpy(ch.pinned, img.data(), img_bytes);
+        PT_TRY(ch.table.ensure(img_bytes));
+        PT_HIP(hipMemcpyAsync(ch.table.p, ch.pinned, img_bytes, hipMemcpyHostToDevice, stream));
         PT_HIP(hipEventRecord(ch.copied, stream));
         d_table = ch.table.p;
     }
-    PT_TRY(ch.counter.ensure(64));
-    PT_TRY(fill_u32(stream, ch.counter.p, 0u, 1));
-    RenderParams P;
-    std::memset(static_cast<void *>(&P), 0, sizeof(P));
-    P.acc = table[0].acc;   // unused by the batch variant
-    P.ray_counter = d_ray_counter;
-    P.trav_stats = nullptr;
-    P.work_counter = ch.counter.as<unsigned>();
-    P.total = (int)total;
-    P.regroup = ctx->batch_regroup;
-    P.quad = ctx->quad;
-    P.jobs = static_cast<const RenderJob *>(d_table);
-    P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(d_table) + boff);
-    P.njobs = njobs;
-    P.one = 1.0f;
-    P.depth_cap = wide_depth_cap(st);
-    P.status = checks ? ctx->rec_status.as<unsigned>() : nullptr;
-    long long blocks = (long long)ctx->cu_count * ctx->batch_blocks_per_cu;
-    const long long need = ((long long)total + 255) / 256;
-    if (blocks > need) blocks = need;
-    if (blocks < 1) blocks = 1;
-#define PT_BATCH(ST, HALF_) \
-    hipLaunchKernelGGL((k_render_lanes<ST, false, true, true, HALF_>), dim3((unsigned)blocks), dim3(256), 0, stream, P)
-    if (half) {
-        if (st == 32) PT_BATCH(32, true);
-        else if (st == 48) PT_BATCH(48, true);
-        else PT_BATCH(64, true);
-    } else {
-        if (st == 32) PT_BATCH(32, false);
-        else if (st == 48) PT_BATCH(48, false);
-        else PT_BATCH(64, false);
+    const int ncls = (int)cls.size();
+    PT_TRY(ch.counter.ensure((size_t)64 * ncls));
+    PT_TRY(fill_u32(stream, ch.counter.p, 0u, (size_t)16 * ncls));   // one work counter per class, 64 B apart
+    if (ncls > 1) {
+        if (!ch.fork) PT_HIP(hipEventCreateWithFlags(&ch.fork, hipEventDisableTiming));
+        PT_HIP(hipEventRecord(ch.fork, stream));
     }
+    for (int k = 0; k < ncls; k++) {
+        const Cls &K = cls[(size_t)k];
+        hipStream_t s = stream;
+        if (k > 0) {
+            if (!ch.sub[k - 1]) PT_HIP(hipStreamCreateWithFlags(&ch.sub[k - 1], hipStreamNonBlocking));
+            if (!ch.join[k - 1]) PT_HIP(hipEventCreateWithFlags(&ch.join[k - 1], hipEventDisableTiming));
+            s = ch.sub[k - 1];
+            PT_HIP(hipStreamWaitEvent(s, ch.fork, 0));
+        }
+        RenderParams P;
+        std::memset(static_cast<void *>(&P), 0, sizeof(P));
+        P.acc = table[(size_t)K.jobs[0]].acc;   // unused by the batch variant
+        P.ray_counter = d_ray_counter;
+        P.trav_stats = nullptr;
+        P.work_counter = ch.counter.as<unsigned>() + 16 * k;
+        P.total = (int)K.total;
+        P.regroup = ctx->batch_regroup;
+        P.quad = ctx->quad;
+        P.jobs = reinterpret_cast<const RenderJob *>(static_cast<char *>(d_table) + K.jo);
+        P.job_begin = reinterpret_cast<const unsigned *>(static_cast<char *>(d_table) + K.bo);
+        P.njobs = (int)K.jobs.size();
+        P.one = 1.0f;
+        P.depth_cap = wide_depth_cap(K.st);
+        P.status = K.checks ? ctx->rec_status.as<unsigned>() : nullptr;
+        long long blocks = (long long)ctx->cu_count * ctx->batch_blocks_per_cu;
+        const long long need = ((long long)K.total + 255) / 256;
+        if (blocks > need) blocks = need;
+        if (blocks < 1) blocks = 1;
+#define PT_BATCH(ST, HALF_) \
+    hipLaunchKernelGGL((k_render_lanes<ST, false, true, true, HALF_>), dim3((unsigned)blocks), dim3(256), 0, s, P)
+        if (half) {
+            if (K.st == 32) PT_BATCH(32, true);
+            else if (K.st == 48) PT_BATCH(48, true);
+            else PT_BATCH(64, true);
+        } else {
+            if (K.st == 32) PT_BATCH(32, false);
+            else if (K.st == 48) PT_BATCH(48, false);
+            else PT_BATCH(64, false);
+        }
 #undef PT_BATCH
-    PT_HIP(hipGetLastError());
+        PT_HIP(hipGetLastError());
+        if (k > 0) {
+            PT_HIP(hipEventRecord(ch.join[k - 1], s));
+            PT_HIP(hipStreamWaitEvent(stream, ch.join[k - 1], 0));
+        }
+    }
     return PTGPU_OK;
 }
 }  // namespace ptgpu
