@@ -194,6 +194,14 @@ int ptgpu_clbvh_device_sorted(ptgpu_clbvh *h, const int **d_sorted);
  * 0 = the exported index-keyed Karras tree (BVH.cl:127-134), 1 = the Morton-keyed LBVH over the same
  * sorted leaves (default; about half the SAH cost).  btn/btl are the index-keyed tree either way. */
 int ptgpu_clbvh_set_traversal_topology(ptgpu_clbvh *h, int mode);
+/* No reference counterpart.  readback 1 (default): the first traversal or treelet pass after a build
+ * waits for the build and reads the new tree's height back to size the traversal stack.  readback 0:
+ * once a height has been read for a build of the same kind (plain or optimised), later ones are not
+ * waited for: the stack is sized for that height plus 4 (the bound recorded renders use) and the render
+ * kernels check the built height on the device; a frame whose tree is deeper is not rendered and the
+ * next ptgpu_synchronize on its context returns PTGPU_ERR_UNSUPPORTED.  For pipelines that rebuild
+ * trees of a steady height frame after frame, so the host never waits for a build. */
+int ptgpu_clbvh_set_height_readback(ptgpu_clbvh *h, int readback);
 /* Read back the traversal tree (N-1 internal nodes): refs4[i] = {left ref, right ref, lo, hi} (ref < 0 is
  * leaf ~k; [lo, hi] the node's leaf range) and boxes12[i] = left child Bound, right child Bound; *height =
  * root height.  For tests and tools. */

@@ -336,12 +336,16 @@ static int clbvh_sort(ptgpu_clbvh *h) {
 namespace ptgpu {
 // The Morton-keyed tree's height is known on the device after the refit; the traversal stack is
 // sized from it, so the first use after a build reads it back (one 4-byte copy).
-int clbvh_prepare_traversal(ptgpu_clbvh *h, bool recording) {
-    if (h->height_pending && (recording || h->ctx->capturing)) {
-        // Recorded into a graph: the height cannot be read back.  Size the stack for a bound and
-        // let the traversal check the built tree against it on the device (error word, height).
-        // The read-back stays pending for the next eager use.
-        const int known = h->known_height[h->optimized ? 1 : 0];
+int clbvh_prepare_traversal(ptgpu_clbvh *h, bool recording, bool exact) {
+    const int known = h->known_height[h->optimized ? 1 : 0];
+    const bool no_readback = recording || h->ctx->capturing || (h->speculate && known > 0);
+    if (h->height_pending && exact)
+        PT_CHECK(!recording && !h->ctx->capturing, PTGPU_ERR_UNSUPPORTED,
+                 "this call needs the tree height, which a recording cannot read back");
+    if (h->height_pending && !exact && no_readback) {
+        // Recorded into a graph, or a handle set to skip the read-back: size the stack for a bound
+        // and let the traversal check the built tree against it on the device (error word,
+        // height).  The read-back stays pending for a use that needs the height itself.
         int bound = known > 0 ? known + kRecordedHeightMargin : kRecordedHeightDefault;
         // a known height the four-wide stack holds keeps the bound within it (the margin is a guess;
         // the four-wide walk is the fast one)
@@ -436,7 +440,7 @@ int ptgpu_copy_calibration(ptgpu_ctx *ctx, void *d_dst, const void *d_src, size_
 int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12, int *height) {
     PT_CHECK(h && h_refs4 && h_boxes12, PTGPU_ERR_INVALID, "null argument");
     PT_CHECK(h->state >= 3, PTGPU_ERR_INVALID, "build_bvh_tree has not run");
-    PT_TRY(clbvh_prepare_traversal(h));
+    PT_TRY(clbvh_prepare_traversal(h, false, true));
     const int m = h->n - 1;
     if (m <= 0) {
         if (height) *height = 0;
@@ -467,6 +471,14 @@ int ptgpu_clbvh_export_traversal(ptgpu_clbvh *h, int *h_refs4, float *h_boxes12,
         for (int f = 0; f < 12; f++) h_boxes12[i * 12 + f] = v[f];   // two Bounds in reference order
     }
     if (height) *height = h->acc.depth;
+    return PTGPU_OK;
+}
+
+int ptgpu_clbvh_set_height_readback(ptgpu_clbvh *h, int readback) {
+    PT_CHECK(h, PTGPU_ERR_INVALID, "null handle");
+    PT_CHECK(readback == 0 || readback == 1, PTGPU_ERR_INVALID, "readback %d (1 read back, 0 device-checked bound)",
+             readback);
+    h->speculate = readback == 0;
     return PTGPU_OK;
 }
 

@@ -7,8 +7,9 @@
 
 namespace ptgpu {
 // Makes the handle's traversal layout current (tree height, four-wide nodes).  `recording`: the
-// caller's stream is being recorded into a graph, so nothing may be read back.
-int clbvh_prepare_traversal(ptgpu_clbvh *h, bool recording = false);
+// caller's stream is being recorded into a graph, so nothing may be read back.  `exact`: the caller
+// needs the tree's height itself, not a bound checked on the device (exports, batch queries).
+int clbvh_prepare_traversal(ptgpu_clbvh *h, bool recording = false, bool exact = false);
 // Height bound of a traversal recorded before the height could be read back: the height last read
 // for a build of the same kind (plain or optimised) plus this margin, or the default without one.
 constexpr int kRecordedHeightMargin = 4;
@@ -58,6 +59,7 @@ struct ptgpu_clbvh {
     int trav_topology = 1;    // traversal tree: 0 = the exported index-keyed tree, 1 = Morton-keyed LBVH
     bool height_pending = false;
     int known_height[2] = {0, 0};   // last heights read back: [0] plain build, [1] after optimize
+    bool speculate = false;         // no read-back once a height is known: a bound, checked on the device
     unsigned *sorted = nullptr;
     int lights_for = -1, nlights = 0;
     ptgpu::DevAccel acc;

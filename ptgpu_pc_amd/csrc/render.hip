@@ -1585,17 +1585,17 @@ using namespace ptgpu;
 
 // Resolve the traversal structure for (bvh, kd, shapes); NONE mode builds index-order prims.
 static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes, unsigned n,
-                         DevAccel *acc);
+                         DevAccel *acc, bool exact = false);
 
 extern "C" int ptgpu_kdtree_device_accel(ptgpu_kdtree *kd, ptgpu::DevAccel *acc);
 
 static int resolve_accel(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Shape *d_shapes, unsigned n,
-                         DevAccel *acc) {
+                         DevAccel *acc, bool exact) {
     PT_CHECK(!(bvh && kd), PTGPU_ERR_INVALID, "pass at most one of bvh / kd");
     if (bvh) {
         PT_CHECK(bvh->state >= 3, PTGPU_ERR_INVALID, "BVH not built (call build_bvh_tree)");
         PT_CHECK((unsigned)bvh->n == n, PTGPU_ERR_INVALID, "shapeCnt %u differs from the BVH's %d", n, bvh->n);
-        PT_TRY(clbvh_prepare_traversal(bvh, ctx->capturing));
+        PT_TRY(clbvh_prepare_traversal(bvh, ctx->capturing, exact));
         PT_CHECK(bvh->ctx->device == ctx->device, PTGPU_ERR_INVALID, "BVH built on device %d, render on device %d",
                  bvh->ctx->device, ctx->device);
         if (bvh->ctx != ctx && bvh->ctx->stream != ctx->stream) {
@@ -2122,7 +2122,7 @@ int ptgpu_trace_closest(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, cons
     PT_CHECK(shapeCnt >= 1 && n >= 0, PTGPU_ERR_INVALID, "bad sizes");
     PT_HIP(hipSetDevice(ctx->device));
     DevAccel acc;
-    PT_TRY(resolve_accel(ctx, bvh, kd, d_shapes, shapeCnt, &acc));
+    PT_TRY(resolve_accel(ctx, bvh, kd, d_shapes, shapeCnt, &acc, true));
     return launch_trace(ctx, acc, shapeCnt, d_rays, nullptr, n, d_t, d_id, nullptr, false);
 }
 
@@ -2132,7 +2132,7 @@ int ptgpu_trace_any(ptgpu_ctx *ctx, ptgpu_clbvh *bvh, ptgpu_kdtree *kd, const Sh
     PT_CHECK(shapeCnt >= 1 && n >= 0, PTGPU_ERR_INVALID, "bad sizes");
     PT_HIP(hipSetDevice(ctx->device));
     DevAccel acc;
-    PT_TRY(resolve_accel(ctx, bvh, kd, d_shapes, shapeCnt, &acc));
+    PT_TRY(resolve_accel(ctx, bvh, kd, d_shapes, shapeCnt, &acc, true));
     return launch_trace(ctx, acc, shapeCnt, d_rays, d_maxt, n, nullptr, nullptr, d_hit, true);
 }
 

@@ -101,6 +101,7 @@ _SIGS = {
     "ptgpu_clbvh_optimize": [vp, i32],
     "ptgpu_clbvh_set_digit_bits": [vp, i32],
     "ptgpu_clbvh_set_traversal_topology": [vp, i32],
+    "ptgpu_clbvh_set_height_readback": [vp, i32],
     "ptgpu_clbvh_export_traversal": [vp, vp, vp, C.POINTER(i32)],
     "ptgpu_clbvh_get_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
     "ptgpu_clbvh_device_trees": [vp, C.POINTER(vp), C.POINTER(vp)],
@@ -456,6 +457,11 @@ class CLBVH:
         (ptgpu_clbvh_build_batch): reset + buildRadixTree + buildBVHTree per handle."""
         arr = (vp * max(len(handles), 1))(*[h.handle for h in handles])
         _check(ctx.lib.ptgpu_clbvh_build_batch(ctx.handle, arr, len(handles)))
+
+    def set_height_readback(self, readback: bool) -> None:
+        """False: after the first height read back, later builds are not waited for; renders size their
+        stack for that height + 4 and check the built height on the device (ptgpu_clbvh_set_height_readback)."""
+        _check(self.ctx.lib.ptgpu_clbvh_set_height_readback(self.handle, 1 if readback else 0))
 
     def set_traversal(self, topology: str) -> None:
         """'morton' (Morton-keyed LBVH, default) or 'reference' (the exported index-keyed tree)."""

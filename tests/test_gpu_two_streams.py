@@ -169,3 +169,54 @@ def test_three_frames_in_flight(ctx, treelet):
     finally:
         r1.close()
         cb.close()
+
+
+def test_height_readback_off(ctx):
+    """ptgpu_clbvh_set_height_readback(h, 0): after the first read-back, rebuilds are not waited for;
+    frames equal the read-back frames bit for bit, with a treelet pass too.  A rebuild into a tree
+    deeper than the bound renders nothing and the next synchronisation reports it; with the read-back
+    back on, the deep tree renders."""
+    say("case height read-back off")
+    sc = _scenes()
+    n, lc = len(sc[0][0]), sc[0][1]
+    cam = sc[0][2]
+    pt.update_camera(cam)
+    d_cam = ctx.to_device(cam)
+    d_sh = ctx.to_device(sc[0][0])
+    bvh = pt.CLBVH(ctx, d_sh, n)
+    bvh.set_height_readback(False)
+
+    def frame(shapes_src, treelet):
+        d_sh.upload(shapes_src)
+        bvh.reset()
+        bvh.buildRadixTree()
+        bvh.buildBVHTree()
+        if treelet:
+            bvh.optimize(treelet)
+        col = ctx.alloc(W * H * 12, F32).zero()
+        seeds = ctx.to_device(default_seeds(W, H))
+        pix = ctx.alloc(W * H * 4, np.uint32).upload(np.full(W * H, 0xDEADBEEF, np.uint32))
+        pt.render(ctx, bvh, None, colors=col, seeds=seeds, camera=d_cam, shapes=d_sh, shapeCnt=n, lightCnt=lc,
+                  width=W, height=H, currentSample=0, spp=SPP, max_depth=DEPTH, pixels=pix)
+        return col, seeds, pix
+
+    for treelet in (0, 5):
+        refs = [_serial(ctx, s, lc, cam, treelet) for s, _, _ in sc]
+        for k in range(4):
+            col, seeds, pix = frame(sc[k % 3][0], treelet)
+            ctx.synchronize()
+            got = (col.download(F32), seeds.download(np.uint32), pix.download(np.uint32))
+            for x, y in zip(got, refs[k % 3]):
+                assert x.tobytes() == y.tobytes(), (treelet, k)
+    chain, lc2, _ = scenes.morton_chain(n - 2, seed=5, width=W, height=H)
+    assert lc2 == lc
+    _, _, pix = frame(chain, 0)
+    with pytest.raises(pt.PtgpuError, match="deeper than the traversal stack"):
+        ctx.synchronize()
+    assert (pix.download(np.uint32) == 0xDEADBEEF).all()
+    bvh.set_height_readback(True)
+    col, seeds, pix = frame(chain, 0)
+    ctx.synchronize()
+    want = _serial(ctx, chain, lc, cam, 0)
+    assert pix.download(np.uint32).tobytes() == want[2].tobytes()
+    bvh.close()
