@@ -108,7 +108,7 @@ __device__ bool recorded_jobs_unfit(const RenderParams &P) {
         if (!hc) continue;
         const unsigned err = hc[0], h = hc[2];
         if (err == 0u && h <= (unsigned)P.depth_cap) continue;
-        s_unfit = 1;
+        atomicOr(&s_unfit, 1);
         if (blockIdx.x == 0) {
             atomicOr(P.status, err ? 1u : 2u);
             atomicMax(P.status + 1, h);
