@@ -122,7 +122,10 @@ int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine);
  * (engine 3's node format: 0 binary, 1 four-wide fp32, 2 four-wide with binary16 boxes rounded
  * outward, half the bytes per node, where the tree's leaves allow it; default 2), "quad" (0..16,
  * with wide 2: when at most this many lanes of a wave are traversing, four lanes share each query,
- * one child box and leaf each; default 16, 0 off);
+ * one child box and leaf each; default 16, 0 off); "stack" (0..64, the smallest traversal stack
+ * class a launch uses: the launch takes the first class (24 or 64 entries for binary nodes, 32, 48 or 64
+ * for four-wide nodes) that holds both this many entries and the tree's height; default 0, the height
+ * alone decides; a larger class only costs LDS, so this is a test knob for every kernel instance);
  * "batch_regroup" and "batch_blocks_per_cu" (the same for ptgpu_render_batch, defaults 32 and 3). */
 int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value);
 int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);

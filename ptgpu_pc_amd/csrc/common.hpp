@@ -195,6 +195,7 @@ struct ptgpu_ctx {
     int blocks_per_cu = 2;        // single-frame renders: few waves shorten each pixel's chain
     int batch_blocks_per_cu = 3;  // batched renders: throughput-bound, more waves per SIMD
     int wide = 2;        // per-lane-state megakernel: 0 binary nodes, 1 four-wide, 2 four-wide binary16 (when they exist)
+    int stack_min = 0;   // smallest traversal stack class a render or query launch uses (0: the tree's height decides)
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;

@@ -1443,9 +1443,11 @@ int convert_kd_ref_into(ptgpu_ctx *ctx, const Shape *d_shapes, int n, const KDNo
     return PTGPU_OK;
 }
 
-// four-wide walk: at most three pushes per visit and one visit per two binary levels
-static int stack_for_wide(const DevAccel &acc) {
-    const int need = 3 * ((acc.depth + 2) / 2) + 1;
+// four-wide walk: at most three pushes per visit and one visit per two binary levels; `least` raises
+// the class for tests that run every stack instance on a shallow tree
+static int stack_for_wide(const DevAccel &acc, int least) {
+    int need = 3 * ((acc.depth + 2) / 2) + 1;
+    if (need < least) need = least;
     if (need <= 32) return 32;
     if (need <= 48) return 48;   // 48 KB of LDS per workgroup: three workgroups per CU
     if (need <= 64) return 64;
@@ -1456,10 +1458,10 @@ static int stack_for_wide(const DevAccel &acc) {
 static int wide_depth_cap(int stack) { return 2 * ((stack - 1) / 3) - 1; }
 static int binary_depth_cap(int stack) { return stack - 2; }
 
-static int stack_for(const DevAccel &acc) {
-    if (acc.mode == PTGPU_ACCEL_NONE) return 24;
-    if (acc.depth + 2 <= 24) return 24;
-    if (acc.depth + 2 <= 64) return 64;
+static int stack_for(const DevAccel &acc, int least) {
+    const int need = acc.mode == PTGPU_ACCEL_NONE ? 0 : acc.depth + 2;
+    if (need <= 24 && least <= 24) return 24;
+    if (need <= 64) return 64;
     return -1;
 }
 
@@ -1475,7 +1477,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     PT_CHECK(!(acc.height_check && ctx->engine == 2), PTGPU_ERR_UNSUPPORTED,
              "wavefront engine: a traversal recorded before the tree height was read back");
     if (wf_ok && ctx->engine == 2) return launch_render_wavefront(ctx, acc, L);
-    const int st = stack_for(acc);
+    const int st = stack_for(acc, ctx->stack_min);
     PT_CHECK(st > 0, PTGPU_ERR_UNSUPPORTED, "tree depth %d exceeds the 62-level traversal stack", acc.depth);
     unsigned *work_counter = L.work_counter;
     if (!work_counter) {
@@ -1524,7 +1526,7 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
     // auto and engine 3: per-lane states whenever there is a tree to walk (measured fastest at every
     // size tried); brute force and single-leaf scenes keep the lockstep megakernel
     if ((ctx->engine == 0 || ctx->engine == 3) && acc.mode != PTGPU_ACCEL_NONE && acc.root >= 0) {
-        const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc) : -1;
+        const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc, ctx->stack_min) : -1;
         const bool half = ctx->wide == 2 && acc.nodes4h;
         if (sw > 0) P.depth_cap = wide_depth_cap(sw);
 #define PT_LANES(ST, STATS_, WIDE_, HALF_)                                                                      \
@@ -1567,7 +1569,7 @@ int launch_trace(ptgpu_ctx *ctx, const DevAccel &acc, unsigned int shapeCnt, con
     if (n <= 0) return PTGPU_OK;
     PT_CHECK(!acc.height_check, PTGPU_ERR_UNSUPPORTED,
              "batch queries cannot be recorded before the tree height was read back");
-    const int st = stack_for(acc);
+    const int st = stack_for(acc, ctx->stack_min);
     PT_CHECK(st > 0, PTGPU_ERR_UNSUPPORTED, "tree depth %d exceeds the 62-level traversal stack", acc.depth);
     if (st == 24)
         hipLaunchKernelGGL(k_trace<24>, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, acc, d_rays, d_maxt, n, d_t,
@@ -1956,7 +1958,7 @@ int launch_batch(ptgpu_ctx *ctx, BatchChannel &ch, const ptgpu_render_job *jobs,
         RenderJob &R = table[(size_t)j];
         PT_TRY(resolve_accel(ctx, J.bvh, J.kd, a->shapes, a->shapeCnt, &R.acc));
         PT_CHECK(R.acc.root >= 0 && R.acc.nodes4, PTGPU_ERR_UNSUPPORTED, "job %d: the tree has no internal node", j);
-        const int sw = stack_for_wide(R.acc);
+        const int sw = stack_for_wide(R.acc, ctx->stack_min);
         PT_CHECK(sw > 0, PTGPU_ERR_UNSUPPORTED, "job %d: tree height %d exceeds the traversal stack", j, R.acc.depth);
         st = sw > st ? sw : st;
         jst[(size_t)j] = sw;

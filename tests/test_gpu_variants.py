@@ -1,0 +1,145 @@
+"""Every instance of the render and query kernels, each checked against the oracle.
+
+A launch picks its kernel instance from the tree's height (the traversal stack class), the node
+format (tuning "wide": binary, four-wide fp32, four-wide binary16), whether traversal statistics are
+gathered, the engine, and whether frames are batched.  A small tree only reaches the smallest stack
+class, so the "stack" tuning raises the class and one frame is rendered by every instance:
+
+* the lockstep megakernel over binary nodes, stacks 24 and 64, with and without statistics;
+* the per-lane-state megakernel over binary nodes (stacks 24, 64) and over four-wide fp32 and
+  binary16 nodes (stacks 32, 48, 64), with and without statistics;
+* the batched instances (stacks 32, 48, 64; fp32 and binary16 nodes);
+* the ray queries (stacks 24, 64).
+
+Each frame equals the oracle's frame (RadianceGPU over the reference's LBVH walk, rendering_kernel.cl:
+969-1150) bit for bit in colours and seeds, packed pixels within one step of powf.  The statistics
+(box and primitive tests) of one node format do not depend on the stack class, since the stack size
+does not change the walk.
+"""
+import numpy as np
+import pytest
+
+import ptgpu_pc_amd as pt
+import pyoracle as O
+from ptgpu_pc_amd import scenes
+from ptgpu_pc_amd.layout import default_seeds
+
+from test_gpu_parity import (assert_closest_parity, gpu_lbvh, gpu_trace, image_checks, random_rays,
+                             render_oracle, say)
+
+pytestmark = pytest.mark.gpu
+F32 = np.float32
+W, H, SPP, DEPTH = 40, 32, 3, 5
+
+
+@pytest.fixture(scope="module")
+def frame(ctx):
+    shapes, lc, cam = scenes.soup(3000, seed=4, width=W, height=H)
+    pt.update_camera(cam)
+    s, bvh = gpu_lbvh(ctx, shapes)
+    _, _, btn, btl = O.build_lbvh(shapes)
+    ref = render_oracle(s, lc, cam, W, H, SPP, DEPTH, O.accel_bvh(btn, btl))
+    yield dict(s=s, lc=lc, cam=cam, bvh=bvh, d_sh=ctx.to_device(s), d_cam=ctx.to_device(cam), ref=ref)
+    bvh.close()
+
+
+def _tuned(ctx, engine, wide, stack):
+    ctx.set_engine(engine)
+    ctx.set_tuning("wide", wide)
+    ctx.set_tuning("stack", stack)
+
+
+def _restore(ctx):
+    ctx.set_engine("auto")
+    ctx.set_tuning("wide", 2)
+    ctx.set_tuning("stack", 0)
+
+
+def _render(ctx, f, stats):
+    col = ctx.alloc(W * H * 12, F32).zero()
+    seeds = ctx.to_device(default_seeds(W, H))
+    pix = ctx.alloc(W * H * 4, np.uint32)
+    st = ctx.alloc(16, np.uint64).zero() if stats else None
+    pt.render(ctx, f["bvh"], None, colors=col, seeds=seeds, camera=f["d_cam"], shapes=f["d_sh"],
+              shapeCnt=len(f["s"]), lightCnt=f["lc"], width=W, height=H, currentSample=0, spp=SPP,
+              max_depth=DEPTH, pixels=pix, trav_stats=st)
+    ctx.synchronize()
+    return (col.download(F32), seeds.download(np.uint32), pix.download(np.uint32),
+            st.download(np.uint64) if stats else None)
+
+
+# (engine, wide, stack classes): stack 0 lets the height pick (the tree here fits the smallest class)
+GROUPS = [("megakernel", 0, (0, 64)), ("lanes", 0, (0, 64)), ("lanes", 1, (0, 48, 64)), ("lanes", 2, (0, 48, 64))]
+
+
+@pytest.mark.parametrize("engine,wide,stacks", GROUPS, ids=[f"{e}-wide{w}" for e, w, _ in GROUPS])
+def test_every_render_instance_vs_oracle(ctx, frame, engine, wide, stacks):
+    counts = {}
+    try:
+        for stack in stacks:
+            _tuned(ctx, engine, wide, stack)
+            for stats in (False, True):
+                say("case variant", engine, "wide", wide, "stack", stack, "stats", stats)
+                c, s, p, st = _render(ctx, frame, stats)
+                image_checks(c, s, p, *frame["ref"])
+                if stats:
+                    counts[stack] = tuple(int(v) for v in st[:2])
+    finally:
+        _restore(ctx)
+    # one walk whatever the stack class: the same box and primitive test counts
+    assert len(set(counts.values())) == 1, counts
+    assert counts[0][0] > 0 and counts[0][1] > 0
+
+
+@pytest.mark.parametrize("wide", [1, 2])
+def test_every_batch_instance_vs_oracle(ctx, frame, wide):
+    f = frame
+    try:
+        for stack in (0, 48, 64):
+            _tuned(ctx, "auto", wide, stack)
+            say("case batch variant wide", wide, "stack", stack)
+            jobs = []
+            for _ in range(2):
+                jobs.append(dict(bvh=f["bvh"], kd=None, shapes=f["d_sh"], shapeCnt=len(f["s"]), lightCnt=f["lc"],
+                                 width=W, height=H, spp=SPP, currentSample=0, max_depth=DEPTH, camera=f["d_cam"],
+                                 colors=ctx.alloc(W * H * 12, F32).zero(),
+                                 seeds=ctx.to_device(default_seeds(W, H)),
+                                 pixels=ctx.alloc(W * H * 4, np.uint32)))
+            pt.render_batch(ctx, jobs)
+            ctx.synchronize()
+            for j in jobs:
+                image_checks(j["colors"].download(F32), j["seeds"].download(np.uint32),
+                             j["pixels"].download(np.uint32), *f["ref"])
+    finally:
+        _restore(ctx)
+
+
+def test_every_query_instance(ctx, frame):
+    """Closest-hit and any-hit queries with stacks 24 and 64: the same bits, and both equal to brute
+    force (the oracle's all-primitive loop) on ids and t."""
+    f = frame
+    rays = random_rays(3000, 9)
+    t_o, id_o = O.trace_closest(f["s"], O.accel_none(), rays)
+    maxt = np.where(id_o >= 0, t_o * F32(1.5), F32(0.5)).astype(F32)
+    h_o = O.trace_any(f["s"], rays, maxt)
+    got = []
+    try:
+        for stack in (0, 64):
+            _tuned(ctx, "auto", 2, stack)
+            say("case query variant stack", stack)
+            t_g, id_g = gpu_trace(ctx, f["bvh"], None, f["d_sh"], len(f["s"]), rays)
+            h_g = gpu_trace(ctx, f["bvh"], None, f["d_sh"], len(f["s"]), rays, maxt)
+            assert_closest_parity(t_g, id_g, t_o, id_o)
+            assert (h_g == h_o).mean() >= 0.9999
+            got.append((t_g.tobytes(), id_g.tobytes(), h_g.tobytes()))
+    finally:
+        _restore(ctx)
+    assert (id_o >= 0).sum() > 100
+    assert got[0] == got[1]
+
+
+def test_stack_tuning_range(ctx):
+    for bad in (-1, 65):
+        with pytest.raises(pt.PtgpuError, match="stack"):
+            ctx.set_tuning("stack", bad)
+    ctx.set_tuning("stack", 0)
