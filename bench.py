@@ -1038,6 +1038,8 @@ def main():
     ap.add_argument("--regroup", type=int, default=None)
     ap.add_argument("--blocks-per-cu", type=int, default=None)
     ap.add_argument("--wide", type=int, default=None)
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="a launch tuning value (ptgpu_ctx_set_tuning) for every render context, e.g. split=0")
     ap.add_argument("--treelet", type=int, default=0, help="treelet size for CLBVH::optimize (0: no optimize)")
     ap.add_argument("--treelet-passes", type=int, default=1)
     ap.add_argument("--serve", action="store_true", help="only the served vs plain comparison (a JSON line)")
@@ -1121,6 +1123,9 @@ def main():
         ctx.set_tuning("blocks_per_cu", args.blocks_per_cu)
     if args.wide is not None:
         ctx.set_tuning("wide", args.wide)
+    extra = [(kv.split("=")[0], int(kv.split("=")[1])) for kv in args.set]
+    for k, v in extra:
+        ctx.set_tuning(k, v)
     d_shapes = ctx.to_device(shapes)
     d_cam = ctx.to_device(cam)
     d_col = ctx.alloc(W * H * 12, np.float32).zero()
@@ -1204,6 +1209,8 @@ def main():
                                               "wide": args.wide}[k]
                 if v:
                     r1.set_tuning(k, int(v))
+            for k, v in extra:
+                r1.set_tuning(k, v)
             rctx.append(r1)
             fbuf.append((r1.alloc(W * H * 12, np.float32).zero(), r1.alloc(W * H * 8, np.uint32),
                          r1.alloc(W * H * 4, np.uint32)))
