@@ -1156,6 +1156,11 @@ def main():
                              max_depth=DEPTH, pixels=d_pix)
         if not args.treelet and tuning["treelet"]:
             treelet, passes = tuning["treelet"], 1
+    # --set values win over the replayed or autotuned ones (the line's tuning object shows them)
+    for k, v in extra:
+        ctx.set_tuning(k, v)
+        if tuning is not None and k in tuning:
+            tuning[k] = v
 
     def step(ray_counter=None, trav_stats=None):
         ctx.record(0)
