@@ -48,6 +48,16 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0, w=None, h=None, depth
     rays), built from the reference sources into oracle/_ref/, timed on this host's cores on a
     pixel subset; its query count comes from the oracle in host-twin semantics, which follows the
     same paths bit for bit.  Falls back to the oracle itself when oracle/_ref was not built."""
+    rays, elapsed, total_pix, kind, threads = cpu_render_rate(shapes, lc, cam, btn, btl, budget_s, w, h, depth)
+    W, H = (w or globals()["W"]), (h or globals()["H"])
+    what = what or f"the same {W}x{H} frame"
+    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
+            "sample": f"{total_pix} pixels x 1 spp of {what} (BVH closest hit, brute-force shadow rays over "
+                      f"{len(shapes):,} shapes), {elapsed:.1f} s"}
+
+
+def cpu_render_rate(shapes, lc, cam, btn, btl, budget_s, w=None, h=None, depth=None):
+    """(queries, seconds, pixels, kind, threads) of the host render path on pixel samples of one frame."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
     from ptgpu_pc_amd.layout import default_seeds
@@ -80,10 +90,7 @@ def cpu_baseline(shapes, lc, cam, btn, btl, budget_s=12.0, w=None, h=None, depth
         rays += q[0] + q[1]
         total_pix += npix
         npix = min(npix * 2, 4096)
-    what = what or f"the same {W}x{H} frame"
-    return {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": kind,
-            "sample": f"{total_pix} pixels x 1 spp of {what} (BVH closest hit, brute-force shadow rays over "
-                      f"{len(shapes):,} shapes), {elapsed:.1f} s"}
+    return rays, elapsed, total_pix, kind, threads
 
 
 RENDER_KERNEL = "k_render_lanes"
@@ -565,7 +572,43 @@ def batch_workload(n_scenes: int = 64, seed: int = 4000):
     return scenes.batch_set(n_scenes, seed)
 
 
-def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: bool = True, record: bool = True):
+def batch_cpu_baseline(W, render_scenes: int = 6, budget_s: float = 1.5):
+    """The many-scene workload on the host: every scene's LBVH as the oracle restates the reference's
+    host construction (CLBVH.cpp:90-164 Morton codes, BVH.cl:141-258 topology and refit) with
+    std::stable_sort in place of the O(N^2) bubble sort (CLBVH.cpp:31-47, about 10^12 steps at this
+    size), single-threaded, summed over the scenes; and the reference's host render path on a pixel
+    sample of every len(W) // render_scenes-th scene, its rays over its seconds summed over them."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    build_s, n_total, picked = 0.0, 0, []
+    step = max(1, len(W) // render_scenes)
+    for k, w in enumerate(W):
+        t0 = time.perf_counter()
+        s_h, _, btn, btl = O.build_lbvh(w["shapes"])
+        build_s += time.perf_counter() - t0
+        n_total += len(w["shapes"])
+        if k % step == 0 and len(picked) < render_scenes:
+            picked.append((s_h, w, btn, btl))
+    rays = secs = 0.0
+    pix = 0
+    kind, cores = "port", 1
+    for s_h, w, btn, btl in picked:
+        q, sec, npx, kind, cores = cpu_render_rate(s_h, w["lc"], w["cam"], btn, btl, budget_s, w["w"], w["h"], 5)
+        rays += q
+        secs += sec
+        pix += npx
+    sizes = ", ".join(f"{len(s_h):,}" for s_h, _, _, _ in picked)
+    return {"build": {"value": round(n_total / build_s / 1e6, 3), "unit": "Mtri/s", "cores": 1, "kind": "port",
+                      "sample": f"the LBVH of each of the {len(W)} scenes ({n_total:,} shapes) with std::stable_sort "
+                                f"for the bubble sort, {build_s:.2f} s"},
+            "render": {"value": round(rays / secs / 1e6, 4) if secs else None, "unit": "Mrays/s", "cores": cores,
+                       "kind": kind,
+                       "sample": f"{pix} pixels x 1 spp over {len(picked)} of the scenes ({sizes} shapes; BVH "
+                                 f"closest hit, brute-force shadow rays), {secs:.1f} s"}}
+
+
+def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: bool = True, record: bool = True,
+                  cpu: bool = False):
     """Plain path (per scene: LBVH build, then one fused 4-spp render launch) against the batched
     path (every scene's build in one batched sequence of launches, every frame in one render
     launch) on the same resident scenes.  Both rebuild every tree and re-render every frame per step."""
@@ -660,6 +703,7 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: boo
     for h in plain_h + batch_h:
         h.close()
     ctx.close()
+    cpu_res = batch_cpu_baseline(W) if cpu else None
     alg = BUILD_BYTES_PER_PRIM * total_n
     nfr = len(W)
     return {"scenes": nfr, "shapes": total_n, "soup_scenes": sum(1 for w in W if w["kind"] == 0),
@@ -672,7 +716,7 @@ def batch_compare(steps: int = 3, warmup: int = 1, n_scenes: int = 64, tune: boo
                         "build_alg_gbs": round(alg / (b_ms * 1e-3) / 1e9, 1)},
             "batched_recorded": recorded,
             "frames_speedup": round(t_plain / t_batch, 2), "build_speedup": round(plain_build_ms / b_ms, 2),
-            "pixels_identical": same, "tuning": tuning,
+            "pixels_identical": same, "tuning": tuning, "cpu_baseline": cpu_res,
             "workload": f"{nfr} scenes, triangles log-uniform [1e3, 1e6] (soup or displaced icosphere by seed), "
                         "orbit cameras, 256^2/512^2/1024^2 by seed, 4 spp, depth 5; every tree rebuilt and every "
                         "frame rendered per step"}
@@ -1371,7 +1415,7 @@ def main():
 
     batched = None
     if not args.no_batch:
-        batched = batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune)
+        batched = batch_compare(n_scenes=args.batch_scenes, tune=not args.no_tune, cpu=not args.no_cpu)
 
     c1 = None
     if not args.no_c1:
