@@ -60,7 +60,7 @@ struct RenderParams {
     int njobs;
     const int2 *tiles;            // tiled launches (RadianceBoxGPU): tile origins; NULL renders the whole frame
     int tile_w, tile_h;
-    float one;                    // 1.0f, opaque to the compiler (slab_h's mixed-precision differences)
+    float one;                    // 1.0f, opaque to the compiler (slab_hn's mixed-precision differences)
     int depth_cap;                // deepest tree the launch's stack holds (checked when acc.height_check is set)
     unsigned *status;             // the context's check words: [0] flags, [1] deepest height seen
 };
@@ -350,6 +350,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
     unsigned s0 = 0, s1 = 0;
     f3 col = mk(0.f, 0.f, 0.f), ro = col, rd = mk(0.f, 0.f, 1.f), rad = col, thr = mk(1.f, 1.f, 1.f);
     f3 nl = col, Ld = col, contrib = col, inv = col;
+    RaySlab rsl = ray_slab(mk(1.f, 1.f, 1.f));
     float tb = kInf;
     int bid = -1, node = 0, sp = 0;
     int pl = -1;   // binary16 walk: first primitive of a leaf run waiting to be tested (-1 none)
@@ -397,6 +398,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
         tb = any ? maxt : kInf;
         bid = -1;
         inv = mk(1.0f / rd.x, 1.0f / rd.y, 1.0f / rd.z);
+        if (HALF) rsl = ray_slab(inv);
         node = t_root;
         pl = -1;
         sp = 0;
@@ -439,6 +441,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                 q_o.x = __shfl(ro.x, owner); q_o.y = __shfl(ro.y, owner); q_o.z = __shfl(ro.z, owner);
                 q_d.x = __shfl(rd.x, owner); q_d.y = __shfl(rd.y, owner); q_d.z = __shfl(rd.z, owner);
                 q_inv.x = __shfl(inv.x, owner); q_inv.y = __shfl(inv.y, owner); q_inv.z = __shfl(inv.z, owner);
+                const RaySlab q_rs = ray_slab(q_inv);
                 float q_tb = __shfl(tb, owner);
                 int q_bid = __shfl(bid, owner), q_node = __shfl(node, owner), q_sp = __shfl(sp, owner);
                 int q_pl = __shfl(pl, owner);
@@ -507,7 +510,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                 }
                             }
                             const int ref = qhn ? (int)w.w : kEmptyRef;
-                            float tc = slab_h(w.x, w.y, w.z, q_o, q_inv, q_tb, P.one);
+                            float tc = slab_hn(w.x, w.y, w.z, q_o, q_inv, q_rs, q_tb * kSlack, P.one);
                             if (!qhn) tc = inf;
                             if (STATS) st_nodes += ref != kEmptyRef;
                             if (!occl && ref < 0 && tc != inf && ~ref < q_nprims) {
@@ -684,10 +687,11 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     int nn = -1;
                     if (!done && hn) {
                         int r0 = (int)q0.w, r1 = (int)q1.w, r2 = (int)q2.w, r3 = (int)q3.w;
-                        float t0 = slab_h(q0.x, q0.y, q0.z, ro, inv, tb, P.one);
-                        float t1 = slab_h(q1.x, q1.y, q1.z, ro, inv, tb, P.one);
-                        float t2 = slab_h(q2.x, q2.y, q2.z, ro, inv, tb, P.one);
-                        float t3 = slab_h(q3.x, q3.y, q3.z, ro, inv, tb, P.one);
+                        const float tbs = tb * kSlack;
+                        float t0 = slab_hn(q0.x, q0.y, q0.z, ro, inv, rsl, tbs, P.one);
+                        float t1 = slab_hn(q1.x, q1.y, q1.z, ro, inv, rsl, tbs, P.one);
+                        float t2 = slab_hn(q2.x, q2.y, q2.z, ro, inv, rsl, tbs, P.one);
+                        float t3 = slab_hn(q3.x, q3.y, q3.z, ro, inv, rsl, tbs, P.one);
                         if (STATS) st_nodes += (r0 != kEmptyRef) + (r1 != kEmptyRef) + (r2 != kEmptyRef) + (r3 != kEmptyRef);
                         auto cswap = [](float &ta, int &ra, float &tb_, int &rb) {
                             const bool sw = tb_ < ta;

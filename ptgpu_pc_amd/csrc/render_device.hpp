@@ -180,17 +180,43 @@ PT_HD float slab(float lx, float ly, float lz, float hx, float hy, float hz, f3 
 // product by one is exact, so the result is the rounded difference, bit for bit what a conversion
 // and a subtraction give.  `one` is 1.0f held where the compiler cannot see it (a literal would be
 // folded into that conversion and subtraction, two instructions instead of one).
-__device__ __forceinline__ float slab_h(unsigned wx, unsigned wy, unsigned wz, f3 o, f3 inv, float tlimit,
-                                        float one) {
-    const float tx0 = __builtin_fmaf(half_lo(wx), one, -o.x) * inv.x;
-    const float tx1 = __builtin_fmaf(half_hi(wx), one, -o.x) * inv.x;
-    const float ty0 = __builtin_fmaf(half_lo(wy), one, -o.y) * inv.y;
-    const float ty1 = __builtin_fmaf(half_hi(wy), one, -o.y) * inv.y;
-    const float tz0 = __builtin_fmaf(half_lo(wz), one, -o.z) * inv.z;
-    const float tz1 = __builtin_fmaf(half_hi(wz), one, -o.z) * inv.z;
-    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
-    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tlimit));
-    return tn <= tf * kSlack ? tn : __builtin_inff();
+//
+// A ray's side of the test, set once per query (RaySlab): per axis, the rotation that brings the
+// near bound of a bound word into its low half (0 when the reciprocal direction is >= 0, where lo is
+// nearer; 16 otherwise), and the reciprocal for the far bound scaled by the slack.  Choosing near and
+// far by the direction's sign gives the values a min and a max of the two distances would (rounding
+// is monotone), with no min/max pair per axis.  The slack applied to the far reciprocal and to the
+// limit, rather than to the exit distance as slab() does, differs by rounding far below the slack
+// itself, so the test stays conservative.  Where a distance is NaN (a ray in a bound's plane with an
+// infinite reciprocal) the test ignores that axis's bound, which only accepts more boxes.
+struct RaySlab {
+    unsigned rx, ry, rz;
+    f3 inv_far;
+};
+__device__ __forceinline__ RaySlab ray_slab(f3 inv) {
+    RaySlab r;
+    r.rx = inv.x >= 0.f ? 0u : 16u;
+    r.ry = inv.y >= 0.f ? 0u : 16u;
+    r.rz = inv.z >= 0.f ? 0u : 16u;
+    r.inv_far = mk(inv.x * kSlack, inv.y * kSlack, inv.z * kSlack);
+    return r;
+}
+
+// tlimit_s: the query's limit times kSlack
+__device__ __forceinline__ float slab_hn(unsigned wx, unsigned wy, unsigned wz, f3 o, f3 inv, const RaySlab &rs,
+                                         float tlimit_s, float one) {
+    const unsigned nx = __builtin_amdgcn_alignbit(wx, wx, rs.rx);
+    const unsigned ny = __builtin_amdgcn_alignbit(wy, wy, rs.ry);
+    const unsigned nz = __builtin_amdgcn_alignbit(wz, wz, rs.rz);
+    const float tnx = __builtin_fmaf(half_lo(nx), one, -o.x) * inv.x;
+    const float tfx = __builtin_fmaf(half_hi(nx), one, -o.x) * rs.inv_far.x;
+    const float tny = __builtin_fmaf(half_lo(ny), one, -o.y) * inv.y;
+    const float tfy = __builtin_fmaf(half_hi(ny), one, -o.y) * rs.inv_far.y;
+    const float tnz = __builtin_fmaf(half_lo(nz), one, -o.z) * inv.z;
+    const float tfz = __builtin_fmaf(half_hi(nz), one, -o.z) * rs.inv_far.z;
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tlimit_s));
+    return tn <= tf ? tn : __builtin_inff();
 }
 
 // Closest hit over a prepared structure.  Returns the hit shape index or -1; t in *tout.
