@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scan(unsigned *hist) {
     sort_scan_body<BITS>(hist);
 }
 
-template <int BITS>
+template <int BITS, int IN, int OUT>
 __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const unsigned *__restrict__ keys_in,
                                                             const unsigned *__restrict__ vals_in,
                                                             unsigned *__restrict__ keys_out,
@@ -90,8 +90,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_pass(const unsigned *__re
     __shared__ unsigned s_tile;
     if (threadIdx.x == 0) s_tile = atomicAdd(tile_counter, 1u);
     __syncthreads();
-    sort_pass_body<BITS>(keys_in, vals_in, keys_out, vals_out, n, shift, digit_base, status,
-                         __builtin_amdgcn_readfirstlane(s_tile), err);
+    sort_pass_body<BITS, IN, OUT>(keys_in, vals_in, keys_out, vals_out, n, shift, digit_base, status,
+                                  __builtin_amdgcn_readfirstlane(s_tile), err);
 }
 
 __global__ __launch_bounds__(256) void k_karras(int n, BVHNodeGPU *btn, BVHNodeGPU *btl) {
@@ -145,9 +145,8 @@ static int clbvh_alloc(ptgpu_clbvh *h, int n) {
     const size_t N = (size_t)n;
     const size_t tiles = (N + kSortTile - 1) / kSortTile;
     PT_TRY(h->keysA.ensure(N * 4));
-    PT_TRY(h->keysB.ensure(N * 4));
     PT_TRY(h->valsA.ensure(N * 4));
-    PT_TRY(h->valsB.ensure(N * 4));
+    PT_TRY(h->pairs.ensure(N * 16));   // the sort's packed (key, value) words, two buffers of N
     PT_TRY(h->btn.ensure((N > 1 ? N - 1 : 1) * sizeof(BVHNodeGPU)));
     PT_TRY(h->btl.ensure(N * sizeof(BVHNodeGPU)));
     PT_TRY(h->tnodes.ensure((N > 1 ? N - 1 : 1) * sizeof(TNode)));
@@ -318,18 +317,31 @@ static int clbvh_sort(ptgpu_clbvh *h) {
     PT_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_sort_scan<BITS>, dim3(1), dim3(kSortThreads), 0, st, hist);
     PT_HIP(hipGetLastError());
-    unsigned *kin = h->keysA.as<unsigned>(), *vin = nullptr;
-    unsigned *kout = h->keysB.as<unsigned>(), *vout = h->valsB.as<unsigned>();
+    // The first pass reads the Morton keys (values: the shape indices) and writes packed (key, value)
+    // words; the middle passes go between the two packed buffers; the last writes the sorted keys back
+    // over the Morton keys and the sorted values beside them (an even number of passes).
+    static_assert(C::kPasses % 2 == 0, "the passes end in the packed buffer the first one wrote");
+    unsigned *kA = h->keysA.as<unsigned>(), *vA = h->valsA.as<unsigned>();
+    unsigned *pk[2] = {h->pairs.as<unsigned>(), h->pairs.as<unsigned>() + 2 * (size_t)n};
     for (int p = 0; p < C::kPasses; p++) {
-        hipLaunchKernelGGL(k_sort_pass<BITS>, dim3((unsigned)tiles), dim3(kSortThreads), 0, st, kin, vin, kout, vout, n,
-                           p * BITS, hist + p * C::kRadix, status + (size_t)p * tiles * C::kRadix, ctr + 1 + p,
-                           ctr + 9);
+        const unsigned *pst = status + (size_t)p * tiles * C::kRadix;
+        unsigned *pst_w = status + (size_t)p * tiles * C::kRadix;
+        (void)pst;
+        const dim3 g((unsigned)tiles), b(kSortThreads);
+        if (p == 0)
+            hipLaunchKernelGGL((k_sort_pass<BITS, kSplit, kPacked>), g, b, 0, st, kA, (const unsigned *)nullptr, pk[0],
+                               (unsigned *)nullptr, n, 0, hist, pst_w, ctr + 1, ctr + 9);
+        else if (p == C::kPasses - 1)
+            hipLaunchKernelGGL((k_sort_pass<BITS, kPacked, kSplit>), g, b, 0, st, pk[(p - 1) & 1],
+                               (const unsigned *)nullptr, kA, vA, n, p * BITS, hist + p * C::kRadix, pst_w,
+                               ctr + 1 + p, ctr + 9);
+        else
+            hipLaunchKernelGGL((k_sort_pass<BITS, kPacked, kPacked>), g, b, 0, st, pk[(p - 1) & 1],
+                               (const unsigned *)nullptr, pk[p & 1], (unsigned *)nullptr, n, p * BITS,
+                               hist + p * C::kRadix, pst_w, ctr + 1 + p, ctr + 9);
         PT_HIP(hipGetLastError());
-        unsigned *tk = kin, *tv = vin;
-        kin = kout; vin = vout;
-        kout = tk; vout = (tv ? tv : h->valsA.as<unsigned>());
     }
-    h->sorted = vin;
+    h->sorted = vA;
     return PTGPU_OK;
 }
 
@@ -639,7 +651,7 @@ int ptgpu_clbvh_device_trees(ptgpu_clbvh *h, const BVHNodeGPU **d_btn, const BVH
 
 int ptgpu_clbvh_device_bytes(ptgpu_clbvh *h, size_t *bytes) {
     PT_CHECK(h && bytes, PTGPU_ERR_INVALID, "null argument");
-    const DevBuf *bufs[] = {&h->own_shapes, &h->keysA, &h->keysB, &h->valsA, &h->valsB, &h->btn, &h->btl,
+    const DevBuf *bufs[] = {&h->own_shapes, &h->keysA, &h->pairs, &h->valsA, &h->btn, &h->btl,
                             &h->tnodes, &h->prims, &h->hist, &h->status, &h->misc, &h->lights, &h->scan,
                             &h->opt_tmp, &h->mnode, &h->mpar, &h->gbi, &h->xlist, &h->tnodes4, &h->treelet,
                             &h->boxes};
@@ -687,7 +699,7 @@ int ptgpu_clbvh_destroy(ptgpu_clbvh *h) {
     if (!h) return PTGPU_OK;
     if (h->ctx) PT_NO_CAPTURE(h->ctx, "ptgpu_clbvh_destroy");
     if (h->ctx && h->ctx->stream) (void)hipStreamSynchronize(h->ctx->stream);
-    h->keysA.release(); h->keysB.release(); h->valsA.release(); h->valsB.release();
+    h->keysA.release(); h->pairs.release(); h->valsA.release();
     h->btn.release(); h->btl.release(); h->tnodes.release(); h->prims.release();
     h->hist.release(); h->status.release(); h->misc.release(); h->own_shapes.release();
     h->lights.release(); h->scan.release(); h->opt_tmp.release();

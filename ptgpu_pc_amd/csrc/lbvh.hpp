@@ -47,7 +47,7 @@ struct ptgpu_clbvh {
     ptgpu::DevBuf own_shapes;
     ptgpu::DevBuf boxes;                     // per-shape box records (input order) for the build's passes
     bool radix_pending = false;              // btn/btl topology not yet written (the refit writes it)
-    ptgpu::DevBuf keysA, keysB, valsA, valsB;
+    ptgpu::DevBuf keysA, valsA, pairs;
     ptgpu::DevBuf btn, btl, tnodes, prims;
     ptgpu::DevBuf hist, status, misc;
     ptgpu::DevBuf lights, scan, opt_tmp;

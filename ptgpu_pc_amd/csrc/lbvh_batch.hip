@@ -25,7 +25,8 @@ namespace {
 struct BuildSeg {
     Shape *shapes;
     ShapeBox *boxes;
-    unsigned *keysA, *keysB, *valsA, *valsB;
+    unsigned *keysA, *valsA;
+    unsigned *pairs;        // the sort's packed (key, value) words: two buffers of n
     unsigned *hist, *status;
     unsigned *ctr;          // counters of the handle's misc block (see ptgpu_clbvh::counters)
     float *partials, *scene6;
@@ -112,9 +113,10 @@ __global__ __launch_bounds__(kSortThreads) void kb_sort_scan(const BuildSeg *g) 
     sort_scan_body<BITS>(g[blockIdx.x].hist);
 }
 
-// Pass p of every scene's sort.  Keys and values ping-pong A -> B -> A ...: pass 0 reads the
-// Morton keys in keysA with implicit values 0..N-1, and the last (odd) pass lands in keysA/valsA.
-template <int BITS>
+// Pass p of every scene's sort.  Pass 0 reads the Morton keys in keysA with implicit values
+// 0..N-1 and writes packed (key, value) words; the middle passes go between the two packed buffers;
+// the last (odd) pass writes the sorted keys and values to keysA/valsA.
+template <int BITS, int IN, int OUT>
 __global__ __launch_bounds__(kSortThreads) void kb_sort_pass(const BuildSeg *g, const int *starts, int ns, int p,
                                                              unsigned *tile_counter) {
     using C = SortCfg<BITS>;
@@ -124,13 +126,11 @@ __global__ __launch_bounds__(kSortThreads) void kb_sort_pass(const BuildSeg *g, 
     const int t = (int)__builtin_amdgcn_readfirstlane(s_tile);
     const int s = seg_find(starts, ns, t);
     const BuildSeg &S = g[s];
-    const bool odd = p & 1;
-    const unsigned *kin = odd ? S.keysB : S.keysA;
-    const unsigned *vin = p == 0 ? nullptr : odd ? S.valsB : S.valsA;
-    unsigned *kout = odd ? S.keysA : S.keysB;
-    unsigned *vout = odd ? S.valsA : S.valsB;
-    sort_pass_body<BITS>(kin, vin, kout, vout, S.n, p * BITS, S.hist + p * C::kRadix,
-                         S.status + (size_t)p * S.tiles * C::kRadix, (unsigned)(t - starts[s]), S.ctr + 9);
+    unsigned *pk[2] = {S.pairs, S.pairs + 2 * (size_t)S.n};
+    const unsigned *kin = IN == kPacked ? pk[(p - 1) & 1] : S.keysA;
+    unsigned *kout = OUT == kPacked ? pk[p & 1] : S.keysA;
+    sort_pass_body<BITS, IN, OUT>(kin, nullptr, kout, S.valsA, S.n, p * BITS, S.hist + p * C::kRadix,
+                                  S.status + (size_t)p * S.tiles * C::kRadix, (unsigned)(t - starts[s]), S.ctr + 9);
 }
 
 // Pass p of every scene's level refit: levels [p*kRefitLog, min(L, (p+1)*kRefitLog)).
@@ -200,9 +200,8 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
         S.shapes = h->d_shapes;
         S.boxes = h->boxes.as<ShapeBox>();
         S.keysA = h->keysA.as<unsigned>();
-        S.keysB = h->keysB.as<unsigned>();
         S.valsA = h->valsA.as<unsigned>();
-        S.valsB = h->valsB.as<unsigned>();
+        S.pairs = h->pairs.as<unsigned>();
         S.hist = h->hist.as<unsigned>();
         S.status = h->status.as<unsigned>();
         S.ctr = h->counters();
@@ -338,8 +337,15 @@ int build_batch(ptgpu_ctx *ctx, ptgpu_clbvh *const *hs, int ns) {
     PT_BATCH_LAUNCH(kb_sort_hist<BITS>, kStHist, kSortThreads);
     hipLaunchKernelGGL(kb_sort_scan<BITS>, dim3(ns), dim3(kSortThreads), 0, st, d_segs);
     PT_HIP(hipGetLastError());
+    static_assert(C::kPasses % 2 == 0, "the passes end in the packed buffer the first one wrote");
     for (int p = 0; p < C::kPasses; p++) {
-        PT_BATCH_LAUNCH(kb_sort_pass<BITS>, kStTiles, kSortThreads, p, d_ctr + p);
+        if (p == 0) {
+            PT_BATCH_LAUNCH((kb_sort_pass<BITS, kSplit, kPacked>), kStTiles, kSortThreads, p, d_ctr + p);
+        } else if (p == C::kPasses - 1) {
+            PT_BATCH_LAUNCH((kb_sort_pass<BITS, kPacked, kSplit>), kStTiles, kSortThreads, p, d_ctr + p);
+        } else {
+            PT_BATCH_LAUNCH((kb_sort_pass<BITS, kPacked, kPacked>), kStTiles, kSortThreads, p, d_ctr + p);
+        }
     }
     PT_BATCH_LAUNCH(kb_refit<true>, kStRefit0, 256, 0);
     for (int p = 1; p < kMaxRefitPasses; p++) {

@@ -312,14 +312,21 @@ __device__ __forceinline__ void sort_scan_body(unsigned *hist) {
     }
 }
 
-template <int BITS>
+// The element forms of a pass's input and output.  kSplit: a key array and a value array (an input
+// without a value array takes each element's index as its value).  kPacked: one 8-byte word per
+// element, the key in its low half and the value in its high half, read from / written to the
+// keys_in / keys_out pointer (vals_in / vals_out unused): one load and one store per element, and each
+// tile's digit runs are twice as long in bytes as in either split array, so the scatter leaves fewer
+// partly written cache lines.
+enum SortForm { kSplit = 0, kPacked = 1 };
+
+template <int BITS, int IN = kSplit, int OUT = kSplit>
 __device__ __forceinline__ void sort_pass_body(const unsigned *__restrict__ keys_in, const unsigned *__restrict__ vals_in,
                                                unsigned *__restrict__ keys_out, unsigned *__restrict__ vals_out, int n,
                                                int shift, const unsigned *__restrict__ digit_base, unsigned *status,
                                                unsigned tile, unsigned *err) {
     using C = SortCfg<BITS>;
-    __shared__ unsigned s_keys[kSortTile];
-    __shared__ unsigned s_vals[kSortTile];
+    __shared__ unsigned long long s_kv[kSortTile];   // the tile in digit order, key | value << 32
     __shared__ unsigned s_wcnt[kSortWaves][C::kRadix];
     __shared__ unsigned s_lofs[C::kRadix];
     __shared__ unsigned s_gofs[C::kRadix];
@@ -336,8 +343,14 @@ __device__ __forceinline__ void sort_pass_body(const unsigned *__restrict__ keys
     for (int j = 0; j < kSortItems; j++) {
         long long idx = wbase + j * kWave + lane;
         bool valid = idx < n;
-        k[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
-        v[j] = valid ? (vals_in ? vals_in[idx] : (unsigned)idx) : 0u;
+        if (IN == kPacked) {
+            const unsigned long long x = valid ? reinterpret_cast<const unsigned long long *>(keys_in)[idx] : ~0ull;
+            k[j] = (unsigned)x;
+            v[j] = (unsigned)(x >> 32);
+        } else {
+            k[j] = valid ? keys_in[idx] : 0xFFFFFFFFu;
+            v[j] = valid ? (vals_in ? vals_in[idx] : (unsigned)idx) : 0u;
+        }
     }
     const unsigned long long lt = lanemask_lt();
 #pragma unroll
@@ -428,19 +441,23 @@ __device__ __forceinline__ void sort_pass_body(const unsigned *__restrict__ keys
         if (idx < n) {
             unsigned digit = (k[j] >> shift) & C::kMask;
             unsigned lpos = s_lofs[digit] + s_wcnt[w][digit] + rank[j];
-            s_keys[lpos] = k[j];
-            s_vals[lpos] = v[j];
+            s_kv[lpos] = (unsigned long long)k[j] | ((unsigned long long)v[j] << 32);
         }
     }
     __syncthreads();
     long long rem = (long long)n - base;
     const int count = rem < kSortTile ? (int)rem : kSortTile;
     for (int i = tid; i < count; i += kSortThreads) {
-        unsigned key = s_keys[i];
+        const unsigned long long x = s_kv[i];
+        unsigned key = (unsigned)x;
         unsigned digit = (key >> shift) & C::kMask;
         unsigned dest = s_gofs[digit] + (unsigned)i - s_lofs[digit];
-        keys_out[dest] = key;
-        vals_out[dest] = s_vals[i];
+        if (OUT == kPacked) {
+            reinterpret_cast<unsigned long long *>(keys_out)[dest] = x;
+        } else {
+            keys_out[dest] = key;
+            vals_out[dest] = (unsigned)(x >> 32);
+        }
     }
 }
 
