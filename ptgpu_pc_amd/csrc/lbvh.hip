@@ -324,9 +324,7 @@ static int clbvh_sort(ptgpu_clbvh *h) {
     unsigned *kA = h->keysA.as<unsigned>(), *vA = h->valsA.as<unsigned>();
     unsigned *pk[2] = {h->pairs.as<unsigned>(), h->pairs.as<unsigned>() + 2 * (size_t)n};
     for (int p = 0; p < C::kPasses; p++) {
-        const unsigned *pst = status + (size_t)p * tiles * C::kRadix;
         unsigned *pst_w = status + (size_t)p * tiles * C::kRadix;
-        (void)pst;
         const dim3 g((unsigned)tiles), b(kSortThreads);
         if (p == 0)
             hipLaunchKernelGGL((k_sort_pass<BITS, kSplit, kPacked>), g, b, 0, st, kA, (const unsigned *)nullptr, pk[0],
