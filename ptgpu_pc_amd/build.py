@@ -46,8 +46,6 @@ def _run(cmd: list[str]) -> None:
 
 def build_library(jobs: int = 8) -> Path:
     BUILD.mkdir(exist_ok=True)
-    headers = list(CSRC.glob("*.hpp")) + [ROOT / "include" / "ptgpu.h"]
-    newest_header = max(p.stat().st_mtime for p in headers)
     objs = []
     cmds = []
     for src in SOURCES:
