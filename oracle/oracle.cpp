@@ -24,6 +24,10 @@
  *   intersections            closed forms (axis rays, det==0, t==EPSILON) + oracle/_ref bit-exact (sem=1)
  *   or_intersect (modes)     BVH/KD traversal equals brute force on random rays
  *   radiance                 oracle/_ref RadiancePathTracing bit-exact (sem=1), analytic furnace/miss cases
+ *   or_sincos (sem=0)        within one ulp plus the reduction error of the binary64 libm value, and
+ *                            exactly rounded on most arguments (the mathematics, not a retyped formula)
+ *   per-pixel wrapper        UpdateCamera closed forms (yaw/pitch 0 and 90 degrees), GenerateCameraRay
+ *                            by hand, toInt known values, row flip and running mean on an asymmetric scene
  */
 #include <cfloat>
 #include <cmath>
