@@ -126,6 +126,11 @@ int ptgpu_ctx_set_engine(ptgpu_ctx *ctx, int engine);
  * class a launch uses: the launch takes the first class (24 or 64 entries for binary nodes, 32, 48 or 64
  * for four-wide nodes) that holds both this many entries and the tree's height; default 0, the height
  * alone decides; a larger class only costs LDS, so this is a test knob for every kernel instance);
+ * "short_stack" (0..2: with wide 2, a tree that needs the 48-entry stack is walked on a 36-entry one,
+ * so four workgroups fit on a CU instead of three, and a walk that outgrows it is finished against
+ * every primitive, with the same result; 1 the default, 0 off: about 2.6% less time for one frame
+ * alone, 4% less throughput with frames in flight at C2; 2 walks every binary16 tree on an 8-entry
+ * stack, a test setting);
  * "batch_regroup" and "batch_blocks_per_cu" (the same for ptgpu_render_batch, defaults 32 and 3). */
 int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value);
 int ptgpu_malloc(ptgpu_ctx *ctx, size_t bytes, void **d_out);

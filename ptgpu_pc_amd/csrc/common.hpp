@@ -195,7 +195,8 @@ struct ptgpu_ctx {
     int blocks_per_cu = 2;        // single-frame renders: few waves shorten each pixel's chain
     int batch_blocks_per_cu = 3;  // batched renders: throughput-bound, more waves per SIMD
     int wide = 2;        // per-lane-state megakernel: 0 binary nodes, 1 four-wide, 2 four-wide binary16 (when they exist)
-    int stack_min = 0;   // smallest traversal stack class a render or query launch uses (0: the tree's height decides)
+    int stack_min = 0;
+    int short_stack = 1;  // 48-entry binary16 walks on the 36-entry class (0 off; 2 every binary16 walk on 8 entries, a test)   // smallest traversal stack class a render or query launch uses (0: the tree's height decides)
     void *wf_workspace = nullptr;
     // scratch for raw-layout renders and light lists
     ptgpu::DevBuf tmp_nodes, tmp_nodes4, tmp_prims, tmp_lights, tmp_scan, tmp_counter;
@@ -265,6 +266,7 @@ struct RenderLaunch {
     unsigned *work_counter = nullptr;   // device word for the persistent waves' pixel counter; NULL: the context's
     const int2 *tiles = nullptr;        // device tile origins: render only these tile_w x tile_h tiles
     int ntiles = 0, tile_w = 0, tile_h = 0;
+    bool no_alloc = false;   // the allocation-free entry: kernels that need no scratch memory only
 };
 int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L);
 // ptgpu_render's checks and preparation (traversal structure, light list) without the launch

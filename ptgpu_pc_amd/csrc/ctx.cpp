@@ -206,6 +206,9 @@ int ptgpu_ctx_set_tuning(ptgpu_ctx *ctx, const char *name, int value) {
     } else if (!strcmp(name, "wide")) {
         PT_CHECK(value >= 0 && value <= 2, PTGPU_ERR_INVALID, "wide %d not in {0, 1, 2}", value);
         ctx->wide = value;
+    } else if (!strcmp(name, "short_stack")) {
+        PT_CHECK(value >= 0 && value <= 2, PTGPU_ERR_INVALID, "short_stack %d not in {0, 1, 2}", value);
+        ctx->short_stack = value;
     } else if (!strcmp(name, "stack")) {
         PT_CHECK(value >= 0 && value <= 64, PTGPU_ERR_INVALID, "stack %d not in [0, 64]", value);
         ctx->stack_min = value;

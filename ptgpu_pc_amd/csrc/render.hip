@@ -65,6 +65,13 @@ struct RenderParams {
     unsigned *status;             // the context's check words: [0] flags, [1] deepest height seen
 };
 
+// The short traversal-stack class of the binary16 walk: 36 entries per lane, so a workgroup's stack
+// takes 36 KB of LDS and four workgroups fit on a CU (three with the 48-entry class).  A walk that
+// would push past it marks kOverflow in its visit count and is finished against every primitive.
+constexpr int kStackShort = 36;
+constexpr int kStackTiny = 8;   // a test class: nearly every walk outgrows it (tuning short_stack 2)
+constexpr unsigned kOverflow = 1u << 31;
+
 // A pointer read from device memory (a batch's job table) or moved between lanes is a generic
 // pointer to the compiler, and loads through it become flat loads, which count against both memory
 // wait counters and make every wait cover all outstanding LDS and scalar accesses too.  Every buffer
@@ -287,11 +294,16 @@ enum : int {
 };
 
 template <int STACK, bool STATS, bool WIDE, bool MULTI, bool HALF>
-__global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
+__global__ __launch_bounds__(256, STACK == kStackShort || STACK == kStackTiny ? 4 : 1) void k_render_lanes(RenderParams P) {
     if (!MULTI && recorded_tree_unfit(P)) return;
     if (MULTI && P.status && recorded_jobs_unfit(P)) return;
     __shared__ int s_stack[STACK * 256];
     int *stk = s_stack + threadIdx.x;
+    // The short class (binary16 walk only) holds fewer entries than the tree's height could need: a
+    // push that does not fit sets kOverflow in the lane's visit count, which ends the walk at its next
+    // step, and the query is then finished against every primitive (see kTrav below).
+    constexpr bool kShort = STACK == kStackShort || STACK == kStackTiny;
+    static_assert(!kShort || (WIDE && HALF), "the short stack class serves the binary16 walk");
     const unsigned long long lt = lanemask_lt();
     // A single frame's first kLdsLights lights, 16 words each (words 0-12 of the light's Shape, then
     // its area), copied once per workgroup: SampleLights then reads LDS instead of two dependent
@@ -391,6 +403,24 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
             if (last) break;
         }
         return false;
+    };
+    // A walk that outgrew the short stack is finished against every primitive, from the best hit found
+    // so far.  The closest hit is the least (t, then the higher id) whatever the order of the tests,
+    // and an occluder ends an any-hit query, so the result is the walk's.
+    auto finish_overflowed = [&]() {
+        for (int q = 0; q < t_nprims && !(qany && bid >= 0); q++) {
+            const Prim *tq = from_gmem(t_prims) + q;
+            int id;
+            bool last;
+            const float t = prim_isect_vals(tq->p0, tq->p1, tq->p2, ro, rd, &id, &last);
+            if (STATS) st_prims++;
+            if (qany) {
+                if (t != 0.f && t < tb && id != t_nshapes - 1) bid = 1;
+            } else if (t != 0.f && (t < tb || (t == tb && id > bid))) {
+                tb = t;
+                bid = id;
+            }
+        }
     };
     // start the lane's query along (ro, rd); tb = maxt for any-hit
     auto start = [&](bool any, float maxt) {
@@ -565,6 +595,8 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                                         if (pos < STACK) q_stk[pos * 256] = ref;
                                     }
                                     q_sp += nh - 1;
+                                    // a push past the stack (the same test on every lane of the quad)
+                                    if (kShort && q_sp > STACK) q_nv |= kOverflow;
                                     if (q_sp > STACK) q_sp = STACK;
                                     q_node = nr;
                                 } else if (q_sp == 0) {
@@ -592,7 +624,11 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                     bid = b_bid;
                     tb = b_tb;
                     nv = b_nv;
-                    if (b_end) st = qany ? kADone : kCDone;
+                    if (b_end) {
+                        // a walk that overflowed ends at its next quad step and is finished here
+                        if (kShort && (b_nv & kOverflow)) finish_overflowed();
+                        st = qany ? kADone : kCDone;
+                    }
                 }
                 continue;
             }
@@ -722,10 +758,15 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         choose(t1, r1, c1);
                         choose(t2, r2, c2);
                         choose(t3, r3, c3);
-                        if (t3 != inf && !c3 && sp < STACK) stk[(sp++) * 256] = r3;
-                        if (t2 != inf && !c2 && sp < STACK) stk[(sp++) * 256] = r2;
-                        if (t1 != inf && !c1 && sp < STACK) stk[(sp++) * 256] = r1;
-                        if (t0 != inf && !c0 && sp < STACK) stk[(sp++) * 256] = r0;
+                        auto push = [&](float t, bool c, int r) {
+                            if (t == inf || c) return;
+                            if (sp < STACK) stk[(sp++) * 256] = r;
+                            else if (kShort) nv |= kOverflow;
+                        };
+                        push(t3, c3, r3);
+                        push(t2, c2, r2);
+                        push(t1, c1, r1);
+                        push(t0, c0, r0);
                     }
                     node = nn;
                     if (!done) {
@@ -746,6 +787,7 @@ __global__ __launch_bounds__(256) void k_render_lanes(RenderParams P) {
                         if (node < 0 && pl < 0) done = true;   // the stack is empty
                     }
                 }
+                if (kShort && done && (nv & kOverflow)) finish_overflowed();
                 if (done) st = qany ? kADone : kCDone;
             }
             if (WIDE && !HALF && st == kTrav) {
@@ -1533,11 +1575,23 @@ int launch_render(ptgpu_ctx *ctx, const DevAccel &acc, const RenderLaunch &L) {
         const int sw = acc.nodes4 && ctx->wide ? stack_for_wide(acc, ctx->stack_min) : -1;
         const bool half = ctx->wide == 2 && acc.nodes4h;
         if (sw > 0) P.depth_cap = wide_depth_cap(sw);
+        // the binary16 walk of a tree in the 48-entry class runs on the short stack, four workgroups
+        // per CU (short_stack 2, a test setting, runs every binary16 walk on the 8-entry class).  Held
+        // to 128 registers, the short class keeps one in scratch memory, which the allocation-free
+        // entry does not ask the runtime for.
+        const bool tiny = half && sw > 0 && ctx->short_stack == 2 && !L.no_alloc;
+        const bool shortst = half && sw == 48 && ctx->short_stack == 1 && !L.no_alloc;
 #define PT_LANES(ST, STATS_, WIDE_, HALF_)                                                                      \
     hipLaunchKernelGGL((k_render_lanes<ST, STATS_, WIDE_, false, HALF_>), dim3(blocks), dim3(256), 0, ctx->stream, P)
 #define PT_LANES_W(ST, STATS_) \
     do { if (half) PT_LANES(ST, STATS_, true, true); else PT_LANES(ST, STATS_, true, false); } while (0)
-        if (sw > 0) {
+        if (tiny) {
+            if (L.trav_stats) PT_LANES(kStackTiny, true, true, true);
+            else PT_LANES(kStackTiny, false, true, true);
+        } else if (shortst) {
+            if (L.trav_stats) PT_LANES(kStackShort, true, true, true);
+            else PT_LANES(kStackShort, false, true, true);
+        } else if (sw > 0) {
             if (L.trav_stats) {
                 if (sw == 32) PT_LANES_W(32, true);
                 else if (sw == 48) PT_LANES_W(48, true);
@@ -1837,6 +1891,7 @@ int ptgpu_radiance_gpu_noalloc(ptgpu_ctx *ctx, int accel, const ptgpu_radiance_a
     L.ray_counter = nullptr;
     L.trav_stats = nullptr;
     L.work_counter = counter;
+    L.no_alloc = true;
     return launch_render(ctx, acc, L);
 }
 

@@ -9,7 +9,9 @@ class, so the "stack" tuning raises the class and one frame is rendered by every
 * the per-lane-state megakernel over binary nodes (stacks 24, 64) and over four-wide fp32 and
   binary16 nodes (stacks 32, 48, 64), with and without statistics;
 * the batched instances (stacks 32, 48, 64; fp32 and binary16 nodes);
-* the ray queries (stacks 24, 64).
+* the ray queries (stacks 24, 64);
+* the 36-entry short class of the binary16 walk, and walks that outgrow a short stack (an 8-entry
+  test class), finished against every primitive.
 
 Each frame equals the oracle's frame (RadianceGPU over the reference's LBVH walk, rendering_kernel.cl:
 969-1150) bit for bit in colours and seeds, packed pixels within one step of powf.  The statistics
@@ -43,13 +45,15 @@ def frame(ctx):
     bvh.close()
 
 
-def _tuned(ctx, engine, wide, stack):
+def _tuned(ctx, engine, wide, stack, short_stack=1):
     ctx.set_engine(engine)
     ctx.set_tuning("wide", wide)
     ctx.set_tuning("stack", stack)
+    ctx.set_tuning("short_stack", short_stack)
 
 
 def _restore(ctx):
+    ctx.set_tuning("short_stack", 1)
     ctx.set_engine("auto")
     ctx.set_tuning("wide", 2)
     ctx.set_tuning("stack", 0)
@@ -68,27 +72,92 @@ def _render(ctx, f, stats):
             st.download(np.uint64) if stats else None)
 
 
-# (engine, wide, stack classes): stack 0 lets the height pick (the tree here fits the smallest class)
-GROUPS = [("megakernel", 0, (0, 64)), ("lanes", 0, (0, 64)), ("lanes", 1, (0, 48, 64)), ("lanes", 2, (0, 48, 64))]
+# (engine, wide, (stack, short_stack) pairs): stack 0 lets the height pick (the tree here fits the
+# smallest class); with wide 2, stack 48 runs the 36-entry short class unless short_stack is 0
+GROUPS = [("megakernel", 0, ((0, 1), (64, 1))), ("lanes", 0, ((0, 1), (64, 1))),
+          ("lanes", 1, ((0, 1), (48, 1), (64, 1))), ("lanes", 2, ((0, 1), (48, 0), (48, 1), (64, 1)))]
 
 
 @pytest.mark.parametrize("engine,wide,stacks", GROUPS, ids=[f"{e}-wide{w}" for e, w, _ in GROUPS])
 def test_every_render_instance_vs_oracle(ctx, frame, engine, wide, stacks):
     counts = {}
     try:
-        for stack in stacks:
-            _tuned(ctx, engine, wide, stack)
+        for stack, short in stacks:
+            _tuned(ctx, engine, wide, stack, short)
             for stats in (False, True):
-                say("case variant", engine, "wide", wide, "stack", stack, "stats", stats)
+                say("case variant", engine, "wide", wide, "stack", stack, "short", short, "stats", stats)
                 c, s, p, st = _render(ctx, frame, stats)
                 image_checks(c, s, p, *frame["ref"])
                 if stats:
-                    counts[stack] = tuple(int(v) for v in st[:2])
+                    counts[(stack, short)] = tuple(int(v) for v in st[:2])
     finally:
         _restore(ctx)
     # one walk whatever the stack class: the same box and primitive test counts
     assert len(set(counts.values())) == 1, counts
-    assert counts[0][0] > 0 and counts[0][1] > 0
+    assert counts[(0, 1)][0] > 0 and counts[(0, 1)][1] > 0
+
+
+def test_short_stack_overflow_vs_oracle(ctx):
+    """Walks that outgrow the stack (tuning short_stack 2: every binary16 walk on an 8-entry stack) are
+    finished against every primitive: on a dense 20,000-triangle soup the frame equals the oracle's,
+    with the quad phase on and off, and the primitive tests show the finished walks."""
+    w, h, spp = 32, 24, 1
+    shapes, lc, cam = scenes.soup(20000, seed=12, width=w, height=h)
+    pt.update_camera(cam)
+    s, bvh = gpu_lbvh(ctx, shapes)
+    _, _, btn, btl = O.build_lbvh(shapes)
+    ref = render_oracle(s, lc, cam, w, h, spp, DEPTH, O.accel_bvh(btn, btl))
+    d_sh, d_cam = ctx.to_device(s), ctx.to_device(cam)
+    counts = {}
+    try:
+        for quad in (16, 0):
+            for short in (1, 2):
+                _tuned(ctx, "auto", 2, 0, short)
+                ctx.set_tuning("quad", quad)
+                say("case short stack overflow, short_stack", short, "quad", quad)
+                col = ctx.alloc(w * h * 12, F32).zero()
+                seeds = ctx.to_device(default_seeds(w, h))
+                pix = ctx.alloc(w * h * 4, np.uint32)
+                st = ctx.alloc(16, np.uint64).zero()
+                pt.render(ctx, bvh, None, colors=col, seeds=seeds, camera=d_cam, shapes=d_sh, shapeCnt=len(s),
+                          lightCnt=lc, width=w, height=h, currentSample=0, spp=spp, max_depth=DEPTH, pixels=pix,
+                          trav_stats=st)
+                ctx.synchronize()
+                image_checks(col.download(F32), seeds.download(np.uint32), pix.download(np.uint32), *ref)
+                counts[(quad, short)] = int(st.download(np.uint64)[1])
+    finally:
+        ctx.set_tuning("quad", 16)
+        _restore(ctx)
+        bvh.close()
+    for quad in (16, 0):
+        assert counts[(quad, 2)] > counts[(quad, 1)] + 10 * len(s), counts
+
+
+def test_short_stack_deep_walks(ctx):
+    """The 65,536-triangle soup after a size-5 treelet pass (its tree needs the 48-entry class): a 96x64
+    frame bit for bit the same on the 36-entry short class and on the whole 48-entry stack."""
+    shapes, lc, cam = scenes.soup(65536, seed=1, width=96, height=64)
+    pt.update_camera(cam)
+    s, bvh = gpu_lbvh(ctx, shapes)
+    bvh.optimize(5)
+    d_sh, d_cam = ctx.to_device(s), ctx.to_device(cam)
+    out = {}
+    try:
+        for short in (0, 1):
+            _tuned(ctx, "auto", 2, 0, short)
+            say("case short stack deep walks, short_stack", short)
+            col = ctx.alloc(96 * 64 * 12, F32).zero()
+            seeds = ctx.to_device(default_seeds(96, 64))
+            pix = ctx.alloc(96 * 64 * 4, np.uint32)
+            pt.render(ctx, bvh, None, colors=col, seeds=seeds, camera=d_cam, shapes=d_sh, shapeCnt=len(s),
+                      lightCnt=lc, width=96, height=64, currentSample=0, spp=2, max_depth=DEPTH, pixels=pix)
+            ctx.synchronize()
+            out[short] = (col.download(F32).tobytes(), seeds.download(np.uint32).tobytes(),
+                          pix.download(np.uint32).tobytes())
+    finally:
+        _restore(ctx)
+        bvh.close()
+    assert out[0] == out[1]
 
 
 @pytest.mark.parametrize("wide", [1, 2])
@@ -143,3 +212,7 @@ def test_stack_tuning_range(ctx):
         with pytest.raises(pt.PtgpuError, match="stack"):
             ctx.set_tuning("stack", bad)
     ctx.set_tuning("stack", 0)
+    for bad in (-1, 3):
+        with pytest.raises(pt.PtgpuError, match="short_stack"):
+            ctx.set_tuning("short_stack", bad)
+    ctx.set_tuning("short_stack", 1)
