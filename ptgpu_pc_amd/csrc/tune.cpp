@@ -4,8 +4,9 @@
 // for every value (the tests check each one).  The tuner times real work on the caller's own scene
 // and frame with HIP events on the context's stream and keeps the fastest setting:
 //   * LBVH sort digit width (4 or 8 bits): full rebuild time;
-//   * single-frame render: workgroups per CU (1..4), shading-pass threshold `regroup` (8, 16, 32),
-//     node format (four-wide binary16, four-wide fp32 or binary);
+//   * single-frame render: workgroups per CU (1..4; 1..2 when frames_per_build is 0, see below),
+//     shading-pass threshold `regroup` (8, 16, 32), node format (four-wide binary16, four-wide fp32 or
+//     binary);
 //   * treelet restructuring (none, 5 or 7 leaves): build + optimise time plus `frames_per_build`
 //     renders on the resulting tree, the cost a caller pays per rebuild.  frames_per_build 0 means
 //     each rebuild runs on another stream beside the previous frame's render: the cost is the render
@@ -143,8 +144,12 @@ int autotune_frame(ptgpu_ctx *ctx, ptgpu_clbvh *h, const ptgpu_render_args *a, i
     PT_TRY(render());   // first use after a build: derived views and light list
     float best_render = 1e30f;
     int bb = ctx->blocks_per_cu, br = ctx->regroup, bw = ctx->wide;
+    // With frames in flight (frames_per_build 0) the next frame's render starts while this one drains:
+    // a launch that asks for more than half of a CU's workgroups leaves it no room (measured: three or
+    // four per CU render one frame alone as fast, but frames in flight 2-5% slower at C2).
+    const int bpc_max = frames_per_build == 0 ? 2 : 4;
     for (int wide : {2, 1, 0})
-        for (int bpc : {1, 2, 3, 4})
+        for (int bpc = 1; bpc <= bpc_max; bpc++)
             for (int rg : {8, 16, 32}) {
                 ctx->wide = wide;
                 ctx->blocks_per_cu = bpc;
